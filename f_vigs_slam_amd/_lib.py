@@ -1,0 +1,115 @@
+"""ctypes binding of libfvgs.so (the C ABI declared in include/fvgs/fvgs.h).
+
+The product path is the HIP library: nothing here computes anything on the CPU. Loading fails
+loudly when libfvgs.so has not been built (``python -m f_vigs_slam_amd.build``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfvgs.so")
+
+FVGS_OK = 0
+FVGS_E_ARG = -1
+FVGS_E_EMPTY = -2
+FVGS_E_OOM = -3
+FVGS_E_HIP = -4
+FVGS_E_NOTPD = -5
+FVGS_E_STATE = -6
+MEM_HOST = 0
+MEM_DEVICE = 1
+
+OUT = dict(POS2D=1, COV2D=2, CONIC=3, DEPTH_G=4, PHAT=5, KEYS=6, INDICES=7, RANGES=8,
+           RENDER_RGB=10, RENDER_DEPTH=11, RENDER_ALPHA=12, RGB=20, DEPTH=21, FINAL_T=22,
+           N_CONTRIB=23, COLOR_ERROR=24, MAX_CONTRIB=25, DELTA2D=30, DELTA3D=31,
+           POSE_TILES=40, TRACK_RGB=41, TRACK_DEPTH=42)
+
+# Every function the header declares (checked by the CPU test suite against the library).
+EXPORTS = [
+    "fvgs_default_params", "fvgs_create", "fvgs_destroy", "fvgs_last_error", "fvgs_abi_version",
+    "fvgs_set_gaussians", "fvgs_gaussian_count", "fvgs_set_frame", "fvgs_prepare_rasterization",
+    "fvgs_render", "fvgs_forward", "fvgs_backward", "fvgs_gradients_3d", "fvgs_mapping_step",
+    "fvgs_pose_jacobians", "fvgs_render_tracking", "fvgs_compute_gradients", "fvgs_gn_step", "fvgs_track", "fvgs_read", "fvgs_device_ptr",
+    "fvgs_sync", "fvgs_stats", "fvgs_set_timing", "fvgs_last_timing",
+]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("bg", ctypes.c_float * 3),
+                ("alpha_thresh", ctypes.c_float), ("color_thresh", ctypes.c_float),
+                ("depth_thresh", ctypes.c_float), ("depth_weight", ctypes.c_float),
+                ("w_depth", ctypes.c_float), ("w_dist", ctypes.c_float),
+                ("lambda_iso", ctypes.c_float), ("cull_emit", ctypes.c_int),
+                ("keep_keys", ctypes.c_int), ("mem_budget", ctypes.c_size_t)]
+
+
+class Pose(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_float * 3), ("orientation", ctypes.c_float * 4)]
+
+
+class Intrinsics(ctypes.Structure):
+    _fields_ = [("f", ctypes.c_float * 2), ("c", ctypes.c_float * 2)]
+
+
+DELTA2D_DTYPE = np.dtype([("meanImg", "<f4", 2), ("invSigmaImg", "<f4", 3), ("color", "<f4", 3),
+                          ("depth", "<f4"), ("alpha", "<f4"), ("pHat", "<f4", 2), ("n", "<u4")])
+DELTA3D_DTYPE = np.dtype([("position", "<f4", 3), ("scale", "<f4", 3),
+                          ("orientation", "<f4", 3), ("color", "<f4", 3), ("alpha", "<f4"),
+                          ("n", "<i4")])
+assert DELTA2D_DTYPE.itemsize == 52 and DELTA3D_DTYPE.itemsize == 56
+
+_lib = None
+
+
+def load():
+    """Load libfvgs.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m f_vigs_slam_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, i, u32, f, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_float, ctypes.c_double
+    P = ctypes.POINTER
+    sig = {
+        "fvgs_default_params": ([P(Params)], i),
+        "fvgs_create": ([P(Params), P(vp)], i),
+        "fvgs_destroy": ([vp], i),
+        "fvgs_last_error": ([vp], ctypes.c_char_p),
+        "fvgs_abi_version": ([], i),
+        "fvgs_set_gaussians": ([vp, u32, vp, vp, vp, vp, vp, i], i),
+        "fvgs_gaussian_count": ([vp, P(u32)], i),
+        "fvgs_set_frame": ([vp, i, i, vp, vp, vp, vp, i], i),
+        "fvgs_prepare_rasterization": ([vp, P(Pose), P(Intrinsics), i, i, P(u32)], i),
+        "fvgs_render": ([vp], i),
+        "fvgs_forward": ([vp], i),
+        "fvgs_backward": ([vp], i),
+        "fvgs_gradients_3d": ([vp], i),
+        "fvgs_mapping_step": ([vp, P(Pose), P(Intrinsics)], i),
+        "fvgs_pose_jacobians": ([vp, P(Pose), P(Intrinsics), vp, vp], i),
+        "fvgs_render_tracking": ([vp, P(Pose), P(Intrinsics), i, i], i),
+        "fvgs_compute_gradients": ([vp], i),
+        "fvgs_gn_step": ([vp, vp, vp, vp], i),
+        "fvgs_track": ([vp, P(Intrinsics), i, vp, vp], i),
+        "fvgs_read": ([vp, i, vp, ctypes.c_size_t], i),
+        "fvgs_device_ptr": ([vp, i, P(vp), P(ctypes.c_size_t)], i),
+        "fvgs_sync": ([vp], i),
+        "fvgs_stats": ([vp, P(ctypes.c_uint64), P(ctypes.c_uint64), P(u32), P(u32)], i),
+        "fvgs_set_timing": ([vp, i], i),
+        "fvgs_last_timing": ([vp, P(f)], i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def ptr(a):
+    """Address of a numpy array's data (host memory)."""
+    return ctypes.c_void_p(a.ctypes.data)

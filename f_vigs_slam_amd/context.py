@@ -1,0 +1,232 @@
+"""Python handle over one libfvgs context (one HIP stream, context-owned HBM).
+
+Mirrors the hot methods of the reference's GSSlam (GSSlam.cuh:25-176) at the level of the C ABI:
+``prepare_rasterization`` / ``render`` / ``forward`` / ``backward`` / ``gradients_3d`` /
+``mapping_step`` / ``pose_jacobians`` / ``track``. Errors raise :class:`FvgsError` carrying the
+library's status code and message.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+class FvgsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fvgs error {code}: {msg}")
+        self.code = code
+
+
+def default_params(**overrides) -> L.Params:
+    p = L.Params()
+    L.load().fvgs_default_params(ctypes.byref(p))
+    for k, v in overrides.items():
+        if k == "bg":
+            for j in range(3):
+                p.bg[j] = float(v[j])
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def make_pose(pose) -> L.Pose:
+    """pose7 (px, py, pz, qx, qy, qz, qw) -> fvgs_pose (orientation read (x, y, z, w))."""
+    pz = np.asarray(pose, np.float64).ravel()
+    p = L.Pose()
+    for k in range(3):
+        p.position[k] = float(pz[k])
+    for k in range(4):
+        p.orientation[k] = float(pz[3 + k])
+    return p
+
+
+def make_intr(intr) -> L.Intrinsics:
+    it = np.asarray(intr, np.float64).ravel()
+    r = L.Intrinsics()
+    r.f[0], r.f[1], r.c[0], r.c[1] = (float(x) for x in it[:4])
+    return r
+
+
+def gn_step(jtj21, jtr6, pose7):
+    """FP64 Cholesky Gauss-Newton step + Plus (fvgs_gn_step). Returns (new_pose7, delta6)."""
+    jtj = np.ascontiguousarray(jtj21, np.float64)
+    jtr = np.ascontiguousarray(jtr6, np.float64)
+    pose = np.ascontiguousarray(pose7, np.float64).copy()
+    d = np.zeros(6)
+    rc = L.load().fvgs_gn_step(L.ptr(jtj), L.ptr(jtr), L.ptr(pose), L.ptr(d))
+    if rc != L.FVGS_OK:
+        raise FvgsError(rc, "JtJ is not positive definite")
+    return pose, d
+
+
+class Context:
+    def __init__(self, **params):
+        self._lib = L.load()
+        self._h = ctypes.c_void_p()
+        self.params = default_params(**params)
+        rc = self._lib.fvgs_create(ctypes.byref(self.params), ctypes.byref(self._h))
+        if rc != L.FVGS_OK:
+            raise FvgsError(rc, "fvgs_create failed (no HIP device?)")
+        self.n = 0
+        self.view = None
+        self._keep = []   # host arrays referenced by the last upload (kept alive)
+
+    # -- plumbing ------------------------------------------------------------------------
+    def _check(self, rc):
+        if rc != L.FVGS_OK:
+            raise FvgsError(rc, self._lib.fvgs_last_error(self._h).decode())
+        return rc
+
+    def close(self):
+        if self._h:
+            self._lib.fvgs_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @staticmethod
+    def _arg(a, dtype=np.float32):
+        """(pointer, mem kind, keep-alive) for a numpy array or a device tensor."""
+        if a is None:
+            return None, L.MEM_HOST, None
+        if hasattr(a, "data_ptr") and getattr(a, "is_cuda", False):
+            return ctypes.c_void_p(a.data_ptr()), L.MEM_DEVICE, a
+        arr = np.ascontiguousarray(a, dtype)
+        return L.ptr(arr), L.MEM_HOST, arr
+
+    # -- map and frame -------------------------------------------------------------------
+    def set_gaussians(self, pos, scale, quat, color, opacity):
+        args = [self._arg(x) for x in (pos, scale, quat, color, opacity)]
+        mems = {m for _, m, _ in args}
+        if len(mems) != 1:
+            raise ValueError("all Gaussian arrays must live in the same memory")
+        n = len(pos)
+        self._check(self._lib.fvgs_set_gaussians(self._h, n, *[p for p, _, _ in args],
+                                                 mems.pop()))
+        self._keep_map = [k for _, _, k in args]
+        self.n = n
+
+    def set_scene(self, scene):
+        self.set_gaussians(scene["pos"], scene["scale"], scene["quat"], scene["color"],
+                           scene["opacity"])
+
+    def set_frame(self, rgb, depth, gx=None, gy=None):
+        h, w = np.shape(depth)[:2]
+        args = [self._arg(x) for x in (rgb, depth, gx, gy)]
+        mem = args[0][1]
+        self._check(self._lib.fvgs_set_frame(self._h, w, h, *[p for p, _, _ in args], mem))
+        self._keep_frame = [k for _, _, k in args]
+        self.frame_size = (w, h)
+
+    # -- compute -------------------------------------------------------------------------
+    def prepare_rasterization(self, pose7, intr, width, height):
+        m = ctypes.c_uint32(0)
+        self._check(self._lib.fvgs_prepare_rasterization(
+            self._h, ctypes.byref(make_pose(pose7)), ctypes.byref(make_intr(intr)), width,
+            height, ctypes.byref(m)))
+        self.view = (width, height)
+        return int(m.value)
+
+    def render(self):
+        self._check(self._lib.fvgs_render(self._h))
+
+    def forward(self):
+        self._check(self._lib.fvgs_forward(self._h))
+
+    def backward(self):
+        self._check(self._lib.fvgs_backward(self._h))
+
+    def gradients_3d(self):
+        self._check(self._lib.fvgs_gradients_3d(self._h))
+
+    def mapping_step(self, pose7, intr):
+        self._check(self._lib.fvgs_mapping_step(self._h, ctypes.byref(make_pose(pose7)),
+                                                ctypes.byref(make_intr(intr))))
+        self.view = self.frame_size
+
+    def pose_jacobians(self, pose7, intr):
+        jtj = np.zeros(21)
+        jtr = np.zeros(6)
+        self._check(self._lib.fvgs_pose_jacobians(self._h, ctypes.byref(make_pose(pose7)),
+                                                  ctypes.byref(make_intr(intr)), L.ptr(jtj),
+                                                  L.ptr(jtr)))
+        self.view = self.frame_size
+        return jtj, jtr
+
+    def render_tracking(self, pose7, intr, width, height):
+        """Pass-1 self-render (all pixels valid) into TRACK_RGB / TRACK_DEPTH."""
+        self._check(self._lib.fvgs_render_tracking(self._h, ctypes.byref(make_pose(pose7)),
+                                                   ctypes.byref(make_intr(intr)), width, height))
+        self.view = (width, height)
+
+    def compute_gradients(self):
+        """Device Sobel gradients of the current frame's colour, attached to the frame."""
+        self._check(self._lib.fvgs_compute_gradients(self._h))
+
+    def track(self, intr, iters, pose7):
+        pose = np.ascontiguousarray(pose7, np.float64).copy()
+        ms = np.zeros(max(1, iters))
+        self._check(self._lib.fvgs_track(self._h, ctypes.byref(make_intr(intr)), iters,
+                                         L.ptr(pose), L.ptr(ms)))
+        self.view = self.frame_size
+        return pose, ms[:iters]
+
+    def sync(self):
+        self._check(self._lib.fvgs_sync(self._h))
+
+    def set_timing(self, on=True):
+        self._check(self._lib.fvgs_set_timing(self._h, 1 if on else 0))
+
+    def last_timing(self):
+        ms = (ctypes.c_float * 4)()
+        self._check(self._lib.fvgs_last_timing(self._h, ms))
+        return list(ms)
+
+    def stats(self):
+        m, k = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        t, p = ctypes.c_uint32(0), ctypes.c_uint32(0)
+        self._check(self._lib.fvgs_stats(self._h, ctypes.byref(m), ctypes.byref(k),
+                                         ctypes.byref(t), ctypes.byref(p)))
+        return dict(M=int(m.value), K=int(k.value), tiles=int(t.value), pixels=int(p.value))
+
+    def device_ptr(self, what):
+        p, b = ctypes.c_void_p(), ctypes.c_size_t(0)
+        self._check(self._lib.fvgs_device_ptr(self._h, L.OUT[what], ctypes.byref(p),
+                                              ctypes.byref(b)))
+        return int(p.value or 0), int(b.value)
+
+    # -- results -------------------------------------------------------------------------
+    def read(self, what, count=None):
+        w, h = self.view if self.view else (0, 0)
+        P = w * h
+        T = ((w + 15) // 16) * ((h + 15) // 16)
+        n = self.n
+        m = self.stats()["M"] if what in ("KEYS", "INDICES") else 0
+        shapes = dict(POS2D=((n, 2), np.float32), COV2D=((n, 3), np.float32),
+                      CONIC=((n, 3), np.float32), DEPTH_G=((n,), np.float32),
+                      PHAT=((n, 2), np.float32), KEYS=((m,), np.uint64),
+                      INDICES=((m,), np.uint32), RANGES=((T, 2), np.uint32),
+                      RENDER_RGB=((h, w, 3), np.float32), RENDER_DEPTH=((h, w), np.float32),
+                      RENDER_ALPHA=((h, w), np.float32), RGB=((h, w, 3), np.float32),
+                      DEPTH=((h, w), np.float32), FINAL_T=((h, w), np.float32),
+                      N_CONTRIB=((h, w), np.uint32), COLOR_ERROR=((h, w, 3), np.float32),
+                      MAX_CONTRIB=((T,), np.uint32), DELTA2D=((n,), L.DELTA2D_DTYPE),
+                      DELTA3D=((n,), L.DELTA3D_DTYPE), POSE_TILES=((T, 27), np.float32),
+                      TRACK_RGB=((h, w, 3), np.float32), TRACK_DEPTH=((h, w), np.float32))
+        shape, dt = shapes[what]
+        out = np.zeros(shape, dt)
+        self._check(self._lib.fvgs_read(self._h, L.OUT[what], L.ptr(out), out.nbytes))
+        return out

@@ -1,0 +1,606 @@
+// Projection and (tile, depth) binning for gfx950.
+//
+// Replaces GSSlam::prepareRasterization (GSSlam.cu:408-646) and the kernels it launches:
+// projectGaussiansWorldToScreen_kernel (GSCudaKernels.cu:345-476), countTilesPerGaussian_kernel
+// (549-593), thrust::exclusive_scan, generateTileHashes_kernel (596-667), thrust::sort_by_key
+// and computeIndicesRanges_kernel (669-707).
+//
+// Design (one GPU, HBM resident):
+//   k_project    one thread per Gaussian, 1024 Gaussians per block: screen record (64 B),
+//                tile rectangle, and a per-block tile histogram in LDS written as one row of
+//                the [blocks x tiles] count matrix (no global atomics).
+//   k_colscan1/2 column prefix of the count matrix in chunks of 64 rows, then the tile
+//                totals' exclusive scan: tile ranges and the instance count M.
+//   k_emit       per block, LDS cursors at the block's base offset in every tile; each instance
+//                writes (depth bits << 32 | gaussian index) into its tile's segment.
+//   k_tile_sort  one workgroup per tile: LSD radix sort of the unique key
+//                ((depth bits - tile min) << index bits | index), 8-bit digits, stable
+//                wave-level ranks from ballots. Segments up to NT*E entries stay in LDS;
+//                longer ones stream through a global ping-pong buffer.
+// The resulting order equals the reference's stable sort of (tile << 32 | depth bits) with
+// emission in Gaussian-index order: ties on the depth bits resolve by ascending index.
+//
+// Bit-exactness: everything that feeds keys and tile rectangles is evaluated in the
+// reference's operation order with contraction off and IEEE division/sqrt, so the tile
+// lists equal the oracle's FP32 binning mode.
+#include "fvgs_internal.h"
+
+namespace fvgs {
+
+namespace {
+
+__device__ __forceinline__ void quat_to_mat_exact(float x, float y, float z, float w,
+                                                  float R[3][3]) {
+#pragma clang fp contract(off)
+    // GSCudaKernels.cu:319-334, float4 read as (x, y, z, w).
+    R[0][0] = 1.0f - 2.0f * (y * y + z * z);
+    R[0][1] = 2.0f * (x * y - w * z);
+    R[0][2] = 2.0f * (x * z + w * y);
+    R[1][0] = 2.0f * (x * y + w * z);
+    R[1][1] = 1.0f - 2.0f * (x * x + z * z);
+    R[1][2] = 2.0f * (y * z - w * x);
+    R[2][0] = 2.0f * (x * z - w * y);
+    R[2][1] = 2.0f * (y * z + w * x);
+    R[2][2] = 1.0f - 2.0f * (x * x + y * y);
+}
+
+// GSCudaKernels.cu:345-476 for one Gaussian. Returns false when culled (z <= 0 or centre
+// outside [0,W)x[0,H)); the caller keeps the zero record (Q1).
+__device__ __forceinline__ bool project_exact(const float* __restrict__ pw,
+                                              const float* __restrict__ sc,
+                                              const float* __restrict__ qg, const View& v,
+                                              const float Rc[3][3], float& u, float& vv,
+                                              float& zc, float cov[3], float con[3],
+                                              float ph[2]) {
+#pragma clang fp contract(off)
+    float d0 = pw[0] - v.px, d1 = pw[1] - v.py, d2 = pw[2] - v.pz;
+    float xc = Rc[0][0] * d0 + Rc[1][0] * d1 + Rc[2][0] * d2;
+    float yc = Rc[0][1] * d0 + Rc[1][1] * d1 + Rc[2][1] * d2;
+    zc = Rc[0][2] * d0 + Rc[1][2] * d1 + Rc[2][2] * d2;
+    if (zc <= 0.0f) return false;
+    float inv_z = 1.0f / zc;
+    u = v.fx * (xc * inv_z) + v.cx;
+    vv = v.fy * (yc * inv_z) + v.cy;
+    if (u < 0.0f || u >= (float)v.W || vv < 0.0f || vv >= (float)v.H) return false;
+
+    float Rg[3][3];
+    quat_to_mat_exact(qg[0], qg[1], qg[2], qg[3], Rg);
+    float M[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        M[i][0] = Rg[i][0] * sc[0];
+        M[i][1] = Rg[i][1] * sc[1];
+        M[i][2] = Rg[i][2] * sc[2];
+    }
+    float S[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = i; j < 3; j++) {
+            S[i][j] = M[i][0] * M[j][0] + M[i][1] * M[j][1] + M[i][2] * M[j][2];
+            if (i != j) S[j][i] = S[i][j];
+        }
+    float Sc[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int l = 0; l < 3; l++) acc += Rc[k][i] * S[k][l] * Rc[l][j];
+            Sc[i][j] = acc;
+        }
+    float J[2][3];
+    J[0][0] = v.fx * inv_z; J[0][1] = 0.0f; J[0][2] = -v.fx * xc * inv_z * inv_z;
+    J[1][0] = 0.0f; J[1][1] = v.fy * inv_z; J[1][2] = -v.fy * yc * inv_z * inv_z;
+    float S2[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = i; j < 2; j++) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int l = 0; l < 3; l++) acc += J[i][k] * Sc[k][l] * J[j][l];
+            S2[i][j] = acc;
+        }
+    cov[0] = S2[0][0]; cov[1] = S2[1][1]; cov[2] = S2[0][1];
+    // invert2x2 (GSCudaKernels.cu:1411-1437): det == 0 -> identity.
+    float det = S2[0][0] * S2[1][1] - S2[0][1] * S2[0][1];
+    if (det == 0.0f) {
+        con[0] = 1.0f; con[1] = 1.0f; con[2] = 0.0f;
+    } else {
+        float inv_det = 1.0f / det;
+        con[0] = S2[1][1] * inv_det;
+        con[1] = S2[0][0] * inv_det;
+        con[2] = -S2[0][1] * inv_det;
+    }
+    ph[0] = xc * inv_z * inv_z;
+    ph[1] = yc * inv_z * inv_z;
+    return true;
+}
+
+// 3-sigma tile rectangle shared by the count and hash kernels (GSCudaKernels.cu:566-591).
+__device__ __forceinline__ uint2 tile_rect_exact(float u, float v, float cxx, float cyy,
+                                                 const View& vw) {
+#pragma clang fp contract(off)
+    float rx = 3.0f * sqrtf(cxx);
+    float ry = 3.0f * sqrtf(cyy);
+    float bminx = fmaxf(0.0f, u - rx);
+    float bmaxx = fminf((float)(vw.W - 1), u + rx);
+    float bminy = fmaxf(0.0f, v - ry);
+    float bmaxy = fminf((float)(vw.H - 1), v + ry);
+    int x0 = (int)(bminx / (float)kTile), x1 = (int)(bmaxx / (float)kTile);
+    int y0 = (int)(bminy / (float)kTile), y1 = (int)(bmaxy / (float)kTile);
+    x0 = max(0, min(x0, vw.ntx - 1));
+    x1 = max(0, min(x1, vw.ntx - 1));
+    y0 = max(0, min(y0, vw.nty - 1));
+    y1 = max(0, min(y1, vw.nty - 1));
+    return make_uint2((uint32_t)x0 | ((uint32_t)x1 << 16), (uint32_t)y0 | ((uint32_t)y1 << 16));
+}
+
+// Depth at the tile centre (GSCudaKernels.cu:643-654), raw float bits.
+__device__ __forceinline__ uint32_t tile_depth_bits(float u, float v, float z, float phx,
+                                                    float phy, int tx, int ty) {
+#pragma clang fp contract(off)
+    float tcx = ((float)tx + 0.5f) * (float)kTile;
+    float tcy = ((float)ty + 0.5f) * (float)kTile;
+    float dx = tcx - u;
+    float dy = tcy - v;
+    float d = z + phx * dx + phy * dy;
+    return __float_as_uint(d);
+}
+
+__global__ __launch_bounds__(kBinThreads) void k_project(
+    const float* __restrict__ pos, const float* __restrict__ scale,
+    const float* __restrict__ quat, const float* __restrict__ color,
+    const float* __restrict__ opac, int n, View v, int cull_emit, Rec* __restrict__ rec,
+    uint2* __restrict__ rect, uint32_t* __restrict__ blk_counts) {
+    extern __shared__ uint32_t hist[];
+    const int T = v.ntx * v.nty;
+    for (int t = threadIdx.x; t < T; t += kBinThreads) hist[t] = 0;
+    __syncthreads();
+    float Rc[3][3];
+    quat_to_mat_exact(v.qx, v.qy, v.qz, v.qw, Rc);
+    for (int it = 0; it < kBinItems; ++it) {
+        int g = blockIdx.x * kBinChunk + it * kBinThreads + threadIdx.x;
+        if (g >= n) break;
+        float u = 0.f, vv = 0.f, z = 0.f, cov[3] = {0.f, 0.f, 0.f}, con[3] = {0.f, 0.f, 0.f},
+              ph[2] = {0.f, 0.f};
+        bool vis = project_exact(pos + 3 * (size_t)g, scale + 3 * (size_t)g,
+                                 quat + 4 * (size_t)g, v, Rc, u, vv, z, cov, con, ph);
+        if (!vis) {
+            u = vv = z = 0.f;
+            cov[0] = cov[1] = cov[2] = 0.f;
+            con[0] = con[1] = con[2] = 0.f;
+            ph[0] = ph[1] = 0.f;
+        }
+        Rec r;
+        r.r0 = make_float4(u, vv, con[0], con[1]);
+        r.r1 = make_float4(con[2], z, ph[0], ph[1]);
+        r.r2 = make_float4(color[3 * (size_t)g], color[3 * (size_t)g + 1],
+                           color[3 * (size_t)g + 2], opac[g]);
+        r.r3 = make_float4(cov[0], cov[1], cov[2], 0.f);
+        rec[g] = r;
+        uint2 rc;
+        if (vis || cull_emit) {
+            rc = tile_rect_exact(u, vv, cov[0], cov[1], v);
+        } else {
+            rc = make_uint2(1u, 0u);  // empty rectangle: x0 = 1 > x1 = 0
+        }
+        rect[g] = rc;
+        int x0 = rc.x & 0xffff, x1 = rc.x >> 16, y0 = rc.y & 0xffff, y1 = rc.y >> 16;
+        for (int ty = y0; ty <= y1; ++ty)
+            for (int tx = x0; tx <= x1; ++tx) atomicAdd(&hist[ty * v.ntx + tx], 1u);
+    }
+    __syncthreads();
+    uint32_t* row = blk_counts + (size_t)blockIdx.x * T;
+    for (int t = threadIdx.x; t < T; t += kBinThreads) row[t] = hist[t];
+}
+
+// Column prefix in chunks of kScanRows rows: rows become within-chunk exclusive prefixes,
+// chunk_pre[c][t] receives the chunk's column sum.
+__global__ __launch_bounds__(256) void k_colscan1(uint32_t* __restrict__ blk_counts,
+                                                  uint32_t* __restrict__ chunk_pre, int nblk,
+                                                  int T) {
+    int t = blockIdx.x * 256 + threadIdx.x;
+    int c = blockIdx.y;
+    if (t >= T) return;
+    int b0 = c * kScanRows, b1 = min(nblk, b0 + kScanRows);
+    uint32_t run = 0;
+    for (int b = b0; b < b1; ++b) {
+        uint32_t x = blk_counts[(size_t)b * T + t];
+        blk_counts[(size_t)b * T + t] = run;
+        run += x;
+    }
+    chunk_pre[(size_t)c * T + t] = run;
+}
+
+// Tile totals, their exclusive scan (ranges), and chunk bases folded with the tile offset.
+// One block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_colscan2(uint32_t* __restrict__ chunk_pre, int nchunk,
+                                                   int T, uint32_t* __restrict__ ranges,
+                                                   uint32_t* __restrict__ d_m) {
+    extern __shared__ uint32_t tot[];   // T entries, then 1024 + 32 scratch
+    uint32_t* part = tot + T;
+    const int tid = threadIdx.x;
+    for (int t = tid; t < T; t += 1024) {
+        uint32_t run = 0;
+        for (int c = 0; c < nchunk; ++c) {
+            uint32_t x = chunk_pre[(size_t)c * T + t];
+            chunk_pre[(size_t)c * T + t] = run;
+            run += x;
+        }
+        tot[t] = run;
+    }
+    __syncthreads();
+    // Block exclusive scan of tot[0..T): contiguous segment per thread.
+    int per = (T + 1023) / 1024;
+    int a = tid * per, b = min(T, a + per);
+    uint32_t sum = 0;
+    for (int i = a; i < b; ++i) sum += tot[i];
+    part[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t x = (tid >= off) ? part[tid - off] : 0u;
+        __syncthreads();
+        part[tid] += x;
+        __syncthreads();
+    }
+    uint32_t run = part[tid] - sum;   // exclusive prefix of this thread's segment
+    for (int i = a; i < b; ++i) {
+        uint32_t x = tot[i];
+        ranges[2 * i] = run;
+        ranges[2 * i + 1] = run + x;
+        tot[i] = run;
+        run += x;
+    }
+    if (tid == 1023) *d_m = part[1023];
+    __syncthreads();
+    for (int t = tid; t < T; t += 1024) {
+        uint32_t base = tot[t];
+        for (int c = 0; c < nchunk; ++c) chunk_pre[(size_t)c * T + t] += base;
+    }
+}
+
+__global__ __launch_bounds__(kBinThreads) void k_emit(
+    const Rec* __restrict__ rec, const uint2* __restrict__ rect,
+    const uint32_t* __restrict__ blk_counts, const uint32_t* __restrict__ chunk_pre, int n,
+    View v, uint64_t* __restrict__ pairs, uint64_t cap) {
+    extern __shared__ uint32_t cur[];
+    const int T = v.ntx * v.nty;
+    const uint32_t* row = blk_counts + (size_t)blockIdx.x * T;
+    const uint32_t* crow = chunk_pre + (size_t)(blockIdx.x / kScanRows) * T;
+    for (int t = threadIdx.x; t < T; t += kBinThreads) cur[t] = crow[t] + row[t];
+    __syncthreads();
+    for (int it = 0; it < kBinItems; ++it) {
+        int g = blockIdx.x * kBinChunk + it * kBinThreads + threadIdx.x;
+        if (g >= n) break;
+        uint2 rc = rect[g];
+        int x0 = rc.x & 0xffff, x1 = rc.x >> 16, y0 = rc.y & 0xffff, y1 = rc.y >> 16;
+        if (x0 > x1) continue;
+        float4 r0 = rec[g].r0, r1 = rec[g].r1;
+        for (int ty = y0; ty <= y1; ++ty)
+            for (int tx = x0; tx <= x1; ++tx) {
+                uint32_t bits = tile_depth_bits(r0.x, r0.y, r1.y, r1.z, r1.w, tx, ty);
+                uint32_t p = atomicAdd(&cur[ty * v.ntx + tx], 1u);
+                if (p < cap) pairs[p] = ((uint64_t)bits << 32) | (uint32_t)g;
+            }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Per-tile LSD radix sort.
+// ------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t match8(uint32_t d, uint64_t active) {
+    uint64_t peers = active;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        int bit = (d >> b) & 1;
+        uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
+template <int NT>
+__device__ __forceinline__ void block_minmax(uint32_t& mn, uint32_t& mx, uint32_t* red) {
+    constexpr int NW = NT / 64;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor((int)mn, o));
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
+    }
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { red[w] = mn; red[NW + w] = mx; }
+    __syncthreads();
+    mn = red[0]; mx = red[NW];
+    for (int i = 1; i < NW; ++i) { mn = min(mn, red[i]); mx = max(mx, red[NW + i]); }
+    __syncthreads();
+}
+
+// Exclusive scan of 256 LDS counters in place (NT >= 256 threads, all call).
+template <int NT>
+__device__ __forceinline__ void scan256(uint32_t* a, uint32_t* tmp) {
+    const int tid = threadIdx.x;
+    uint32_t x = (tid < 256) ? a[tid] : 0u;
+    if (tid < 256) tmp[tid] = x;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        uint32_t y = (tid < 256 && tid >= off) ? tmp[tid - off] : 0u;
+        __syncthreads();
+        if (tid < 256) tmp[tid] += y;
+        __syncthreads();
+    }
+    if (tid < 256) a[tid] = tmp[tid] - x;
+    __syncthreads();
+}
+
+template <int NT, int E>
+struct SortLds {
+    static constexpr int NW = NT / 64;
+    static constexpr int CHUNK = NT * E;
+    // whist[NW*256] + dbase[256] + tmp[512] (the reduction scratch lives in tmp's tail)
+    static constexpr size_t kHeader = (size_t)(NW * 256 + 256 + 512) * 4;
+    static constexpr size_t bytes(bool resident) {
+        return kHeader + (resident ? (size_t)CHUNK * 8 : 0);
+    }
+};
+
+// One pass over one chunk: stable ranks per wave, per-digit bases, scatter.
+// resident: digit bases are the chunk's own exclusive digit scan; otherwise dbase[] holds the
+// running global bases of the segment and is advanced by the chunk's digit totals.
+template <int NT, int E>
+__device__ __forceinline__ void radix_chunk(uint64_t (&k)[E], int cnt, int shift,
+                                            uint32_t* whist, uint32_t* dbase, uint32_t* tmp,
+                                            bool resident, uint64_t* dst, uint32_t limit) {
+    constexpr int NW = NT / 64;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for (int i = tid; i < NW * 256; i += NT) whist[i] = 0;
+    __syncthreads();
+    uint32_t rk[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        int j = w * 64 * E + e * 64 + lane;
+        bool valid = j < cnt;
+        uint32_t d = valid ? (uint32_t)(k[e] >> shift) & 255u : 0u;
+        uint64_t act = __ballot(valid);
+        uint64_t peers = match8(d, act);
+        uint32_t lr = __popcll(peers & lt);
+        uint32_t base = 0;
+        if (valid) base = whist[w * 256 + d];
+        if (valid && lr == 0) whist[w * 256 + d] = base + (uint32_t)__popcll(peers);
+        rk[e] = base + lr;
+    }
+    __syncthreads();
+    if (tid < 256) {
+        uint32_t run = 0;
+        for (int ww = 0; ww < NW; ++ww) {
+            uint32_t h = whist[ww * 256 + tid];
+            whist[ww * 256 + tid] = run;
+            run += h;
+        }
+        tmp[256 + tid] = run;  // digit total of this chunk
+    }
+    __syncthreads();
+    if (resident) {
+        if (tid < 256) dbase[tid] = tmp[256 + tid];
+        __syncthreads();
+        scan256<NT>(dbase, tmp);
+    }
+    if (tid < 256) {
+        uint32_t b = dbase[tid];
+        for (int ww = 0; ww < NW; ++ww) whist[ww * 256 + tid] += b;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        int j = w * 64 * E + e * 64 + lane;
+        if (j < cnt) {
+            uint32_t d = (uint32_t)(k[e] >> shift) & 255u;
+            uint32_t pos = whist[w * 256 + d] + rk[e];
+            if (pos < limit) dst[pos] = k[e];   // always true for a consistent ranking
+        }
+    }
+    if (!resident && tid < 256) dbase[tid] += tmp[256 + tid];
+    __syncthreads();
+}
+
+// Sorts the tiles whose list length n satisfies lo < n <= hi (the other tiles are left to a
+// launch with a different configuration). Lists up to CHUNK stay resident in LDS.
+template <int NT, int E>
+__global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ ranges,
+                                                  int idx_bits, uint64_t* __restrict__ pairs,
+                                                  uint64_t* __restrict__ tmpbuf,
+                                                  uint32_t* __restrict__ sorted_idx,
+                                                  uint64_t* __restrict__ keys_out, uint32_t lo,
+                                                  uint32_t hi) {
+    using L = SortLds<NT, E>;
+    constexpr int NW = L::NW, CHUNK = L::CHUNK;
+    extern __shared__ __align__(16) uint8_t smem[];
+    uint32_t* whist = reinterpret_cast<uint32_t*>(smem);
+    uint32_t* dbase = whist + NW * 256;
+    uint32_t* tmp = dbase + 256;          // 512 entries
+    uint32_t* red = tmp + 512 - 64;       // reuse tail for reductions (NW*2 <= 64)
+    uint64_t* xbuf = reinterpret_cast<uint64_t*>(smem + L::kHeader);
+    const int tile = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const uint32_t s = ranges[2 * tile], n = ranges[2 * tile + 1] - s;
+    if (n == 0 || n <= lo || n > hi) return;
+    const bool resident = n <= (uint32_t)CHUNK;
+    const uint64_t idx_mask = (1ull << idx_bits) - 1ull;
+    if (n == 1) {
+        if (tid == 0) {
+            uint64_t p = pairs[s];
+            sorted_idx[s] = (uint32_t)p;
+            if (keys_out) keys_out[s] = ((uint64_t)tile << 32) | (p >> 32);
+        }
+        return;
+    }
+    uint64_t k[E];
+    uint32_t mn = 0xffffffffu, mx = 0u;
+    if (resident) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            int j = w * 64 * E + e * 64 + lane;
+            k[e] = (j < (int)n) ? pairs[s + j] : 0ull;
+            if (j < (int)n) {
+                uint32_t d = (uint32_t)(k[e] >> 32);
+                mn = min(mn, d);
+                mx = max(mx, d);
+            }
+        }
+    } else {
+        for (uint32_t j = tid; j < n; j += NT) {
+            uint32_t d = (uint32_t)(pairs[s + j] >> 32);
+            mn = min(mn, d);
+            mx = max(mx, d);
+        }
+    }
+    block_minmax<NT>(mn, mx, red);
+    const uint32_t span = mx - mn;
+    const int dep_bits = span ? 32 - __clz(span) : 0;
+    const int total_bits = dep_bits + idx_bits;
+    const int passes = (total_bits + 7) / 8;
+    auto xform = [&](uint64_t p) -> uint64_t {
+        return ((uint64_t)((uint32_t)(p >> 32) - mn) << idx_bits) | (p & 0xffffffffull);
+    };
+    if (resident) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            int j = w * 64 * E + e * 64 + lane;
+            if (j < (int)n) k[e] = xform(k[e]);
+        }
+        for (int p = 0; p < passes; ++p) {
+            radix_chunk<NT, E>(k, (int)n, 8 * p, whist, dbase, tmp, true, xbuf, n);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                int j = w * 64 * E + e * 64 + lane;
+                if (j < (int)n) k[e] = xbuf[j];
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            int j = w * 64 * E + e * 64 + lane;
+            if (j < (int)n) {
+                sorted_idx[s + j] = (uint32_t)(k[e] & idx_mask);
+                if (keys_out)
+                    keys_out[s + j] = ((uint64_t)tile << 32) | (uint32_t)((k[e] >> idx_bits) + mn);
+            }
+        }
+        return;
+    }
+    // Streaming: ping-pong between pairs and tmpbuf within [s, s + n).
+    uint64_t* src = pairs + s;
+    uint64_t* dst = tmpbuf + s;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        if (tid < 256) dbase[tid] = 0;
+        __syncthreads();
+        for (uint32_t j = tid; j < n; j += NT) {
+            uint64_t key = (p == 0) ? xform(src[j]) : src[j];
+            atomicAdd(&dbase[(uint32_t)(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        scan256<NT>(dbase, tmp);
+        for (uint32_t c0 = 0; c0 < n; c0 += CHUNK) {
+            int cnt = (int)min((uint32_t)CHUNK, n - c0);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                int j = w * 64 * E + e * 64 + lane;
+                if (j < cnt) {
+                    uint64_t key = src[c0 + j];
+                    k[e] = (p == 0) ? xform(key) : key;
+                } else {
+                    k[e] = 0ull;
+                }
+            }
+            radix_chunk<NT, E>(k, cnt, shift, whist, dbase, tmp, false, dst, n);
+        }
+        // make this pass's global writes visible to the next pass's loads
+        __threadfence_block();
+        __syncthreads();
+        uint64_t* t2 = src; src = dst; dst = t2;
+    }
+    for (uint32_t j = tid; j < n; j += NT) {
+        uint64_t key = src[j];
+        sorted_idx[s + j] = (uint32_t)(key & idx_mask);
+        if (keys_out) keys_out[s + j] = ((uint64_t)tile << 32) | (uint32_t)((key >> idx_bits) + mn);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_project(const float* pos, const float* scale, const float* quat,
+                          const float* color, const float* opac, int n, const View& v,
+                          int cull_emit, Rec* rec, uint2* rect, uint32_t* blk_counts,
+                          hipStream_t s) {
+    int nblk = (n + kBinChunk - 1) / kBinChunk;
+    size_t lds = (size_t)v.ntx * v.nty * 4;
+    hipLaunchKernelGGL(k_project, dim3(nblk), dim3(kBinThreads), lds, s, pos, scale, quat, color,
+                       opac, n, v, cull_emit, rec, rect, blk_counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* chunk_pre, int nblk, int ntiles,
+                          uint32_t* ranges, uint32_t* d_m, hipStream_t s) {
+    int nchunk = (nblk + kScanRows - 1) / kScanRows;
+    hipLaunchKernelGGL(k_colscan1, dim3((ntiles + 255) / 256, nchunk), dim3(256), 0, s,
+                       blk_counts, chunk_pre, nblk, ntiles);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    size_t lds = ((size_t)ntiles + 1024 + 32) * 4;
+    hipLaunchKernelGGL(k_colscan2, dim3(1), dim3(1024), lds, s, chunk_pre, nchunk, ntiles,
+                       ranges, d_m);
+    return hipGetLastError();
+}
+
+hipError_t launch_emit(const Rec* rec, const uint2* rect, const uint32_t* blk_counts,
+                       const uint32_t* chunk_pre, int n, const View& v, uint64_t* pairs,
+                       uint64_t cap, hipStream_t s) {
+    int nblk = (n + kBinChunk - 1) / kBinChunk;
+    size_t lds = (size_t)v.ntx * v.nty * 4;
+    hipLaunchKernelGGL(k_emit, dim3(nblk), dim3(kBinThreads), lds, s, rec, rect, blk_counts,
+                       chunk_pre, n, v, pairs, cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uint64_t* pairs,
+                            uint64_t* tmp, uint32_t* sorted_idx, uint64_t* keys_out,
+                            size_t lds_limit, hipStream_t s) {
+    int idx_bits = 1;
+    while (idx_bits < 32 && ((uint64_t)1 << idx_bits) < (uint64_t)n_gauss) ++idx_bits;
+    // Small tiles: 256 threads x 16 keys resident in LDS; larger ones: 1024 x 16 when the
+    // device allows that much LDS per workgroup, else stream.
+    using Small = SortLds<256, 16>;
+    using Large = SortLds<1024, 16>;
+    size_t small_lds = Small::bytes(true);
+    size_t large_lds = Large::bytes(true);
+    if (large_lds <= lds_limit) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            hipFuncSetAttribute((const void*)k_tile_sort<1024, 16>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)large_lds);
+            attr_set = true;
+        }
+        const uint32_t small_cap = (uint32_t)Small::CHUNK;
+        hipLaunchKernelGGL((k_tile_sort<256, 16>), dim3(ntiles), dim3(256), small_lds, s, ranges,
+                           idx_bits, pairs, tmp, sorted_idx, keys_out, 0u, small_cap);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_tile_sort<1024, 16>), dim3(ntiles), dim3(1024), large_lds, s,
+                           ranges, idx_bits, pairs, tmp, sorted_idx, keys_out, small_cap,
+                           0xffffffffu);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL((k_tile_sort<256, 16>), dim3(ntiles), dim3(256), small_lds, s, ranges,
+                       idx_bits, pairs, tmp, sorted_idx, keys_out, 0u, 0xffffffffu);
+    return hipGetLastError();
+}
+
+}  // namespace fvgs

@@ -1,0 +1,760 @@
+// C ABI of libfvgs (include/fvgs/fvgs.h): context, device memory, stream, call sequencing.
+//
+// Host-side counterpart of GSSlam's hot methods (GSSlam.cu:408-646, 648-799, 1083-1264,
+// 2103-2244). Unlike the reference there is no cudaDeviceSynchronize after each launch: a call
+// enqueues its kernels on the context's stream and synchronises once, where a host result is
+// needed (the instance count M to size the sort buffers, the 27 FP64 sums of a pose call).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/fvgs/fvgs.h"
+#include "fvgs_internal.h"
+
+using namespace fvgs;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct fvgs_ctx {
+    fvgs_params prm;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string err;
+    size_t lds_limit = 65536;
+    size_t allocated = 0;
+    bool timing = false;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    float last_ms[4] = {0, 0, 0, 0};
+
+    // map
+    uint32_t n = 0;
+    DevBuf own_pos, own_scale, own_quat, own_color, own_opac;
+    const float *pos = nullptr, *scale = nullptr, *quat = nullptr, *color = nullptr,
+                *opac = nullptr;
+    // frame
+    int fw = 0, fh = 0;
+    DevBuf own_rgb, own_depth, own_gx, own_gy;
+    const float *obs_rgb = nullptr, *obs_depth = nullptr, *gx = nullptr, *gy = nullptr;
+    // view
+    bool prepared = false, forwarded = false, backwarded = false;
+    View view{};
+    uint32_t m = 0;
+    // screen + binning
+    DevBuf rec, rect, blk_counts, chunk_pre, ranges, d_m, pairs, tmp, sorted_idx, keys;
+    // outputs
+    DevBuf r_rgb, r_depth, r_alpha;
+    DevBuf o_rgb, o_depth, o_T, o_nc, o_err, o_maxc;
+    DevBuf d2, d3;
+    DevBuf pose_tiles, pose_sum, t_rgb, t_depth;
+    uint32_t* h_m = nullptr;      // pinned
+    double* h_sum = nullptr;      // pinned, 27
+};
+
+namespace {
+
+int fail(fvgs_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                  \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess)                                                            \
+            return fail((c), FVGS_E_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+int ensure(fvgs_ctx* c, DevBuf& b, size_t bytes) {
+    if (b.bytes >= bytes && b.p) return FVGS_OK;
+    size_t want = bytes + bytes / 4 + 256;
+    if (c->prm.mem_budget && c->allocated - b.bytes + want > c->prm.mem_budget) {
+        want = bytes;
+        if (c->allocated - b.bytes + want > c->prm.mem_budget)
+            return fail(c, FVGS_E_OOM, "memory budget %zu bytes exceeded (need %zu more)",
+                        c->prm.mem_budget, want);
+    }
+    if (b.p) {
+        hipFree(b.p);
+        c->allocated -= b.bytes;
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        b.p = nullptr;
+        return fail(c, FVGS_E_OOM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    b.bytes = want;
+    c->allocated += want;
+    return FVGS_OK;
+}
+
+void release(fvgs_ctx* c, DevBuf& b) {
+    if (b.p) {
+        hipFree(b.p);
+        c->allocated -= b.bytes;
+    }
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+template <typename T>
+T* as(DevBuf& b) {
+    return reinterpret_cast<T*>(b.p);
+}
+
+#define TRY(x)                 \
+    do {                       \
+        int _r = (x);          \
+        if (_r != FVGS_OK) return _r; \
+    } while (0)
+
+BlendParams blend_params(const fvgs_params& p) {
+    BlendParams b;
+    for (int i = 0; i < 3; i++) b.bg[i] = p.bg[i];
+    b.alpha_thresh = p.alpha_thresh;
+    b.color_thresh = p.color_thresh;
+    b.depth_thresh = p.depth_thresh;
+    b.depth_weight = p.depth_weight;
+    b.w_depth = p.w_depth;
+    b.w_dist = p.w_dist;
+    b.lambda_iso = p.lambda_iso;
+    return b;
+}
+
+bool finite_pose(const fvgs_pose* p) {
+    for (int i = 0; i < 3; i++)
+        if (!std::isfinite(p->position[i])) return false;
+    for (int i = 0; i < 4; i++)
+        if (!std::isfinite(p->orientation[i])) return false;
+    return true;
+}
+
+void mark(fvgs_ctx* c, int i) {
+    if (c->timing) (void)hipEventRecord(c->ev[i], c->stream);
+}
+
+void collect_timing(fvgs_ctx* c, int nmarks) {
+    if (!c->timing) return;
+    for (int i = 0; i + 1 < nmarks; i++) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]) == hipSuccess) c->last_ms[i] = ms;
+    }
+}
+
+// Projection + binning for one view (body of fvgs_prepare_rasterization, mutex held).
+int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int w, int h) {
+    if (!pose || !intr) return fail(c, FVGS_E_ARG, "null pose or intrinsics");
+    if (w <= 0 || h <= 0) return fail(c, FVGS_E_ARG, "invalid image size %d x %d", w, h);
+    if (!finite_pose(pose)) return fail(c, FVGS_E_ARG, "non-finite pose");
+    if (!(intr->f[0] > 0.f) || !(intr->f[1] > 0.f))
+        return fail(c, FVGS_E_ARG, "invalid focal length");
+    c->prepared = c->forwarded = c->backwarded = false;
+    if (c->n == 0) return fail(c, FVGS_E_EMPTY, "no Gaussians");
+    View v;
+    v.px = pose->position[0]; v.py = pose->position[1]; v.pz = pose->position[2];
+    v.qx = pose->orientation[0]; v.qy = pose->orientation[1];
+    v.qz = pose->orientation[2]; v.qw = pose->orientation[3];
+    v.fx = intr->f[0]; v.fy = intr->f[1]; v.cx = intr->c[0]; v.cy = intr->c[1];
+    v.W = w; v.H = h;
+    v.ntx = (w + kTile - 1) / kTile;
+    v.nty = (h + kTile - 1) / kTile;
+    const int T = v.ntx * v.nty;
+    if (T > kMaxTiles)
+        return fail(c, FVGS_E_ARG, "%d tiles exceed the supported %d", T, kMaxTiles);
+    const int n = (int)c->n;
+    const int nblk = (n + kBinChunk - 1) / kBinChunk;
+    const int nchunk = (nblk + kScanRows - 1) / kScanRows;
+    TRY(ensure(c, c->rec, sizeof(Rec) * (size_t)n));
+    TRY(ensure(c, c->rect, sizeof(uint2) * (size_t)n));
+    TRY(ensure(c, c->blk_counts, 4ull * nblk * T));
+    TRY(ensure(c, c->chunk_pre, 4ull * nchunk * T));
+    TRY(ensure(c, c->ranges, 8ull * T));
+    TRY(ensure(c, c->d_m, 16));
+    hipStream_t s = c->stream;
+    mark(c, 0);
+    HIPCHK(c, launch_project(c->pos, c->scale, c->quat, c->color, c->opac, n, v,
+                             c->prm.cull_emit, as<Rec>(c->rec), as<uint2>(c->rect),
+                             as<uint32_t>(c->blk_counts), s));
+    HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->chunk_pre), nblk, T,
+                             as<uint32_t>(c->ranges), as<uint32_t>(c->d_m), s));
+    HIPCHK(c, hipMemcpyAsync(c->h_m, c->d_m.p, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    const uint32_t m = *c->h_m;
+    c->view = v;
+    c->m = m;
+    if (m == 0) return fail(c, FVGS_E_EMPTY, "no tile instances");
+    TRY(ensure(c, c->pairs, 8ull * m));
+    TRY(ensure(c, c->tmp, 8ull * m));
+    TRY(ensure(c, c->sorted_idx, 4ull * m));
+    uint64_t* keys = nullptr;
+    if (c->prm.keep_keys) {
+        TRY(ensure(c, c->keys, 8ull * m));
+        keys = as<uint64_t>(c->keys);
+    }
+    HIPCHK(c, launch_emit(as<Rec>(c->rec), as<uint2>(c->rect), as<uint32_t>(c->blk_counts),
+                          as<uint32_t>(c->chunk_pre), n, v, as<uint64_t>(c->pairs),
+                          c->pairs.bytes / 8, s));
+    HIPCHK(c, launch_tile_sort(as<uint32_t>(c->ranges), T, n, as<uint64_t>(c->pairs),
+                               as<uint64_t>(c->tmp), as<uint32_t>(c->sorted_idx), keys,
+                               c->lds_limit, s));
+    mark(c, 1);
+    c->prepared = true;
+    return FVGS_OK;
+}
+
+int alloc_pixel_outputs(fvgs_ctx* c) {
+    const size_t P = (size_t)c->view.W * c->view.H;
+    const int T = c->view.ntx * c->view.nty;
+    TRY(ensure(c, c->o_rgb, 12 * P));
+    TRY(ensure(c, c->o_depth, 4 * P));
+    TRY(ensure(c, c->o_T, 4 * P));
+    TRY(ensure(c, c->o_nc, 4 * P));
+    TRY(ensure(c, c->o_err, 12 * P));
+    TRY(ensure(c, c->o_maxc, 4ull * T));
+    return FVGS_OK;
+}
+
+int check_frame(fvgs_ctx* c) {
+    if (!c->obs_rgb || !c->obs_depth) return fail(c, FVGS_E_STATE, "no frame set");
+    if (c->fw != c->view.W || c->fh != c->view.H)
+        return fail(c, FVGS_E_ARG, "frame is %d x %d but the view is %d x %d", c->fw, c->fh,
+                    c->view.W, c->view.H);
+    return FVGS_OK;
+}
+
+int run_forward(fvgs_ctx* c, bool with_backward) {
+    if (!c->prepared) return fail(c, FVGS_E_STATE, "call fvgs_prepare_rasterization first");
+    TRY(check_frame(c));
+    TRY(alloc_pixel_outputs(c));
+    Delta2D* d2 = nullptr;
+    if (with_backward) {
+        TRY(ensure(c, c->d2, sizeof(Delta2D) * (size_t)c->n));
+        d2 = as<Delta2D>(c->d2);
+        HIPCHK(c, hipMemsetAsync(d2, 0, sizeof(Delta2D) * (size_t)c->n, c->stream));
+    }
+    mark(c, 2);
+    HIPCHK(c, launch_mapping(c->view, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
+                             as<Rec>(c->rec), c->obs_rgb, c->obs_depth, blend_params(c->prm),
+                             as<float>(c->o_rgb), as<float>(c->o_depth), as<float>(c->o_T),
+                             as<uint32_t>(c->o_nc), as<float>(c->o_err),
+                             as<uint32_t>(c->o_maxc), d2, with_backward ? 1 : 0, c->stream));
+    c->forwarded = true;
+    c->backwarded = with_backward;
+    return FVGS_OK;
+}
+
+int run_delta3d(fvgs_ctx* c) {
+    if (!c->backwarded) return fail(c, FVGS_E_STATE, "call fvgs_backward first");
+    TRY(ensure(c, c->d3, sizeof(Delta3D) * (size_t)c->n));
+    HIPCHK(c, launch_delta3d(c->pos, c->scale, c->quat, (int)c->n, c->view, c->prm.lambda_iso,
+                             as<Delta2D>(c->d2), as<Delta3D>(c->d3), c->stream));
+    return FVGS_OK;
+}
+
+// Pose Jacobians at the prepared view; leaves 27 FP64 sums in c->h_sum.
+int run_pose(fvgs_ctx* c, bool want_render) {
+    TRY(check_frame(c));
+    if (!c->gx || !c->gy) return fail(c, FVGS_E_STATE, "frame has no image gradients");
+    const int T = c->view.ntx * c->view.nty;
+    const size_t P = (size_t)c->view.W * c->view.H;
+    TRY(ensure(c, c->pose_tiles, 27 * 4ull * T));
+    TRY(ensure(c, c->pose_sum, 27 * 8));
+    float *trgb = nullptr, *tdep = nullptr;
+    if (want_render) {
+        TRY(ensure(c, c->t_rgb, 12 * P));
+        TRY(ensure(c, c->t_depth, 4 * P));
+        trgb = as<float>(c->t_rgb);
+        tdep = as<float>(c->t_depth);
+    }
+    HIPCHK(c, launch_pose(c->view, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
+                          as<Rec>(c->rec), c->obs_rgb, c->obs_depth, c->gx, c->gy,
+                          blend_params(c->prm), as<float>(c->pose_tiles), trgb, tdep, c->stream));
+    HIPCHK(c, launch_pose_reduce(as<float>(c->pose_tiles), T, as<double>(c->pose_sum), c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_sum, c->pose_sum.p, 27 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FVGS_OK;
+}
+
+int upload(fvgs_ctx* c, DevBuf& own, const float*& dst, const float* src, size_t bytes, int mem) {
+    if (mem == FVGS_MEM_DEVICE) {
+        dst = src;
+        return FVGS_OK;
+    }
+    TRY(ensure(c, own, bytes));
+    HIPCHK(c, hipMemcpyAsync(own.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    dst = as<float>(own);
+    return FVGS_OK;
+}
+
+struct OutDesc {
+    const void* p;
+    size_t bytes;
+};
+
+OutDesc describe(fvgs_ctx* c, int what) {
+    const size_t n = c->n, P = (size_t)c->view.W * c->view.H;
+    const size_t T = (size_t)c->view.ntx * c->view.nty, M = c->m;
+    switch (what) {
+        case FVGS_OUT_POS2D: case FVGS_OUT_COV2D: case FVGS_OUT_CONIC: case FVGS_OUT_DEPTH_G:
+        case FVGS_OUT_PHAT: return {c->prepared ? c->rec.p : nullptr, sizeof(Rec) * n};
+        case FVGS_OUT_KEYS: return {c->prepared ? c->keys.p : nullptr, 8 * M};
+        case FVGS_OUT_INDICES: return {c->prepared ? c->sorted_idx.p : nullptr, 4 * M};
+        case FVGS_OUT_RANGES: return {c->prepared ? c->ranges.p : nullptr, 8 * T};
+        case FVGS_OUT_RENDER_RGB: return {c->r_rgb.p, 12 * P};
+        case FVGS_OUT_RENDER_DEPTH: return {c->r_depth.p, 4 * P};
+        case FVGS_OUT_RENDER_ALPHA: return {c->r_alpha.p, 4 * P};
+        case FVGS_OUT_RGB: return {c->forwarded ? c->o_rgb.p : nullptr, 12 * P};
+        case FVGS_OUT_DEPTH: return {c->forwarded ? c->o_depth.p : nullptr, 4 * P};
+        case FVGS_OUT_FINAL_T: return {c->forwarded ? c->o_T.p : nullptr, 4 * P};
+        case FVGS_OUT_N_CONTRIB: return {c->forwarded ? c->o_nc.p : nullptr, 4 * P};
+        case FVGS_OUT_COLOR_ERROR: return {c->forwarded ? c->o_err.p : nullptr, 12 * P};
+        case FVGS_OUT_MAX_CONTRIB: return {c->forwarded ? c->o_maxc.p : nullptr, 4 * T};
+        case FVGS_OUT_DELTA2D: return {c->backwarded ? c->d2.p : nullptr, sizeof(Delta2D) * n};
+        case FVGS_OUT_DELTA3D: return {c->d3.p, sizeof(Delta3D) * n};
+        case FVGS_OUT_POSE_TILES: return {c->pose_tiles.p, 108 * T};
+        case FVGS_OUT_TRACK_RGB: return {c->t_rgb.p, 12 * P};
+        case FVGS_OUT_TRACK_DEPTH: return {c->t_depth.p, 4 * P};
+        default: return {nullptr, 0};
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvgs_abi_version(void) { return FVGS_ABI_VERSION; }
+
+int fvgs_default_params(fvgs_params* p) {
+    if (!p) return FVGS_E_ARG;
+    std::memset(p, 0, sizeof *p);
+    p->device = 0;
+    p->bg[0] = p->bg[1] = p->bg[2] = 0.f;
+    p->alpha_thresh = 0.2f;
+    p->color_thresh = 0.1f;
+    p->depth_thresh = 0.2f;
+    p->depth_weight = 1.0f;
+    p->w_depth = 1.0f;
+    p->w_dist = 0.1f;
+    p->lambda_iso = 0.01f;
+    p->cull_emit = 1;
+    p->keep_keys = 0;
+    p->mem_budget = 0;
+    return FVGS_OK;
+}
+
+int fvgs_create(const fvgs_params* params, fvgs_ctx** out) {
+    if (!out) return FVGS_E_ARG;
+    *out = nullptr;
+    fvgs_ctx* c = new (std::nothrow) fvgs_ctx();
+    if (!c) return FVGS_E_OOM;
+    if (params) c->prm = *params;
+    else fvgs_default_params(&c->prm);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        delete c;
+        return FVGS_E_HIP;
+    }
+    if (c->prm.device < 0 || c->prm.device >= ndev) {
+        delete c;
+        return FVGS_E_ARG;
+    }
+    if (hipSetDevice(c->prm.device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return FVGS_E_HIP;
+    }
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->prm.device) ==
+            hipSuccess && lds > 0)
+        c->lds_limit = (size_t)lds;
+    if (hipHostMalloc((void**)&c->h_m, 64, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_sum, 27 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        fvgs_destroy(c);
+        return FVGS_E_HIP;
+    }
+    for (auto& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            fvgs_destroy(c);
+            return FVGS_E_HIP;
+        }
+    *out = c;
+    return FVGS_OK;
+}
+
+int fvgs_destroy(fvgs_ctx* c) {
+    if (!c) return FVGS_E_ARG;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        DevBuf* bufs[] = {&c->own_pos, &c->own_scale, &c->own_quat, &c->own_color, &c->own_opac,
+                          &c->own_rgb, &c->own_depth, &c->own_gx, &c->own_gy, &c->rec, &c->rect,
+                          &c->blk_counts, &c->chunk_pre, &c->ranges, &c->d_m, &c->pairs,
+                          &c->tmp, &c->sorted_idx, &c->keys, &c->r_rgb, &c->r_depth, &c->r_alpha,
+                          &c->o_rgb, &c->o_depth, &c->o_T, &c->o_nc, &c->o_err, &c->o_maxc,
+                          &c->d2, &c->d3, &c->pose_tiles, &c->pose_sum, &c->t_rgb, &c->t_depth};
+        for (DevBuf* b : bufs) release(c, *b);
+        for (auto& e : c->ev)
+            if (e) (void)hipEventDestroy(e);
+        if (c->h_m) (void)hipHostFree(c->h_m);
+        if (c->h_sum) (void)hipHostFree(c->h_sum);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+    return FVGS_OK;
+}
+
+const char* fvgs_last_error(const fvgs_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int fvgs_set_gaussians(fvgs_ctx* c, uint32_t n, const float* pos3, const float* scale3,
+                       const float* quat4, const float* color3, const float* opacity, int mem) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (mem != FVGS_MEM_HOST && mem != FVGS_MEM_DEVICE) return fail(c, FVGS_E_ARG, "bad mem kind");
+    if (n > 0 && (!pos3 || !scale3 || !quat4 || !color3 || !opacity))
+        return fail(c, FVGS_E_ARG, "null Gaussian array");
+    if (n >= (1u << 31)) return fail(c, FVGS_E_ARG, "too many Gaussians");
+    (void)hipSetDevice(c->prm.device);
+    c->prepared = c->forwarded = c->backwarded = false;
+    c->n = 0;
+    if (n == 0) return FVGS_OK;
+    TRY(upload(c, c->own_pos, c->pos, pos3, 12ull * n, mem));
+    TRY(upload(c, c->own_scale, c->scale, scale3, 12ull * n, mem));
+    TRY(upload(c, c->own_quat, c->quat, quat4, 16ull * n, mem));
+    TRY(upload(c, c->own_color, c->color, color3, 12ull * n, mem));
+    TRY(upload(c, c->own_opac, c->opac, opacity, 4ull * n, mem));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->n = n;
+    return FVGS_OK;
+}
+
+int fvgs_gaussian_count(const fvgs_ctx* c, uint32_t* n) {
+    if (!c || !n) return FVGS_E_ARG;
+    *n = c->n;
+    return FVGS_OK;
+}
+
+int fvgs_set_frame(fvgs_ctx* c, int w, int h, const float* rgb3, const float* depth,
+                   const float* gx3, const float* gy3, int mem) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (w <= 0 || h <= 0) return fail(c, FVGS_E_ARG, "invalid frame size %d x %d", w, h);
+    if (!rgb3 || !depth) return fail(c, FVGS_E_ARG, "null frame image");
+    if (mem != FVGS_MEM_HOST && mem != FVGS_MEM_DEVICE) return fail(c, FVGS_E_ARG, "bad mem kind");
+    (void)hipSetDevice(c->prm.device);
+    const size_t P = (size_t)w * h;
+    c->obs_rgb = c->obs_depth = c->gx = c->gy = nullptr;
+    TRY(upload(c, c->own_rgb, c->obs_rgb, rgb3, 12 * P, mem));
+    TRY(upload(c, c->own_depth, c->obs_depth, depth, 4 * P, mem));
+    if (gx3 && gy3) {
+        TRY(upload(c, c->own_gx, c->gx, gx3, 12 * P, mem));
+        TRY(upload(c, c->own_gy, c->gy, gy3, 12 * P, mem));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->fw = w;
+    c->fh = h;
+    return FVGS_OK;
+}
+
+int fvgs_prepare_rasterization(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr,
+                               int w, int h, uint32_t* n_instances) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->prm.device);
+    int r = prepare(c, pose, intr, w, h);
+    if (n_instances) *n_instances = (r == FVGS_OK || r == FVGS_E_EMPTY) ? c->m : 0;
+    if (r == FVGS_OK && c->timing) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        collect_timing(c, 2);
+    }
+    return r;
+}
+
+int fvgs_render(fvgs_ctx* c) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->prepared) return fail(c, FVGS_E_STATE, "call fvgs_prepare_rasterization first");
+    const size_t P = (size_t)c->view.W * c->view.H;
+    TRY(ensure(c, c->r_rgb, 12 * P));
+    TRY(ensure(c, c->r_depth, 4 * P));
+    TRY(ensure(c, c->r_alpha, 4 * P));
+    HIPCHK(c, launch_render(c->view, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
+                            as<Rec>(c->rec), as<float>(c->r_rgb), as<float>(c->r_depth),
+                            as<float>(c->r_alpha), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FVGS_OK;
+}
+
+int fvgs_forward(fvgs_ctx* c) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    TRY(run_forward(c, false));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FVGS_OK;
+}
+
+int fvgs_backward(fvgs_ctx* c) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    // The fused kernel recomputes the forward; its outputs are identical to fvgs_forward's.
+    TRY(run_forward(c, true));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FVGS_OK;
+}
+
+int fvgs_gradients_3d(fvgs_ctx* c) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    TRY(run_delta3d(c));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FVGS_OK;
+}
+
+int fvgs_mapping_step(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->prm.device);
+    if (!c->obs_rgb) return fail(c, FVGS_E_STATE, "no frame set");
+    TRY(prepare(c, pose, intr, c->fw, c->fh));
+    TRY(run_forward(c, true));
+    mark(c, 3);
+    TRY(run_delta3d(c));
+    mark(c, 4);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    collect_timing(c, 5);
+    return FVGS_OK;
+}
+
+int fvgs_pose_jacobians(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr,
+                        double jtj21[21], double jtr6[6]) {
+    if (!c || !jtj21 || !jtr6) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->prm.device);
+    if (!c->obs_rgb) return fail(c, FVGS_E_STATE, "no frame set");
+    TRY(prepare(c, pose, intr, c->fw, c->fh));
+    TRY(run_pose(c, false));
+    for (int k = 0; k < 6; k++) jtr6[k] = c->h_sum[k];
+    for (int k = 0; k < 21; k++) jtj21[k] = c->h_sum[6 + k];
+    return FVGS_OK;
+}
+
+int fvgs_render_tracking(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int w,
+                         int h) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->prm.device);
+    TRY(prepare(c, pose, intr, w, h));
+    const int T = c->view.ntx * c->view.nty;
+    const size_t P = (size_t)w * h;
+    TRY(ensure(c, c->pose_tiles, 27 * 4ull * T));
+    TRY(ensure(c, c->t_rgb, 12 * P));
+    TRY(ensure(c, c->t_depth, 4 * P));
+    HIPCHK(c, launch_pose(c->view, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
+                          as<Rec>(c->rec), nullptr, nullptr, nullptr, nullptr,
+                          blend_params(c->prm), as<float>(c->pose_tiles), as<float>(c->t_rgb),
+                          as<float>(c->t_depth), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FVGS_OK;
+}
+
+int fvgs_compute_gradients(fvgs_ctx* c) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->obs_rgb) return fail(c, FVGS_E_STATE, "no frame set");
+    const size_t P = (size_t)c->fw * c->fh;
+    TRY(ensure(c, c->own_gx, 12 * P));
+    TRY(ensure(c, c->own_gy, 12 * P));
+    HIPCHK(c, launch_sobel(c->obs_rgb, c->fw, c->fh, as<float>(c->own_gx), as<float>(c->own_gy),
+                           c->stream));
+    c->gx = as<float>(c->own_gx);
+    c->gy = as<float>(c->own_gy);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FVGS_OK;
+}
+
+int fvgs_gn_step(const double jtj21[21], const double jtr6[6], double pose7[7],
+                 double delta6[6]) {
+    if (!jtj21 || !jtr6 || !pose7) return FVGS_E_ARG;
+    double A[6][6];
+    int id = 0;
+    for (int r = 0; r < 6; r++)
+        for (int cc = r; cc < 6; cc++) {
+            A[r][cc] = jtj21[id];
+            A[cc][r] = jtj21[id];
+            id++;
+        }
+    double L[6][6] = {{0}};
+    for (int j = 0; j < 6; j++) {
+        double sacc = A[j][j];
+        for (int k = 0; k < j; k++) sacc -= L[j][k] * L[j][k];
+        if (!(sacc > 0.0) || !std::isfinite(sacc)) return FVGS_E_NOTPD;
+        L[j][j] = std::sqrt(sacc);
+        for (int i = j + 1; i < 6; i++) {
+            double t = A[i][j];
+            for (int k = 0; k < j; k++) t -= L[i][k] * L[j][k];
+            L[i][j] = t / L[j][j];
+        }
+    }
+    double y[6], d[6];
+    for (int i = 0; i < 6; i++) {
+        double t = jtr6[i];
+        for (int k = 0; k < i; k++) t -= L[i][k] * y[k];
+        y[i] = t / L[i][i];
+    }
+    for (int i = 5; i >= 0; i--) {
+        double t = y[i];
+        for (int k = i + 1; k < 6; k++) t -= L[k][i] * d[k];
+        d[i] = t / L[i][i];
+    }
+    if (delta6)
+        for (int i = 0; i < 6; i++) delta6[i] = d[i];
+    // PoseLocalParameterization::Plus: p += dp; q = normalize(q * [1, dtheta / 2]).
+    pose7[0] += d[0];
+    pose7[1] += d[1];
+    pose7[2] += d[2];
+    const double qx = pose7[3], qy = pose7[4], qz = pose7[5], qw = pose7[6];
+    const double bx = 0.5 * d[3], by = 0.5 * d[4], bz = 0.5 * d[5];
+    const double rw = qw - qx * bx - qy * by - qz * bz;
+    const double rx = qw * bx + qx + qy * bz - qz * by;
+    const double ry = qw * by - qx * bz + qy + qz * bx;
+    const double rz = qw * bz + qx * by - qy * bx + qz;
+    const double nrm = std::sqrt(rw * rw + rx * rx + ry * ry + rz * rz);
+    pose7[3] = rx / nrm;
+    pose7[4] = ry / nrm;
+    pose7[5] = rz / nrm;
+    pose7[6] = rw / nrm;
+    return FVGS_OK;
+}
+
+int fvgs_track(fvgs_ctx* c, const fvgs_intrinsics* intr, int iters, double pose7[7],
+               double* step_ms) {
+    if (!c || !intr || !pose7 || iters < 0) return FVGS_E_ARG;
+    for (int it = 0; it < iters; it++) {
+        auto t0 = std::chrono::steady_clock::now();
+        fvgs_pose p;
+        for (int k = 0; k < 3; k++) p.position[k] = (float)pose7[k];
+        for (int k = 0; k < 4; k++) p.orientation[k] = (float)pose7[3 + k];
+        double jtj[21], jtr[6];
+        int r = fvgs_pose_jacobians(c, &p, intr, jtj, jtr);
+        if (r != FVGS_OK) return r;
+        r = fvgs_gn_step(jtj, jtr, pose7, nullptr);
+        if (r != FVGS_OK) {
+            std::lock_guard<std::mutex> lk(c->mu);
+            return fail(c, r, "Gauss-Newton step failed at iteration %d (JtJ not SPD)", it);
+        }
+        if (step_ms)
+            step_ms[it] = std::chrono::duration<double, std::milli>(
+                              std::chrono::steady_clock::now() - t0).count();
+    }
+    return FVGS_OK;
+}
+
+int fvgs_read(fvgs_ctx* c, int what, void* host, size_t bytes) {
+    if (!c || !host) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    OutDesc d = describe(c, what);
+    if (!d.p) return fail(c, FVGS_E_STATE, "result %d is not available", what);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // Screen-record fields are strided inside the 64-byte records: gather them on the host.
+    if (what >= FVGS_OUT_POS2D && what <= FVGS_OUT_PHAT) {
+        const size_t n = c->n;
+        Rec* tmp = new (std::nothrow) Rec[n];
+        if (!tmp) return fail(c, FVGS_E_OOM, "host allocation failed");
+        hipError_t e = hipMemcpy(tmp, d.p, sizeof(Rec) * n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            delete[] tmp;
+            return fail(c, FVGS_E_HIP, "hipMemcpy: %s", hipGetErrorString(e));
+        }
+        float* o = static_cast<float*>(host);
+        const size_t width = (what == FVGS_OUT_POS2D || what == FVGS_OUT_PHAT) ? 2
+                             : (what == FVGS_OUT_DEPTH_G) ? 1 : 3;
+        const size_t cnt = std::min(n, bytes / (4 * width));
+        for (size_t i = 0; i < cnt; i++) {
+            const Rec& r = tmp[i];
+            switch (what) {
+                case FVGS_OUT_POS2D: o[2 * i] = r.r0.x; o[2 * i + 1] = r.r0.y; break;
+                case FVGS_OUT_COV2D: o[3 * i] = r.r3.x; o[3 * i + 1] = r.r3.y; o[3 * i + 2] = r.r3.z; break;
+                case FVGS_OUT_CONIC: o[3 * i] = r.r0.z; o[3 * i + 1] = r.r0.w; o[3 * i + 2] = r.r1.x; break;
+                case FVGS_OUT_DEPTH_G: o[i] = r.r1.y; break;
+                case FVGS_OUT_PHAT: o[2 * i] = r.r1.z; o[2 * i + 1] = r.r1.w; break;
+            }
+        }
+        delete[] tmp;
+        return FVGS_OK;
+    }
+    HIPCHK(c, hipMemcpy(host, d.p, std::min(bytes, d.bytes), hipMemcpyDeviceToHost));
+    return FVGS_OK;
+}
+
+int fvgs_device_ptr(fvgs_ctx* c, int what, void** ptr, size_t* bytes) {
+    if (!c || !ptr) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    OutDesc d = describe(c, what);
+    if (!d.p) return fail(c, FVGS_E_STATE, "result %d is not available", what);
+    *ptr = const_cast<void*>(d.p);
+    if (bytes) *bytes = d.bytes;
+    return FVGS_OK;
+}
+
+int fvgs_sync(fvgs_ctx* c) {
+    if (!c) return FVGS_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FVGS_OK;
+}
+
+int fvgs_stats(fvgs_ctx* c, uint64_t* m, uint64_t* k, uint32_t* tiles, uint32_t* pixels) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const uint32_t T = (uint32_t)(c->view.ntx * c->view.nty);
+    if (m) *m = c->m;
+    if (tiles) *tiles = T;
+    if (pixels) *pixels = (uint32_t)(c->view.W * c->view.H);
+    if (k) {
+        *k = 0;
+        if (c->forwarded && T > 0) {
+            uint32_t* h = new (std::nothrow) uint32_t[T];
+            if (!h) return fail(c, FVGS_E_OOM, "host allocation failed");
+            hipError_t e = hipMemcpy(h, c->o_maxc.p, 4ull * T, hipMemcpyDeviceToHost);
+            if (e == hipSuccess)
+                for (uint32_t t = 0; t < T; t++) *k += h[t];
+            delete[] h;
+            if (e != hipSuccess) return fail(c, FVGS_E_HIP, "hipMemcpy: %s", hipGetErrorString(e));
+        }
+    }
+    return FVGS_OK;
+}
+
+int fvgs_set_timing(fvgs_ctx* c, int enable) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->timing = enable != 0;
+    return FVGS_OK;
+}
+
+int fvgs_last_timing(fvgs_ctx* c, float ms[4]) {
+    if (!c || !ms) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    for (int i = 0; i < 4; i++) ms[i] = c->last_ms[i];
+    return FVGS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// Internal interface between the C ABI (fvgs_api.cpp) and the gfx950 kernels (*.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace fvgs {
+
+constexpr int kTile = 16;          // pixels per tile side (reference TILE_SIZE)
+constexpr int kTilePix = 256;      // pixels per tile (reference BLOCK_SIZE)
+constexpr int kBucket = 32;        // backward bucket length (GSCudaKernels.cu:2087)
+constexpr int kMaxBuckets = 8;     // only the first 256 entries get gradients (Q10)
+constexpr int kMaxTiles = 8192;    // LDS tile histogram of the binning kernels (32 KB)
+constexpr int kBinThreads = 256;   // binning block
+constexpr int kBinItems = 4;       // Gaussians per thread per binning block
+constexpr int kBinChunk = kBinThreads * kBinItems;
+constexpr int kScanRows = 64;      // binning blocks per column-scan chunk
+
+// View description passed by value to kernels.
+struct View {
+    float px, py, pz;          // camera position (world)
+    float qx, qy, qz, qw;      // camera orientation, read (x, y, z, w)
+    float fx, fy, cx, cy;      // intrinsics
+    int W, H, ntx, nty;        // image size and tile grid
+};
+
+// Screen-space record of one Gaussian, 64 bytes (4 x float4), one aligned half cache line:
+//   r0 = (u, v, conic0, conic1)      conic as stored by the projection: (inv_xx, inv_yy, inv_xy)
+//   r1 = (conic2, depth, phx, phy)
+//   r2 = (r, g, b, opacity)
+//   r3 = (cov_xx, cov_yy, cov_xy, 0)
+struct alignas(16) Rec {
+    float4 r0, r1, r2, r3;
+};
+
+struct Delta2D {  // DeltaGaussian2D layout (52 bytes)
+    float mx, my, sx, sy, sz, cr, cg, cb, depth, alpha, phx, phy;
+    uint32_t n;
+};
+
+struct Delta3D {  // DeltaGaussian3D layout (56 bytes)
+    float px, py, pz, sx, sy, sz, ox, oy, oz, cr, cg, cb, alpha;
+    int32_t n;
+};
+
+struct BlendParams {
+    float bg[3];
+    float alpha_thresh, color_thresh, depth_thresh, depth_weight;
+    float w_depth, w_dist, lambda_iso;
+};
+
+// ---- binning (binning.hip) ----
+hipError_t launch_project(const float* pos, const float* scale, const float* quat,
+                          const float* color, const float* opac, int n, const View& v,
+                          int cull_emit, Rec* rec, uint2* rect, uint32_t* blk_counts,
+                          hipStream_t s);
+hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* chunk_pre, int nblk, int ntiles,
+                          uint32_t* ranges, uint32_t* d_m, hipStream_t s);
+hipError_t launch_emit(const Rec* rec, const uint2* rect, const uint32_t* blk_counts,
+                       const uint32_t* chunk_pre, int n, const View& v, uint64_t* pairs,
+                       uint64_t cap, hipStream_t s);
+hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uint64_t* pairs,
+                            uint64_t* tmp, uint32_t* sorted_idx, uint64_t* keys_out,
+                            size_t lds_limit, hipStream_t s);
+
+// ---- blending (raster.hip) ----
+hipError_t launch_render(const View& v, const uint32_t* ranges, const uint32_t* idx,
+                         const Rec* rec, float* rgb, float* depth, float* alpha, hipStream_t s);
+hipError_t launch_mapping(const View& v, const uint32_t* ranges, const uint32_t* idx,
+                          const Rec* rec, const float* obs_rgb, const float* obs_depth,
+                          const BlendParams& bp, float* rgb, float* depth, float* final_T,
+                          uint32_t* n_contrib, float* color_err, uint32_t* max_contrib,
+                          Delta2D* delta, int do_backward, hipStream_t s);
+hipError_t launch_delta3d(const float* pos, const float* scale, const float* quat, int n,
+                          const View& v, float lambda_iso, const Delta2D* d2, Delta3D* d3,
+                          hipStream_t s);
+
+// ---- tracking (track.hip) ----
+hipError_t launch_pose(const View& v, const uint32_t* ranges, const uint32_t* idx,
+                       const Rec* rec, const float* obs_rgb, const float* obs_depth,
+                       const float* gx, const float* gy, const BlendParams& bp,
+                       float* tiles27, float* out_rgb, float* out_depth, hipStream_t s);
+hipError_t launch_pose_reduce(const float* tiles27, int ntiles, double* out27, hipStream_t s);
+hipError_t launch_sobel(const float* img, int W, int H, float* gx, float* gy, hipStream_t s);
+
+}  // namespace fvgs

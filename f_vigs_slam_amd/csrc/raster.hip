@@ -1,0 +1,499 @@
+// Tile blending for gfx950: render-only forward, fused mapping forward + backward, 2D->3D.
+//
+// Replaces forwardPassTileKernel (GSCudaKernels.cu:710-859), optimizeGaussiansForwardPass
+// (2092-2263), optimizeGaussiansPerGaussianPass (2265-2502) and computeDeltaGaussians3D_kernel
+// (2504-2720).
+//
+// k_mapping fuses the reference's forward and per-bucket backward into one workgroup per tile:
+//  * forward: one thread per pixel walks the depth-sorted tile list in LDS-staged batches of
+//    256 records (64 B each); the (T, colour) checkpoints at list indices 0, 32, ..., 224 stay in
+//    LDS instead of a per-call [buckets x 256] HBM buffer, since only the first 256 entries of a
+//    tile receive gradients (GSCudaKernels.cu:2329-2330);
+//  * backward: each 64-lane wave replays two 32-entry buckets (lanes 0-31 and 32-63); lane l owns
+//    entry b*32 + (l & 31) and the per-pixel transmittance/colour state flows lane to lane, so
+//    step i of lane l handles pixel i - (l & 31) — the reference's warp pipeline widened to
+//    wave64. Per-pixel forward results are read from LDS; one atomicAdd per field and entry.
+// Reference defects Q8, Q9, Q10 and Q14 are transcribed (see the oracle header).
+#include "fvgs_internal.h"
+
+namespace fvgs {
+
+namespace {
+
+constexpr float kAlphaMin = 1.0f / 255.0f;
+
+// ---- A12 ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_render(View v, const uint32_t* __restrict__ ranges,
+                                                const uint32_t* __restrict__ idx,
+                                                const Rec* __restrict__ rec,
+                                                float* __restrict__ out_rgb,
+                                                float* __restrict__ out_depth,
+                                                float* __restrict__ out_alpha) {
+    __shared__ Rec recs[256];
+    const int tile = blockIdx.x, tid = threadIdx.x;
+    const int x = (tile % v.ntx) * kTile + (tid & 15), y = (tile / v.ntx) * kTile + (tid >> 4);
+    const bool inside = x < v.W && y < v.H;
+    const uint32_t s = ranges[2 * tile], n = ranges[2 * tile + 1] - s;
+    const float px = (float)x, py = (float)y;
+    float cr = 0.f, cg = 0.f, cb = 0.f, A = 0.f, D = 1e10f;
+    bool has_med = false, done = !inside;
+    for (uint32_t base = 0; base < n; base += 256) {
+        if (__syncthreads_count(!done) == 0) break;
+        if (base + tid < n) recs[tid] = rec[idx[s + base + tid]];
+        __syncthreads();
+        const int cnt = (int)min(256u, n - base);
+        if (!done) {
+            for (int i = 0; i < cnt; ++i) {
+                const float4 r0 = recs[i].r0, r1 = recs[i].r1, r2 = recs[i].r2;
+                // The reference inverts the covariance per pixel; the stored conic is that same
+                // FP32 inverse, except for the culled zero record whose inverse is the identity.
+                float ixx = r0.z, iyy = r0.w, ixy = r1.x;
+                if (ixx == 0.f && iyy == 0.f && ixy == 0.f) { ixx = 1.f; iyy = 1.f; }
+                const float dx = px - r0.x, dy = py - r0.y;
+                const float gv = __expf(-0.5f * (ixx * dx * dx + iyy * dy * dy + 2.0f * ixy * dx * dy));
+                const float wa = r2.w * gv;
+                const float Tb = 1.0f - A;
+                cr += Tb * wa * r2.x;
+                cg += Tb * wa * r2.y;
+                cb += Tb * wa * r2.z;
+                A += Tb * wa;
+                const float Ta = 1.0f - A;
+                if (!has_med && Tb > 0.5f && Ta <= 0.5f) {
+                    D = r1.y + r1.z * dx + r1.w * dy;
+                    has_med = true;
+                }
+                if (A >= 0.99f) { done = true; break; }
+            }
+        }
+    }
+    if (inside) {
+        const size_t p = (size_t)y * v.W + x;
+        out_rgb[3 * p] = cr; out_rgb[3 * p + 1] = cg; out_rgb[3 * p + 2] = cb;
+        out_depth[p] = D;
+        out_alpha[p] = A;
+    }
+}
+
+// ---- A14 + A15 ------------------------------------------------------------------------------
+struct MapLds {
+    Rec recs[256];
+    float ckT[kMaxBuckets][256];
+    float ckR[kMaxBuckets][256], ckG[kMaxBuckets][256], ckB[kMaxBuckets][256];
+    float pR[256], pG[256], pB[256], pD[256], pER[256], pEG[256], pEB[256], pObs[256];
+    uint32_t pLast[256];
+    uint32_t maxc;
+};
+
+__global__ __launch_bounds__(256) void k_mapping(
+    View v, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ idx,
+    const Rec* __restrict__ rec, const float* __restrict__ obs_rgb,
+    const float* __restrict__ obs_depth, BlendParams bp, float* __restrict__ out_rgb,
+    float* __restrict__ out_depth, float* __restrict__ out_T, uint32_t* __restrict__ out_nc,
+    float* __restrict__ out_err, uint32_t* __restrict__ max_contrib,
+    Delta2D* __restrict__ delta, int do_backward) {
+    __shared__ MapLds L;
+    const int tile = blockIdx.x, tid = threadIdx.x;
+    const int tx0 = (tile % v.ntx) * kTile, ty0 = (tile / v.ntx) * kTile;
+    const int x = tx0 + (tid & 15), y = ty0 + (tid >> 4);
+    const bool inside = x < v.W && y < v.H;
+    const uint32_t s = ranges[2 * tile], n = ranges[2 * tile + 1] - s;
+    const float px = (float)x, py = (float)y;
+    if (tid == 0) L.maxc = 0;
+
+    // ---------------- forward (GSCudaKernels.cu:2150-2232) ----------------
+    float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, D = 0.f;
+    uint32_t contributor = 0, last = 0;
+    bool done = !inside;
+    for (uint32_t base = 0; base < n; base += 256) {
+        if (__syncthreads_count(!done) == 0) break;
+        if (base + tid < n) L.recs[tid] = rec[idx[s + base + tid]];
+        __syncthreads();
+        const int cnt = (int)min(256u, n - base);
+        if (!done) {
+            for (int i = 0; i < cnt; ++i) {
+                const uint32_t j = base + i;
+                if ((j & 31u) == 0u && j < 256u) {
+                    const int b = (int)(j >> 5);
+                    L.ckT[b][tid] = T; L.ckR[b][tid] = cr; L.ckG[b][tid] = cg; L.ckB[b][tid] = cb;
+                }
+                ++contributor;
+                const float4 r0 = L.recs[i].r0, r1 = L.recs[i].r1;
+                const float dx = px - r0.x, dy = py - r0.y;
+                // Q14: conic slots read as (xx, xy, yy).
+                const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;
+                const float4 r2 = L.recs[i].r2;
+                const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));
+                if (a < kAlphaMin || q <= 0.0f) continue;
+                const float ta = T * a;
+                cr += ta * r2.x; cg += ta * r2.y; cb += ta * r2.z;
+                const float tT = T * (1.0f - a);
+                if (T > 0.5f && tT <= 0.5f) D = r1.y + r1.z * dx + r1.w * dy;
+                T = tT;
+                last = contributor;
+                if (tT < 0.0001f) { done = true; break; }
+            }
+        }
+    }
+    const float fr = cr + T * bp.bg[0], fg = cg + T * bp.bg[1], fb = cb + T * bp.bg[2];
+    float er = 0.f, eg = 0.f, eb = 0.f, od = 0.f;
+    if (inside) {
+        const size_t p = (size_t)y * v.W + x;
+        er = fr - obs_rgb[3 * p]; eg = fg - obs_rgb[3 * p + 1]; eb = fb - obs_rgb[3 * p + 2];
+        od = obs_depth[p];
+        out_T[p] = T;
+        out_nc[p] = last;
+        out_rgb[3 * p] = fr; out_rgb[3 * p + 1] = fg; out_rgb[3 * p + 2] = fb;
+        out_depth[p] = D;
+        out_err[3 * p] = er; out_err[3 * p + 1] = eg; out_err[3 * p + 2] = eb;
+    }
+    // Block max of n_contrib (cub::BlockReduce Max in the reference).
+    uint32_t m = last;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+    __syncthreads();
+    if ((tid & 63) == 0) atomicMax(&L.maxc, m);
+    L.pR[tid] = fr; L.pG[tid] = fg; L.pB[tid] = fb; L.pD[tid] = D;
+    L.pER[tid] = er; L.pEG[tid] = eg; L.pEB[tid] = eb; L.pObs[tid] = od;
+    L.pLast[tid] = inside ? last : 0u;
+    __syncthreads();
+    const uint32_t maxc = L.maxc;
+    if (tid == 0) max_contrib[tile] = maxc;
+    if (!do_backward || maxc == 0) return;
+
+    // ---------------- backward (GSCudaKernels.cu:2304-2501) ----------------
+    const uint32_t nsplat = min(n, 256u);
+    if (n > 256u) {  // batch 0 was overwritten by later batches
+        if (tid < (int)nsplat) L.recs[tid] = rec[idx[s + tid]];
+        __syncthreads();
+    }
+    const int w = tid >> 6, lane = tid & 63, sl = lane & 31;
+    const int b = 2 * w + (lane >> 5);
+    if ((uint32_t)(2 * w * kBucket) >= maxc) return;  // both buckets of this wave are idle
+    const uint32_t sidx = (uint32_t)(b * kBucket + sl);
+    const bool valid = (uint32_t)(b * kBucket) < maxc && sidx < nsplat;
+    float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
+    if (valid) { r0 = L.recs[sidx].r0; r1 = L.recs[sidx].r1; r2 = L.recs[sidx].r2; }
+    float dmx = 0.f, dmy = 0.f, dsx = 0.f, dsy = 0.f, dsz = 0.f, dcr = 0.f, dcg = 0.f, dcb = 0.f,
+          ddep = 0.f, dal = 0.f, dphx = 0.f, dphy = 0.f;
+    uint32_t dn = 0;
+    float bT = 0.f, ar = 0.f, ag = 0.f, ab = 0.f;
+    const int wlim = v.W - tx0, hlim = v.H - ty0;
+    for (int i = 0; i < 256 + 31; ++i) {
+        bT = __shfl_up(bT, 1);
+        ar = __shfl_up(ar, 1);
+        ag = __shfl_up(ag, 1);
+        ab = __shfl_up(ab, 1);
+        const int p = i - sl;
+        const bool pix_ok = p >= 0 && p < 256 && (p & 15) < wlim && (p >> 4) < hlim;
+        if (!(valid && pix_ok)) continue;
+        if (sl == 0) { bT = L.ckT[b][p]; ar = L.ckR[b][p]; ag = L.ckG[b][p]; ab = L.ckB[b][p]; }
+        if (sidx >= L.pLast[p]) continue;
+        const float dx = r0.x - (float)(tx0 + (p & 15));   // Q9: mean - pixel
+        const float dy = r0.y - (float)(ty0 + (p >> 4));
+        const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;
+        const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));   // Q8: no q <= 0 test
+        if (a < kAlphaMin) continue;
+        const float Er = L.pER[p], Eg = L.pEG[p], Eb = L.pEB[p];
+        float dar = r2.x * bT, dag = r2.y * bT, dab = r2.z * bT;
+        ar += dar * a; ag += dag * a; ab += dab * a;
+        const float iom = 1.0f / (1.0f - a);
+        dar -= (L.pR[p] - ar) * iom;
+        dag -= (L.pG[p] - ag) * iom;
+        dab -= (L.pB[p] - ab) * iom;
+        dar *= -Er; dag *= -Eg; dab *= -Eb;
+        const float aG = a * (dar + dag + dab);
+        ++dn;
+        const float at = a * bT;
+        dcr -= at * Er; dcg -= at * Eg; dcb -= at * Eb;
+        dal -= aG;
+        dmx -= aG * (r0.z * dx + r0.w * dy);
+        dmy -= aG * (r0.w * dx + r1.x * dy);
+        dsx -= 0.5f * aG * dx * dx;
+        dsy -= 0.5f * aG * dx * dy;
+        dsz -= 0.5f * aG * dy * dy;
+        const float tT = bT * (1.0f - a);
+        const float Dp = L.pD[p];
+        if (bT > 0.5f && tT <= 0.5f) {
+            const float obs = L.pObs[p];
+            const float de = (obs > 0.1f) ? (Dp - obs) : 0.0f;
+            const float wde = bp.w_depth * de;
+            ddep -= wde;
+            dmx -= wde * r1.z; dmy -= wde * r1.w;
+            dphx -= wde * dx; dphy -= wde * dy;
+        }
+        const float di = r1.y + dx * r1.z + dy * r1.w;
+        const float dc = bp.w_dist * a * bT * (di - Dp);
+        ddep -= dc;
+        dmx -= dc * r1.z; dmy -= dc * r1.w;
+        dphx -= dc * dx; dphy -= dc * dy;
+        bT = tT;
+    }
+    if (valid && dn > 0) {
+        Delta2D* d = delta + idx[s + sidx];
+        atomicAdd(&d->depth, ddep);
+        atomicAdd(&d->phx, dphx);
+        atomicAdd(&d->phy, dphy);
+        atomicAdd(&d->mx, dmx);
+        atomicAdd(&d->my, dmy);
+        atomicAdd(&d->sx, dsx);
+        atomicAdd(&d->sy, dsy);
+        atomicAdd(&d->sz, dsz);
+        atomicAdd(&d->cr, dcr);
+        atomicAdd(&d->cg, dcg);
+        atomicAdd(&d->cb, dcb);
+        atomicAdd(&d->alpha, dal);
+        atomicAdd(&d->n, dn);
+    }
+}
+
+// ---- A16 ------------------------------------------------------------------------------------
+__device__ __forceinline__ void qmat(float x, float y, float z, float w, float R[3][3]) {
+    R[0][0] = 1.0f - 2.0f * (y * y + z * z);
+    R[0][1] = 2.0f * (x * y - w * z);
+    R[0][2] = 2.0f * (x * z + w * y);
+    R[1][0] = 2.0f * (x * y + w * z);
+    R[1][1] = 1.0f - 2.0f * (x * x + z * z);
+    R[1][2] = 2.0f * (y * z - w * x);
+    R[2][0] = 2.0f * (x * z - w * y);
+    R[2][1] = 2.0f * (y * z + w * x);
+    R[2][2] = 1.0f - 2.0f * (x * x + y * y);
+}
+
+__global__ __launch_bounds__(256) void k_delta3d(const float* __restrict__ pos,
+                                                 const float* __restrict__ scale,
+                                                 const float* __restrict__ quat, int n, View v,
+                                                 float lambda_iso,
+                                                 const Delta2D* __restrict__ d2in,
+                                                 Delta3D* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const Delta2D d2 = d2in[i];
+    Delta3D o;
+    o.px = o.py = o.pz = o.sx = o.sy = o.sz = o.ox = o.oy = o.oz = 0.f;
+    o.cr = o.cg = o.cb = o.alpha = 0.f;
+    o.n = 0;
+    if (d2.n == 0) { out[i] = o; return; }   // Q11: zero record
+    o.n = (int32_t)d2.n;
+    const float eta = 1.0f / (5.0f + (float)d2.n);
+    o.cr = d2.cr * eta; o.cg = d2.cg * eta; o.cb = d2.cb * eta;
+    o.alpha = d2.alpha * eta;
+    float Rq[3][3];
+    qmat(v.qx, v.qy, v.qz, v.qw, Rq);
+    const float dw[3] = {pos[3 * (size_t)i] - v.px, pos[3 * (size_t)i + 1] - v.py,
+                         pos[3 * (size_t)i + 2] - v.pz};
+    float mu[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) mu[a] = Rq[0][a] * dw[0] + Rq[1][a] * dw[1] + Rq[2][a] * dw[2];
+    const float J[3][3] = {{v.fx / mu[2], 0.f, -v.fx * mu[0] / (mu[2] * mu[2])},
+                           {0.f, v.fy / mu[2], -v.fy * mu[1] / (mu[2] * mu[2])},
+                           {0.f, 0.f, 1.f}};
+    float Rcw[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) Rcw[a][b] = Rq[b][a];
+    float Rg[3][3];
+    qmat(quat[4 * (size_t)i], quat[4 * (size_t)i + 1], quat[4 * (size_t)i + 2],
+         quat[4 * (size_t)i + 3], Rg);
+    const float s[3] = {scale[3 * (size_t)i], scale[3 * (size_t)i + 1], scale[3 * (size_t)i + 2]};
+    float T[3][3], RS[3][3], M[3][3], Sp[3][3], S3[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            T[a][b] = J[a][0] * Rcw[0][b] + J[a][1] * Rcw[1][b] + J[a][2] * Rcw[2][b];
+            RS[a][b] = Rg[a][b] * s[b];
+        }
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+            M[a][b] = T[a][0] * RS[0][b] + T[a][1] * RS[1][b] + T[a][2] * RS[2][b];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            Sp[a][b] = M[a][0] * M[b][0] + M[a][1] * M[b][1] + M[a][2] * M[b][2];
+            S3[a][b] = RS[a][0] * RS[b][0] + RS[a][1] * RS[b][1] + RS[a][2] * RS[b][2];
+        }
+    float Si[3][3];
+    {
+        const float c00 = Sp[1][1] * Sp[2][2] - Sp[1][2] * Sp[2][1];
+        const float c01 = Sp[1][2] * Sp[2][0] - Sp[1][0] * Sp[2][2];
+        const float c02 = Sp[1][0] * Sp[2][1] - Sp[1][1] * Sp[2][0];
+        const float det = Sp[0][0] * c00 + Sp[0][1] * c01 + Sp[0][2] * c02;
+        const float id = 1.0f / det;
+        Si[0][0] = c00 * id;
+        Si[0][1] = (Sp[0][2] * Sp[2][1] - Sp[0][1] * Sp[2][2]) * id;
+        Si[0][2] = (Sp[0][1] * Sp[1][2] - Sp[0][2] * Sp[1][1]) * id;
+        Si[1][0] = c01 * id;
+        Si[1][1] = (Sp[0][0] * Sp[2][2] - Sp[0][2] * Sp[2][0]) * id;
+        Si[1][2] = (Sp[0][2] * Sp[1][0] - Sp[0][0] * Sp[1][2]) * id;
+        Si[2][0] = c02 * id;
+        Si[2][1] = (Sp[0][1] * Sp[2][0] - Sp[0][0] * Sp[2][1]) * id;
+        Si[2][2] = (Sp[0][0] * Sp[1][1] - Sp[0][1] * Sp[1][0]) * id;
+    }
+    // dL/dmu_3d = q_cw * (J^T dL/dmu_2d + depth * (x/z, y/z, 1))   (GSCudaKernels.cu:2594)
+    float tv[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) tv[a] = J[0][a] * d2.mx + J[1][a] * d2.my;
+    tv[0] += d2.depth * (mu[0] / mu[2]);
+    tv[1] += d2.depth * (mu[1] / mu[2]);
+    tv[2] += d2.depth;
+    float dmu[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) dmu[a] = Rq[a][0] * tv[0] + Rq[a][1] * tv[1] + Rq[a][2] * tv[2];
+    const float phx = d2.phx, phy = d2.phy;
+    const float n2 = mu[0] * mu[0] + mu[1] * mu[1] + mu[2] * mu[2];
+    const float nn = sqrtf(n2);
+    const float in3 = 1.0f / (n2 * nn);
+    const float dph[3] = {-mu[0] * mu[2] * in3, -mu[1] * mu[2] * in3,
+                          (mu[0] * mu[0] + mu[1] * mu[1]) * in3};
+    const float fph = (Si[2][0] * phx + Si[2][1] * phy) / Si[2][2];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        dmu[a] += Rq[a][0] * dph[0] * fph + Rq[a][1] * dph[1] * fph + Rq[a][2] * dph[2] * fph;
+    // dL/dSigma' from dL/dSigma'^{-1} (GSCudaKernels.cu:2606-2619)
+    const float sxx = Sp[0][0] + 0.001f, sxy = Sp[1][0], syy = Sp[1][1] + 0.001f;
+    const float den = sxx * syy - sxy * sxy;
+    const float d2i = 1.0f / ((den * den) + 0.0000001f);
+    const float gx = d2.sx, gy = d2.sy, gz = d2.sz;
+    const float dSxx = d2i * (-syy * syy * gx + 2 * sxy * syy * gy + (den - sxx * syy) * gz);
+    const float dSyy = d2i * (-sxx * sxx * gz + 2 * sxx * sxy * gy + (den - sxx * syy) * gx);
+    const float dSxy = d2i * 2 * (sxy * syy * gx - (den + 2 * sxy * sxy) * gy + sxx * sxy * gz);
+    float dT[2][3];
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        const float q0 = T[0][0] * S3[b][0] + T[0][1] * S3[b][1] + T[0][2] * S3[b][2];
+        const float q1 = T[1][0] * S3[b][0] + T[1][1] * S3[b][1] + T[1][2] * S3[b][2];
+        dT[0][b] = 2 * q0 * dSxx + q1 * dSxy;
+        dT[1][b] = 2 * q1 * dSyy + q0 * dSxy;
+    }
+    const float zt = mu[2] / nn;
+    const float s22 = Si[2][2];
+    const float dSI[3][3] = {{0.f, 0.f, 0.5f * zt * phx / s22},
+                             {0.f, 0.f, 0.5f * zt * phy / s22},
+                             {0.5f * zt * phx / s22, 0.5f * zt * phy / s22,
+                              -zt * (phx * Si[2][0] + phy * Si[2][1]) / (s22 * s22)}};
+    float tmp[3][3], dSP[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+            tmp[a][b] = Si[a][0] * dSI[0][b] + Si[a][1] * dSI[1][b] + Si[a][2] * dSI[2][b];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+            dSP[a][b] = -(tmp[a][0] * Si[0][b] + tmp[a][1] * Si[1][b] + tmp[a][2] * Si[2][b]);
+    float TS[3][3], ST[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            TS[a][b] = T[a][0] * S3[0][b] + T[a][1] * S3[1][b] + T[a][2] * S3[2][b];
+            ST[a][b] = S3[a][0] * T[b][0] + S3[a][1] * T[b][1] + S3[a][2] * T[b][2];
+        }
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+            dT[r][b] += (TS[0][b] * dSP[0][r] + TS[1][b] * dSP[1][r] + TS[2][b] * dSP[2][r]) +
+                        (ST[b][0] * dSP[r][0] + ST[b][1] * dSP[r][1] + ST[b][2] * dSP[r][2]);
+    const float dJ00 = Rcw[0][0] * dT[0][0] + Rcw[0][1] * dT[0][1] + Rcw[0][2] * dT[0][2];
+    const float dJ02 = Rcw[2][0] * dT[0][0] + Rcw[2][1] * dT[0][1] + Rcw[2][2] * dT[0][2];
+    const float dJ11 = Rcw[1][0] * dT[1][0] + Rcw[1][1] * dT[1][1] + Rcw[1][2] * dT[1][2];
+    const float dJ12 = Rcw[2][0] * dT[1][0] + Rcw[2][1] * dT[1][1] + Rcw[2][2] * dT[1][2];
+    const float iz = 1.0f / mu[2], iz2 = iz * iz, iz3 = iz2 * iz;
+    const float dmc[3] = {-v.fx * iz2 * dJ02, -v.fy * iz2 * dJ12,
+                          -v.fx * iz2 * dJ00 - v.fy * iz2 * dJ11 +
+                              (2 * v.fx * mu[0]) * iz3 * dJ02 + (2 * v.fy * mu[1]) * iz3 * dJ12};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        dmu[a] += Rq[a][0] * dmc[0] + Rq[a][1] * dmc[1] + Rq[a][2] * dmc[2];
+    o.px = eta * dmu[0]; o.py = eta * dmu[1]; o.pz = eta * dmu[2];
+    // dL/dSigma_3d (GSCudaKernels.cu:2666-2675)
+    float dS3[3][3];
+    dS3[0][0] = T[0][0] * T[0][0] * dSxx + T[0][0] * T[1][0] * dSxy + T[1][0] * T[1][0] * dSyy;
+    dS3[1][1] = T[0][1] * T[0][1] * dSxx + T[0][1] * T[1][1] * dSxy + T[1][1] * T[1][1] * dSyy;
+    dS3[2][2] = T[0][2] * T[0][2] * dSxx + T[0][2] * T[1][2] * dSxy + T[1][2] * T[1][2] * dSyy;
+    dS3[1][0] = dS3[0][1] = T[0][0] * T[0][1] * dSxx +
+                            0.5f * (T[0][0] * T[1][1] + T[0][1] * T[1][0]) * dSxy +
+                            T[1][0] * T[1][1] * dSyy;
+    dS3[2][0] = dS3[0][2] = T[0][0] * T[0][2] * dSxx +
+                            0.5f * (T[0][0] * T[1][2] + T[0][2] * T[1][0]) * dSxy +
+                            T[1][0] * T[1][2] * dSyy;
+    dS3[2][1] = dS3[1][2] = T[0][2] * T[0][1] * dSxx +
+                            0.5f * (T[0][1] * T[1][2] + T[0][2] * T[1][1]) * dSxy +
+                            T[1][1] * T[1][2] * dSyy;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            float acc = 0.f;
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int l = 0; l < 3; l++) acc += T[k][a] * dSP[k][l] * T[l][b];
+            dS3[a][b] += acc;
+        }
+    float dM[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+            dM[a][b] = 2.0f * (RS[0][a] * dS3[0][b] + RS[1][a] * dS3[1][b] + RS[2][a] * dS3[2][b]);
+    float ds[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) ds[a] = Rg[0][a] * dM[a][0] + Rg[1][a] * dM[a][1] + Rg[2][a] * dM[a][2];
+    const float ms = (s[0] + s[1] + s[2]) / 3.f;
+    const float li[3] = {s[0] - ms, s[1] - ms, s[2] - ms};
+    const float lr = lambda_iso * (1.f / 3.f);
+    ds[0] -= lr * (2.f * li[0] - li[1] - li[2]);
+    ds[1] -= lr * (-li[0] + 2.f * li[1] - li[2]);
+    ds[2] -= lr * (-li[0] - li[1] + 2.f * li[2]);
+#pragma unroll
+    for (int b = 0; b < 3; b++) { dM[0][b] *= s[0]; dM[1][b] *= s[1]; dM[2][b] *= s[2]; }
+    float th[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const float a0 = Rg[r][0], a1 = Rg[r][1], a2 = Rg[r][2];
+        const float b0 = dM[0][r], b1 = dM[1][r], b2 = dM[2][r];
+        th[0] += a1 * b2 - a2 * b1;
+        th[1] += a2 * b0 - a0 * b2;
+        th[2] += a0 * b1 - a1 * b0;
+    }
+    o.ox = -eta * th[0]; o.oy = -eta * th[1]; o.oz = -eta * th[2];
+    o.sx = eta * ds[0]; o.sy = eta * ds[1]; o.sz = eta * ds[2];
+    out[i] = o;
+}
+
+}  // namespace
+
+hipError_t launch_render(const View& v, const uint32_t* ranges, const uint32_t* idx,
+                         const Rec* rec, float* rgb, float* depth, float* alpha, hipStream_t s) {
+    hipLaunchKernelGGL(k_render, dim3(v.ntx * v.nty), dim3(256), 0, s, v, ranges, idx, rec, rgb,
+                       depth, alpha);
+    return hipGetLastError();
+}
+
+hipError_t launch_mapping(const View& v, const uint32_t* ranges, const uint32_t* idx,
+                          const Rec* rec, const float* obs_rgb, const float* obs_depth,
+                          const BlendParams& bp, float* rgb, float* depth, float* final_T,
+                          uint32_t* n_contrib, float* color_err, uint32_t* max_contrib,
+                          Delta2D* delta, int do_backward, hipStream_t s) {
+    hipLaunchKernelGGL(k_mapping, dim3(v.ntx * v.nty), dim3(256), 0, s, v, ranges, idx, rec,
+                       obs_rgb, obs_depth, bp, rgb, depth, final_T, n_contrib, color_err,
+                       max_contrib, delta, do_backward);
+    return hipGetLastError();
+}
+
+hipError_t launch_delta3d(const float* pos, const float* scale, const float* quat, int n,
+                          const View& v, float lambda_iso, const Delta2D* d2, Delta3D* d3,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_delta3d, dim3((n + 255) / 256), dim3(256), 0, s, pos, scale, quat, n, v,
+                       lambda_iso, d2, d3);
+    return hipGetLastError();
+}
+
+}  // namespace fvgs

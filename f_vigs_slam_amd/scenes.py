@@ -1,0 +1,157 @@
+"""Seeded synthetic scenes, cameras and frames for the rasterizer and the RGB-D tracker.
+
+The distributions are those of the benchmark workloads:
+
+* means: pixel (u, v) uniform over the image, depth z on [0.5, 5] m with density proportional
+  to z^2 (uniform in the frustum volume), back-projected through the pinhole at the identity
+  pose of the kernels' camera frame (position 0, orientation float4 (0, 0, 0, 1) read as
+  (x, y, z, w));
+* scales log-uniform on [0.005, 0.05] m per axis; rotations normalised N(0, I4) quaternions
+  stored (x, y, z, w); opacity sigmoid(N(0, 1)); RGB uniform [0, 1].
+
+Everything is float32 (the reference's `Gaussians` SoA, RepresentationClasses.hpp:53-99).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+FX = FY = 525.0
+CX, CY = 319.5, 239.5
+IDENTITY_QUAT = np.array([0.0, 0.0, 0.0, 1.0], np.float32)  # (x, y, z, w) as the kernels read it
+
+
+def intrinsics(width=640, height=480, f=FX):
+    """Pinhole (fx, fy, cx, cy); the centre scales with the frame size."""
+    return np.array([f, f, (width - 1) / 2.0, (height - 1) / 2.0], np.float32)
+
+
+def make_scene(n, seed=0, width=640, height=480, intr=None, zmin=0.5, zmax=5.0,
+               depth_law="volume"):
+    """Config-0 style map of n Gaussians in front of the identity camera."""
+    rng = np.random.default_rng(seed)
+    it = intrinsics(width, height) if intr is None else np.asarray(intr, np.float32)
+    u = rng.uniform(0.0, width, n)
+    v = rng.uniform(0.0, height, n)
+    if depth_law == "volume":
+        a3, b3 = zmin ** 3, zmax ** 3
+        z = np.cbrt(a3 + rng.uniform(0.0, 1.0, n) * (b3 - a3))
+    else:
+        z = rng.uniform(zmin, zmax, n)
+    x = (u - it[2]) * z / it[0]
+    y = (v - it[3]) * z / it[1]
+    pos = np.stack([x, y, z], 1).astype(np.float32)
+    scale = np.exp(rng.uniform(math.log(0.005), math.log(0.05), (n, 3))).astype(np.float32)
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    quat = q.astype(np.float32)
+    opacity = (1.0 / (1.0 + np.exp(-rng.standard_normal(n)))).astype(np.float32)
+    color = rng.uniform(0.0, 1.0, (n, 3)).astype(np.float32)
+    return dict(pos=pos, scale=scale, quat=quat, color=color, opacity=opacity)
+
+
+def make_map_cube(n, seed=0, center=(0.0, 0.0, 0.0), half=1.0):
+    """A map with means uniform in a cube (mapping-window workload), config-0 attributes."""
+    rng = np.random.default_rng(seed)
+    pos = (np.asarray(center) + rng.uniform(-half, half, (n, 3))).astype(np.float32)
+    scale = np.exp(rng.uniform(math.log(0.005), math.log(0.05), (n, 3))).astype(np.float32)
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    opacity = (1.0 / (1.0 + np.exp(-rng.standard_normal(n)))).astype(np.float32)
+    color = rng.uniform(0.0, 1.0, (n, 3)).astype(np.float32)
+    return dict(pos=pos, scale=scale, quat=q.astype(np.float32), color=color, opacity=opacity)
+
+
+def quat_mul(a, b):
+    """Hamilton product of (x, y, z, w) quaternions."""
+    ax, ay, az, aw = a
+    bx, by, bz, bw = b
+    return np.array([aw * bx + ax * bw + ay * bz - az * by,
+                     aw * by - ax * bz + ay * bw + az * bx,
+                     aw * bz + ax * by - ay * bx + az * bw,
+                     aw * bw - ax * bx - ay * by - az * bz])
+
+
+def axis_angle_quat(axis, angle):
+    axis = np.asarray(axis, np.float64)
+    axis = axis / np.linalg.norm(axis)
+    s = math.sin(angle / 2.0)
+    return np.array([axis[0] * s, axis[1] * s, axis[2] * s, math.cos(angle / 2.0)])
+
+
+def perturbed_pose(seed, rot_deg=2.0, trans_m=0.03):
+    """Identity pose perturbed by rot_deg about a seeded unit axis and trans_m along a seeded
+    unit vector. Returns pose7 = (px, py, pz, qx, qy, qz, qw) in FP64."""
+    rng = np.random.default_rng(seed + 7919)
+    ax = rng.standard_normal(3)
+    tr = rng.standard_normal(3)
+    tr = tr / np.linalg.norm(tr) * trans_m
+    q = axis_angle_quat(ax, math.radians(rot_deg))
+    return np.concatenate([tr, q])
+
+
+def arc_poses(k, radius=2.0, center=(0.0, 0.0, 0.0), span_deg=90.0):
+    """k camera poses on a horizontal arc of the given radius, each looking at the centre.
+    Camera frame convention of the kernels: +z forward, +x right, +y down. Returns pose7 list."""
+    poses = []
+    c = np.asarray(center, np.float64)
+    for i in range(k):
+        a = math.radians(-span_deg / 2 + span_deg * (i / max(1, k - 1)))
+        p = c + radius * np.array([math.sin(a), 0.0, -math.cos(a)])
+        fwd = c - p
+        fwd /= np.linalg.norm(fwd)
+        down = np.array([0.0, 1.0, 0.0])
+        right = np.cross(down, fwd)
+        right /= np.linalg.norm(right)
+        down = np.cross(fwd, right)
+        R = np.stack([right, down, fwd], 1)  # columns: camera axes in world (camera-to-world)
+        poses.append(np.concatenate([p, mat_to_quat(R)]))
+    return poses
+
+
+def mat_to_quat(R):
+    """Rotation matrix -> (x, y, z, w) quaternion (the kernels' quaternionToMatrix inverse)."""
+    t = np.trace(R)
+    if t > 0:
+        s = math.sqrt(t + 1.0) * 2
+        w = 0.25 * s
+        x = (R[2, 1] - R[1, 2]) / s
+        y = (R[0, 2] - R[2, 0]) / s
+        z = (R[1, 0] - R[0, 1]) / s
+    elif R[0, 0] > R[1, 1] and R[0, 0] > R[2, 2]:
+        s = math.sqrt(1.0 + R[0, 0] - R[1, 1] - R[2, 2]) * 2
+        w = (R[2, 1] - R[1, 2]) / s
+        x = 0.25 * s
+        y = (R[0, 1] + R[1, 0]) / s
+        z = (R[0, 2] + R[2, 0]) / s
+    elif R[1, 1] > R[2, 2]:
+        s = math.sqrt(1.0 + R[1, 1] - R[0, 0] - R[2, 2]) * 2
+        w = (R[0, 2] - R[2, 0]) / s
+        x = (R[0, 1] + R[1, 0]) / s
+        y = 0.25 * s
+        z = (R[1, 2] + R[2, 1]) / s
+    else:
+        s = math.sqrt(1.0 + R[2, 2] - R[0, 0] - R[1, 1]) * 2
+        w = (R[1, 0] - R[0, 1]) / s
+        x = (R[0, 2] + R[2, 0]) / s
+        y = (R[1, 2] + R[2, 1]) / s
+        z = 0.25 * s
+    q = np.array([x, y, z, w])
+    return q / np.linalg.norm(q)
+
+
+def pose_error(pose_a, pose_b):
+    """(rotation angle in rad, translation distance in m) between two pose7 vectors."""
+    pa, pb = np.asarray(pose_a, np.float64), np.asarray(pose_b, np.float64)
+    dt = float(np.linalg.norm(pa[:3] - pb[:3]))
+    qa, qb = pa[3:] / np.linalg.norm(pa[3:]), pb[3:] / np.linalg.norm(pb[3:])
+    d = abs(float(np.dot(qa, qb)))
+    return 2.0 * math.acos(min(1.0, d)), dt
+
+
+def upstream_grads(width, height, seed=1, sigma=1e-2):
+    """N(0, sigma) per-channel perturbation used to build observations for the backward."""
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal((height, width, 3)) * sigma).astype(np.float32), \
+        (rng.standard_normal((height, width)) * sigma).astype(np.float32)

@@ -1,0 +1,245 @@
+/*
+ * fvgs — MI355X-native differentiable tile rasterizer and RGB-D pose tracker.
+ *
+ * C ABI of libfvgs.so. Plain pointers and sizes only; no C++ types cross it. Every function
+ * returns an int status: FVGS_OK (0) or a negative FVGS_E_* code; the text of the last error of
+ * a context is available from fvgs_last_error(). No exception crosses the ABI.
+ *
+ * Ownership. A context (fvgs_ctx) owns its device memory and one HIP stream. The Gaussian map
+ * and the current frame are either copied into context-owned HBM (FVGS_MEM_HOST pointers) or
+ * referenced in place (FVGS_MEM_DEVICE pointers, zero copy: the caller keeps them alive and
+ * unchanged while the context uses them). Compute calls leave their results in HBM; fvgs_read()
+ * copies one result to host memory and fvgs_device_ptr() exposes it in place (as the reference's
+ * renderView returns aliases of its internal buffers, GSSlam.cu:796-797).
+ *
+ * Threading. A context is thread-compatible: calls on one context are serialised by an internal
+ * mutex (the reference serialises on optimization_mutex_, GSSlam.cu:239,781,826,984). Distinct
+ * contexts may be used concurrently from different threads.
+ *
+ * Conventions shared with the reference kernels:
+ *  - a pose is the camera-in-world position plus a quaternion float4 that the kernels read as
+ *    (x, y, z, w) (quaternionToMatrix, GSCudaKernels.cu:319-334); p_cam = R^T (p_world - t);
+ *  - intrinsics are (fx, fy, cx, cy) (IntrinsicParameters, RepresentationClasses.hpp:101-105);
+ *  - images are row-major, colour interleaved RGB float32 in [0, 1] (CV_32FC3), depth float32
+ *    metres (CV_32FC1);
+ *  - tiles are 16x16 pixels, tile id = ty * tiles_x + tx (GSCudaKernels.cu:6, GSSlam.cu:432-435).
+ */
+#ifndef FVGS_FVGS_H
+#define FVGS_FVGS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FVGS_ABI_VERSION 1
+
+/* Status codes. */
+#define FVGS_OK 0
+#define FVGS_E_ARG (-1)   /* invalid argument (null pointer, size <= 0, too many tiles, ...) */
+#define FVGS_E_EMPTY (-2) /* nothing to do: no Gaussians, or no tile instances */
+#define FVGS_E_OOM (-3)   /* device allocation failed or memory budget exceeded */
+#define FVGS_E_HIP (-4)   /* a HIP runtime call failed */
+#define FVGS_E_NOTPD (-5) /* JtJ is not positive definite (Gauss-Newton step) */
+#define FVGS_E_STATE (-6) /* call order: e.g. fvgs_forward before fvgs_prepare_rasterization */
+
+/* Memory kind of pointer arguments. */
+#define FVGS_MEM_HOST 0
+#define FVGS_MEM_DEVICE 1
+
+/* Storage precision of the Gaussian attributes (the precision form streams FP16/BF16). */
+#define FVGS_STORE_F32 0
+
+typedef struct fvgs_ctx fvgs_ctx;
+
+/* CameraPose (RepresentationClasses.hpp:107-122) without the CUDA float4 alignment padding:
+ * 28 bytes, orientation read (x, y, z, w). */
+typedef struct fvgs_pose {
+    float position[3];
+    float orientation[4];
+} fvgs_pose;
+
+/* IntrinsicParameters (RepresentationClasses.hpp:101-105). */
+typedef struct fvgs_intrinsics {
+    float f[2];
+    float c[2];
+} fvgs_intrinsics;
+
+/* DeltaGaussian2D (GSCudaKernels.cuh:11-20), 52 bytes. invSigmaImg holds the gradient of the
+ * three stored conic slots as the mapping kernels read them (xx, xy, yy). */
+typedef struct fvgs_delta2d {
+    float meanImg[2];
+    float invSigmaImg[3];
+    float color[3];
+    float depth;
+    float alpha;
+    float pHat[2];
+    uint32_t n;
+} fvgs_delta2d;
+
+/* DeltaGaussian3D (GSCudaKernels.cuh:22-30), 56 bytes. */
+typedef struct fvgs_delta3d {
+    float position[3];
+    float scale[3];
+    float orientation[3];
+    float color[3];
+    float alpha;
+    int32_t n;
+} fvgs_delta3d;
+
+/* PoseOptimizationRgbdData (RepresentationClasses.hpp:249-274), 108 bytes: Jtr then the upper
+ * triangle of JtJ, row-major with r <= c (GSCudaKernels.cu:1155-1165). */
+typedef struct fvgs_pose_data {
+    float Jtr[6];
+    float JtJ[21];
+} fvgs_pose_data;
+
+/* Tunables. fvgs_default_params() reproduces the reference's hard-coded constants. */
+typedef struct fvgs_params {
+    int device;            /* HIP device ordinal (0) */
+    float bg[3];           /* background colour (0,0,0), GSSlam.cu:117 */
+    float alpha_thresh;    /* 0.2  pose pixels need final_T < alpha_thresh, GSSlam.cu:2175 */
+    float color_thresh;    /* 0.1  colour Huber threshold, GSSlam.cu:2176 */
+    float depth_thresh;    /* 0.2  depth Huber threshold, GSSlam.cu:2177 */
+    float depth_weight;    /* 1.0  multiplier on the depth weight (1 reproduces the reference) */
+    float w_depth;         /* 1.0  median-depth weight of the mapping backward, GSSlam.cu:1241 */
+    float w_dist;          /* 0.1  distortion weight of the mapping backward, GSSlam.cu:1242 */
+    float lambda_iso;      /* 0.01 isotropy regulariser of the 2D->3D step, GSSlam.cu:1250 */
+    int cull_emit;         /* 1: a culled Gaussian keeps the zero screen record and its single
+                              tile-0 instance (the reference's first-call behaviour);
+                              0: culled Gaussians emit no instance */
+    int keep_keys;         /* 1: keep the sorted 64-bit keys for fvgs_read(FVGS_OUT_KEYS) */
+    size_t mem_budget;     /* 0 = unlimited; otherwise device bytes the context may allocate */
+} fvgs_params;
+
+/* Result / buffer identifiers for fvgs_read and fvgs_device_ptr. */
+enum fvgs_out {
+    FVGS_OUT_POS2D = 1,        /* float2[N]  projected means */
+    FVGS_OUT_COV2D = 2,        /* float3[N]  2D covariance (xx, yy, xy) */
+    FVGS_OUT_CONIC = 3,        /* float3[N]  inverse covariance as stored (xx, yy, xy) */
+    FVGS_OUT_DEPTH_G = 4,      /* float[N]   camera-space depth */
+    FVGS_OUT_PHAT = 5,         /* float2[N]  (x/z^2, y/z^2) */
+    FVGS_OUT_KEYS = 6,         /* uint64[M]  sorted (tile << 32 | depth bits); needs keep_keys */
+    FVGS_OUT_INDICES = 7,      /* uint32[M]  sorted Gaussian indices */
+    FVGS_OUT_RANGES = 8,       /* uint2[T]   (start, end) per tile */
+    FVGS_OUT_RENDER_RGB = 10,  /* float3[P]  fvgs_render colour */
+    FVGS_OUT_RENDER_DEPTH = 11,/* float[P]   fvgs_render depth (1e10 where no crossing) */
+    FVGS_OUT_RENDER_ALPHA = 12,/* float[P]   fvgs_render accumulated alpha */
+    FVGS_OUT_RGB = 20,         /* float3[P]  fvgs_forward colour (incl. T*bg) */
+    FVGS_OUT_DEPTH = 21,       /* float[P]   fvgs_forward median depth */
+    FVGS_OUT_FINAL_T = 22,     /* float[P]   fvgs_forward final transmittance */
+    FVGS_OUT_N_CONTRIB = 23,   /* uint32[P]  index+1 of the last contributor */
+    FVGS_OUT_COLOR_ERROR = 24, /* float3[P]  colour - observed */
+    FVGS_OUT_MAX_CONTRIB = 25, /* uint32[T]  per-tile max of n_contrib */
+    FVGS_OUT_DELTA2D = 30,     /* fvgs_delta2d[N] */
+    FVGS_OUT_DELTA3D = 31,     /* fvgs_delta3d[N] */
+    FVGS_OUT_POSE_TILES = 40,  /* fvgs_pose_data[T] per-tile partials of the last pose call */
+    FVGS_OUT_TRACK_RGB = 41,   /* float3[P]  pass-1 render of the last pose call */
+    FVGS_OUT_TRACK_DEPTH = 42  /* float[P]   pass-1 median depth of the last pose call */
+};
+
+int fvgs_default_params(fvgs_params* params);
+int fvgs_create(const fvgs_params* params, fvgs_ctx** out_ctx);
+int fvgs_destroy(fvgs_ctx* ctx);
+const char* fvgs_last_error(const fvgs_ctx* ctx);
+int fvgs_abi_version(void);
+
+/* Map upload. Replaces GSSlam's Gaussians SoA (RepresentationClasses.hpp:53-99,
+ * GSSlam.cuh:185): positions float3, scales float3 (sigma per axis), orientations float4 read
+ * (x, y, z, w), colours float3 in [0, 1], opacities float (linear). */
+int fvgs_set_gaussians(fvgs_ctx* ctx, uint32_t n, const float* pos3, const float* scale3,
+                       const float* quat4, const float* color3, const float* opacity, int mem);
+int fvgs_gaussian_count(const fvgs_ctx* ctx, uint32_t* n);
+
+/* Current frame: observed colour (float3 per pixel), observed depth, and the Sobel gradients of
+ * the observed colour used by tracking (gx, gy may be NULL for mapping-only frames). Replaces the
+ * GpuMats GSSlam keeps per pyramid level (GSSlam.cu:1913-1969) and KeyframeData
+ * (RepresentationClasses.hpp:128-230). */
+int fvgs_set_frame(fvgs_ctx* ctx, int width, int height, const float* rgb3, const float* depth,
+                   const float* gx3, const float* gy3, int mem);
+
+/* Projection + binning of the map for one view. Replaces GSSlam::prepareRasterization
+ * (GSSlam.cu:408-646): projectGaussiansWorldToScreen_kernel, countTilesPerGaussian_kernel,
+ * exclusive scan, generateTileHashes_kernel, sort_by_key, computeIndicesRanges_kernel.
+ * n_instances (may be NULL) receives M. Returns FVGS_E_EMPTY when N == 0 or M == 0. */
+int fvgs_prepare_rasterization(fvgs_ctx* ctx, const fvgs_pose* pose, const fvgs_intrinsics* intr,
+                               int width, int height, uint32_t* n_instances);
+
+/* Render-only blend of the prepared view. Replaces forwardPassTileKernel via GSSlam::rasterize /
+ * renderView (GSCudaKernels.cu:710-859, GSSlam.cu:648-799); outputs FP32 RGB (not CV_8UC3),
+ * depth (1e10 without a 0.5 crossing) and accumulated alpha. */
+int fvgs_render(fvgs_ctx* ctx);
+
+/* Mapping forward of the prepared view against the current frame. Replaces
+ * optimizeGaussiansForwardPass (GSCudaKernels.cu:2092-2263, launched at GSSlam.cu:1192). */
+int fvgs_forward(fvgs_ctx* ctx);
+
+/* Mapping backward: 2D gradient accumulation into fvgs_delta2d[N] (zeroed first). Replaces
+ * optimizeGaussiansPerGaussianPass (GSCudaKernels.cu:2265-2502, GSSlam.cu:1221-1246). */
+int fvgs_backward(fvgs_ctx* ctx);
+
+/* 2D -> 3D gradients for the prepared pose. Replaces computeDeltaGaussians3D_kernel
+ * (GSCudaKernels.cu:2504-2720, GSSlam.cu:1251-1264). */
+int fvgs_gradients_3d(fvgs_ctx* ctx);
+
+/* One keyframe's mapping gradient step without the Adam update: prepare + forward + backward +
+ * fvgs_gradients_3d. Replaces GSSlam::optimizeGaussiansKeyframe minus updateGaussiansParametersAdam_kernel
+ * (GSSlam.cu:1083-1264). */
+int fvgs_mapping_step(fvgs_ctx* ctx, const fvgs_pose* pose, const fvgs_intrinsics* intr);
+
+/* RGB-D pose Jacobians at a kernel-frame pose against the current frame (which needs gx, gy):
+ * prepare + getRgbdPoseJacobians (GSCudaKernels.cu:922-1392) + an FP64 reduction of the per-tile
+ * partials in tile order (the reference sums them in FP32 on the host, GSSlam.cu:2205-2216).
+ * jtj21 receives the upper triangle (row-major, r <= c), jtr6 the kernel-convention sum (the
+ * reference's computeRgbdPoseJacobians returns its negation, GSSlam.cu:2225). */
+int fvgs_pose_jacobians(fvgs_ctx* ctx, const fvgs_pose* pose, const fvgs_intrinsics* intr,
+                        double jtj21[21], double jtr6[6]);
+
+/* Pass-1 render of getRgbdPoseJacobians (GSCudaKernels.cu:978-1036) at a pose, with every pixel
+ * treated as having valid depth: colour (incl. T*bg) and median depth into FVGS_OUT_TRACK_RGB /
+ * FVGS_OUT_TRACK_DEPTH. A frame rendered this way and used as the observation makes Jtr vanish
+ * at that pose. No counterpart call in the reference (it renders tracking targets only inside
+ * the Jacobian kernel). */
+int fvgs_render_tracking(fvgs_ctx* ctx, const fvgs_pose* pose, const fvgs_intrinsics* intr,
+                         int width, int height);
+
+/* 3x3 Sobel gradients (scale 1, BORDER_REFLECT_101) of the current frame's colour, computed on
+ * the device and attached to the frame as its gx / gy. Replaces the host cv::Sobel calls of
+ * GSSlam::initAndCopyImgs (GSSlam.cu:1959-1962). */
+int fvgs_compute_gradients(fvgs_ctx* ctx);
+
+/* FP64 Gauss-Newton step: Cholesky solve of JtJ * d = jtr6, then the reference's
+ * PoseLocalParameterization::Plus (PoseLocalParameterization.cpp:22-44) on pose7 =
+ * (px, py, pz, qx, qy, qz, qw): p += d[0:3], q = normalize(q * [1, d[3:6] / 2]).
+ * Replaces the Ceres solve of RgbdPoseCostFunction (RgbdPoseCost.cu:29-109, GSSlam.cu:2246-2297)
+ * for a purely visual problem. delta6 may be NULL. Returns FVGS_E_NOTPD if JtJ is not SPD. */
+int fvgs_gn_step(const double jtj21[21], const double jtr6[6], double pose7[7], double delta6[6]);
+
+/* iters Gauss-Newton iterations of fvgs_pose_jacobians + fvgs_gn_step from pose7 (updated in
+ * place). step_ms (may be NULL) receives the wall time of each iteration in milliseconds. */
+int fvgs_track(fvgs_ctx* ctx, const fvgs_intrinsics* intr, int iters, double pose7[7],
+               double* step_ms);
+
+/* Results. fvgs_read copies min(bytes, size) bytes to host memory; fvgs_device_ptr returns the
+ * device address and size of the buffer (valid until the next call that reallocates it). */
+int fvgs_read(fvgs_ctx* ctx, int what, void* host, size_t bytes);
+int fvgs_device_ptr(fvgs_ctx* ctx, int what, void** ptr, size_t* bytes);
+int fvgs_sync(fvgs_ctx* ctx);
+
+/* Last view's counters: instances M, walked entries K = sum of max_contrib (after fvgs_forward),
+ * tiles T, pixels P. Any pointer may be NULL. */
+int fvgs_stats(fvgs_ctx* ctx, uint64_t* m, uint64_t* k, uint32_t* tiles, uint32_t* pixels);
+
+/* Device time of the stages of the last fvgs_mapping_step in milliseconds, measured with HIP
+ * events on the context's stream: [0] projection + binning (including the host read of M),
+ * [1] clearing the 2D gradient buffer, [2] the fused forward + backward blend, [3] the 2D->3D
+ * gradients. [0] is also set by fvgs_prepare_rasterization. Enable with fvgs_set_timing. */
+int fvgs_set_timing(fvgs_ctx* ctx, int enable);
+int fvgs_last_timing(fvgs_ctx* ctx, float ms[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FVGS_FVGS_H */

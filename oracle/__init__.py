@@ -1,0 +1,230 @@
+"""The oracle — TEST INFRASTRUCTURE ONLY.
+
+A CPU restatement (C++17, FP64 value mode + FP32 binning mode) of the reference's hot path,
+loaded with ctypes. Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg may import this package, and only as the checker / the timed CPU baseline.
+The product library never calls it.
+
+Each wrapper names the reference lines it follows; the C++ source (``fvgs_oracle.cpp``) carries
+the transcription. Parity status of every function is listed in the oracle section of the
+design notes ("parity unpinned" where no closed form, invariant or finite difference pins it).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+TILE = 16
+BLOCK = 256
+BUCKET = 32
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile (g++ -O3 -fopenmp, no contraction)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []), check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _lib.orc_count_tiles.restype = ctypes.c_int64
+        _lib.orc_bin.restype = ctypes.c_int
+        _lib.orc_gn_step.restype = ctypes.c_int
+        _lib.orc_num_threads.restype = ctypes.c_int
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def tiles(width, height):
+    return (width + TILE - 1) // TILE, (height + TILE - 1) // TILE
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())
+
+
+# ------------------------------------------------------------------------------------------
+# Binning mode (FP32, bit-exact target)
+# ------------------------------------------------------------------------------------------
+def project_f32(pos, scale, quat, cam_pos, cam_quat, intr, width, height):
+    """A5 (GSCudaKernels.cu:345-476) in FP32, reference operation order; zero record when culled."""
+    n = len(pos)
+    pos, scale, quat = _f32(pos), _f32(scale), _f32(quat)
+    cp, cq, it = _f32(cam_pos), _f32(cam_quat), _f32(intr)
+    out = dict(pos2d=np.zeros((n, 2), np.float32), cov2d=np.zeros((n, 3), np.float32),
+               conic=np.zeros((n, 3), np.float32), depth=np.zeros(n, np.float32),
+               phat=np.zeros((n, 2), np.float32))
+    lib().orc_project_f32(ctypes.c_int(n), _p(pos), _p(scale), _p(quat), _p(cp), _p(cq), _p(it),
+                          ctypes.c_int(width), ctypes.c_int(height), _p(out["pos2d"]),
+                          _p(out["cov2d"]), _p(out["conic"]), _p(out["depth"]), _p(out["phat"]))
+    return out
+
+
+def bin_f32(proj, width, height):
+    """A6-A10 (GSCudaKernels.cu:549-707, GSSlam.cu:531-641): counts, keys, sorted idx, ranges."""
+    n = len(proj["depth"])
+    counts = np.zeros(n, np.uint32)
+    m = lib().orc_count_tiles(ctypes.c_int(n), _p(proj["pos2d"]), _p(proj["cov2d"]),
+                              ctypes.c_int(width), ctypes.c_int(height), _p(counts))
+    ntx, nty = tiles(width, height)
+    keys = np.zeros(m, np.uint64)
+    idx = np.zeros(m, np.uint32)
+    ranges = np.zeros((ntx * nty, 2), np.uint32)
+    rc = lib().orc_bin(ctypes.c_int(n), _p(proj["pos2d"]), _p(proj["cov2d"]), _p(proj["depth"]),
+                       _p(proj["phat"]), ctypes.c_int(width), ctypes.c_int(height), _p(counts),
+                       ctypes.c_int64(m), _p(keys), _p(idx), _p(ranges))
+    assert rc == 0
+    return dict(counts=counts, keys=keys, idx=idx, ranges=ranges, n_instances=int(m))
+
+
+# ------------------------------------------------------------------------------------------
+# Value mode (FP64)
+# ------------------------------------------------------------------------------------------
+def project_f64(pos, scale, quat, cam_pos, cam_quat, intr, width, height):
+    """A5 in FP64 over the FP32 inputs (value-mode screen-space records)."""
+    n = len(pos)
+    pos, scale, quat = _f32(pos), _f32(scale), _f32(quat)
+    cp, cq, it = _f32(cam_pos), _f32(cam_quat), _f32(intr)
+    out = dict(pos2d=np.zeros((n, 2)), cov2d=np.zeros((n, 3)), conic=np.zeros((n, 3)),
+               depth=np.zeros(n), phat=np.zeros((n, 2)))
+    lib().orc_project_f64(ctypes.c_int(n), _p(pos), _p(scale), _p(quat), _p(cp), _p(cq), _p(it),
+                          ctypes.c_int(width), ctypes.c_int(height), _p(out["pos2d"]),
+                          _p(out["cov2d"]), _p(out["conic"]), _p(out["depth"]), _p(out["phat"]))
+    return out
+
+
+def render(width, height, binning, rec, colors, opac):
+    """A12 render-only forward (GSCudaKernels.cu:710-859): rgb, depth (default 1e10), alpha."""
+    p = width * height
+    rgb, depth, alpha = np.zeros((p, 3)), np.zeros(p), np.zeros(p)
+    lib().orc_render(ctypes.c_int(width), ctypes.c_int(height), _p(binning["ranges"]),
+                     _p(binning["idx"]), _p(_f64(rec["pos2d"])), _p(_f64(rec["cov2d"])),
+                     _p(_f64(rec["conic"])), _p(_f64(rec["depth"])), _p(_f64(rec["phat"])),
+                     _p(_f32(colors)), _p(_f32(opac)), _p(rgb), _p(depth), _p(alpha))
+    return dict(rgb=rgb.reshape(height, width, 3), depth=depth.reshape(height, width),
+                alpha=alpha.reshape(height, width))
+
+
+def forward(width, height, binning, rec, colors, opac, obs_rgb, bg=(0.0, 0.0, 0.0)):
+    """A14 mapping forward (GSCudaKernels.cu:2092-2263) with bucket checkpoints for A15."""
+    p = width * height
+    ntx, nty = tiles(width, height)
+    nt = ntx * nty
+    out = dict(rgb=np.zeros((p, 3)), depth=np.zeros(p), final_T=np.zeros(p),
+               n_contrib=np.zeros(p, np.uint32), color_error=np.zeros((p, 3)),
+               max_contrib=np.zeros(nt, np.uint32),
+               ck_T=np.zeros(nt * 8 * BLOCK), ck_C=np.zeros(nt * 8 * BLOCK * 3))
+    r = {k: _f64(rec[k]) for k in ("pos2d", "conic", "depth", "phat")}
+    lib().orc_forward(ctypes.c_int(width), ctypes.c_int(height), _p(binning["ranges"]),
+                      _p(binning["idx"]), _p(r["pos2d"]), _p(r["conic"]), _p(r["depth"]),
+                      _p(r["phat"]), _p(_f32(colors)), _p(_f32(opac)), _p(_f32(obs_rgb)),
+                      _p(_f32(bg)), _p(out["rgb"]), _p(out["depth"]), _p(out["final_T"]),
+                      _p(out["n_contrib"]), _p(out["color_error"]), _p(out["max_contrib"]),
+                      _p(out["ck_T"]), _p(out["ck_C"]))
+    return out
+
+
+def backward(width, height, binning, rec, colors, opac, fwd, obs_depth, w_depth=1.0, w_dist=0.1):
+    """A15 mapping backward (GSCudaKernels.cu:2265-2502). Returns (N, 13) deltas:
+    meanImg(2) invSigmaImg(3) color(3) depth alpha pHat(2) n."""
+    n = len(opac)
+    delta = np.zeros((n, 13))
+    r = {k: _f64(rec[k]) for k in ("pos2d", "conic", "depth", "phat")}
+    lib().orc_backward(ctypes.c_int(width), ctypes.c_int(height), ctypes.c_int(n),
+                       _p(binning["ranges"]), _p(binning["idx"]), _p(r["pos2d"]), _p(r["conic"]),
+                       _p(r["depth"]), _p(r["phat"]), _p(_f32(colors)), _p(_f32(opac)),
+                       _p(fwd["rgb"]), _p(fwd["depth"]), _p(fwd["n_contrib"]),
+                       _p(fwd["color_error"]), _p(_f32(obs_depth)), _p(fwd["max_contrib"]),
+                       _p(fwd["ck_T"]), _p(fwd["ck_C"]), ctypes.c_double(w_depth),
+                       ctypes.c_double(w_dist), _p(delta))
+    return delta
+
+
+def delta3d(pos, scale, quat, cam_pos, cam_quat, intr, delta2d, lambda_iso=0.01):
+    """A16 (GSCudaKernels.cu:2504-2720). Returns (N, 16): position(3) scale(3) orientation(3)
+    color(3) alpha n pad pad."""
+    n = len(pos)
+    out = np.zeros((n, 16))
+    lib().orc_delta3d(ctypes.c_int(n), _p(_f32(pos)), _p(_f32(scale)), _p(_f32(quat)),
+                      _p(_f32(cam_pos)), _p(_f32(cam_quat)), _p(_f32(intr)),
+                      ctypes.c_double(lambda_iso), _p(_f64(delta2d)), _p(out))
+    return out
+
+
+def pose_jacobians(width, height, binning, rec, colors, opac, gx, gy, obs_rgb, obs_depth,
+                   cam_quat, intr, bg=(0.0, 0.0, 0.0), alpha_thresh=0.2, color_thresh=0.1,
+                   depth_thresh=0.2, depth_weight=1.0, want_render=False):
+    """A18 (GSCudaKernels.cu:922-1392) per tile and summed in tile order.
+    Returns (tiles (T, 27), total (27,)[, pass-1 render dict]): Jtr(6) then JtJ upper (21)."""
+    ntx, nty = tiles(width, height)
+    t = np.zeros((ntx * nty, 27))
+    tot = np.zeros(27)
+    p = width * height
+    rd = dict(rgb=np.zeros((p, 3)), depth=np.zeros(p), final_T=np.ones(p),
+              n_contrib=np.zeros(p, np.uint32)) if want_render else None
+    r = {k: _f64(rec[k]) for k in ("pos2d", "conic", "depth", "phat")}
+    lib().orc_pose_jacobians(ctypes.c_int(width), ctypes.c_int(height), _p(binning["ranges"]),
+                             _p(binning["idx"]), _p(r["pos2d"]), _p(r["conic"]), _p(r["depth"]),
+                             _p(r["phat"]), _p(_f32(colors)), _p(_f32(opac)), _p(_f32(gx)),
+                             _p(_f32(gy)), _p(_f32(obs_rgb)), _p(_f32(obs_depth)),
+                             _p(_f32(cam_quat)), _p(_f32(intr)), _p(_f32(bg)),
+                             ctypes.c_double(alpha_thresh), ctypes.c_double(color_thresh),
+                             ctypes.c_double(depth_thresh), ctypes.c_double(depth_weight),
+                             _p(t), _p(tot), *(
+                                 [_p(rd["rgb"]), _p(rd["depth"]), _p(rd["final_T"]),
+                                  _p(rd["n_contrib"])] if rd else [None] * 4))
+    if rd is not None:
+        return t, tot, rd
+    return t, tot
+
+
+def gn_step(jtj21, jtr6, pose7):
+    """A21: FP64 Cholesky solve of JtJ d = Jtr + PoseLocalParameterization::Plus.
+    Returns (rc, new_pose7, delta6)."""
+    pose = _f64(pose7).copy()
+    d = np.zeros(6)
+    rc = lib().orc_gn_step(_p(_f64(jtj21)), _p(_f64(jtr6)), _p(pose), _p(d))
+    return rc, pose, d
+
+
+def sobel3(img):
+    """3x3 Sobel (scale 1, BORDER_REFLECT_101) of an (H, W, 3) float image: gx, gy."""
+    h, w, _ = img.shape
+    im = _f32(img)
+    gx = np.zeros((h, w, 3), np.float32)
+    gy = np.zeros((h, w, 3), np.float32)
+    lib().orc_sobel3(ctypes.c_int(w), ctypes.c_int(h), _p(im), _p(gx), _p(gy))
+    return gx, gy
+
+
+def pipeline(scene, cam_pos, cam_quat, intr, width, height):
+    """Binning (FP32) + value-mode records (FP64) for one view."""
+    pf = project_f32(scene["pos"], scene["scale"], scene["quat"], cam_pos, cam_quat, intr,
+                     width, height)
+    b = bin_f32(pf, width, height)
+    rec = project_f64(scene["pos"], scene["scale"], scene["quat"], cam_pos, cam_quat, intr,
+                      width, height)
+    return pf, b, rec

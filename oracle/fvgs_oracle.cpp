@@ -1,0 +1,980 @@
+// fvgs oracle — TEST INFRASTRUCTURE ONLY.
+//
+// A CPU restatement of the reference's hot path (f_vigs_slam GSCudaKernels.cu / GSSlam.cu /
+// RgbdPoseCost.cu / PoseLocalParameterization.cpp), used only by tests/, by
+// __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker and the timed
+// CPU baseline. Nothing in the product library links or calls it.
+//
+// Two coupled modes:
+//   * binning mode (orc_project_f32, orc_bin): FP32 arithmetic in the reference kernel's
+//     operation order, built with -ffp-contract=off, so tile lists and sorted keys can be
+//     compared bit-exact with the GPU;
+//   * value mode (everything else): FP64 arithmetic over the FP32 tile lists, with the
+//     reference's FP32 threshold constants widened exactly to double.
+//
+// Reference defects are handled as follows (the notes are Q-numbered in the design notes):
+//   Q1  culled Gaussians get the zero screen-space record (first-call behaviour of the reference).
+//   Q2  keys carry raw float bits; negative depths sort after positive ones.
+//   Q4  ranges cover every instance (the reference's launch covers only ceil(T/256)*256 of them).
+//   Q7/Q12 every pixel sees the whole sorted list (the reference's divergent barriers are UB).
+//   Q8  the mapping backward and the pose pass 2 have no v<=0 test (transcribed).
+//   Q9  the backward uses dx = mean - pixel (transcribed).
+//   Q10 only the first 256 entries of each tile list receive gradients (transcribed).
+//   Q11 gradient-free Gaussians get an all-zero DeltaGaussian3D.
+//   Q14 the projection stores the conic as (inv_xx, inv_yy, inv_xy) but the mapping forward,
+//       backward and pose kernels read it as (xx, xy, yy): v = c0*dx*dx + 2*c1*dx*dy + c2*dy*dy
+//       (GSCudaKernels.cu:448 vs 1003, 2188, 2414). Transcribed.
+//
+// Parity status of each function is recorded in the oracle section of the design notes; the
+// functions pinned only by transcription say so at their definition.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <omp.h>
+
+#define ORC_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+constexpr int kTile = 16;        // GSCudaKernels.cu:6
+constexpr int kBlock = 256;      // GSCudaKernels.cu:7
+constexpr int kBucket = 32;      // GSCudaKernels.cu:2087
+
+// ---------------------------------------------------------------------------------------------
+// Binning mode: FP32, reference operation order.
+// ---------------------------------------------------------------------------------------------
+
+// GSCudaKernels.cu:319-334 — float4 read as (x, y, z, w).
+static void quat_to_mat_f(const float* q, float R[3][3]) {
+    float x = q[0], y = q[1], z = q[2], w = q[3];
+    R[0][0] = 1.0f - 2.0f * (y * y + z * z);
+    R[0][1] = 2.0f * (x * y - w * z);
+    R[0][2] = 2.0f * (x * z + w * y);
+    R[1][0] = 2.0f * (x * y + w * z);
+    R[1][1] = 1.0f - 2.0f * (x * x + z * z);
+    R[1][2] = 2.0f * (y * z - w * x);
+    R[2][0] = 2.0f * (x * z - w * y);
+    R[2][1] = 2.0f * (y * z + w * x);
+    R[2][2] = 1.0f - 2.0f * (x * x + y * y);
+}
+
+// GSCudaKernels.cu:1411-1437.
+static void invert2x2_f(float xx, float yy, float xy, float& ixx, float& iyy, float& ixy) {
+    float det = xx * yy - xy * xy;
+    if (det == 0.0f) { ixx = 1.0f; iyy = 1.0f; ixy = 0.0f; return; }
+    float inv_det = 1.0f / det;
+    ixx = yy * inv_det;
+    iyy = xx * inv_det;
+    ixy = -xy * inv_det;
+}
+
+// GSCudaKernels.cu:345-476 (projectGaussiansWorldToScreen_kernel), one Gaussian.
+// Returns false when culled; the caller writes the zero record (Q1).
+static bool project_one_f(const float* pw, const float* sc, const float* qg, const float* cam_p,
+                          const float* cam_q, const float* intr, int width, int height,
+                          float* p2, float* cov, float* con, float* dep, float* ph) {
+    float d0 = pw[0] - cam_p[0], d1 = pw[1] - cam_p[1], d2 = pw[2] - cam_p[2];
+    float Rc[3][3];
+    quat_to_mat_f(cam_q, Rc);
+    float xc = Rc[0][0] * d0 + Rc[1][0] * d1 + Rc[2][0] * d2;
+    float yc = Rc[0][1] * d0 + Rc[1][1] * d1 + Rc[2][1] * d2;
+    float zc = Rc[0][2] * d0 + Rc[1][2] * d1 + Rc[2][2] * d2;
+    if (zc <= 0.0f) return false;
+    float inv_z = 1.0f / zc;
+    float u = intr[0] * (xc * inv_z) + intr[2];
+    float v = intr[1] * (yc * inv_z) + intr[3];
+    if (u < 0.0f || u >= (float)width || v < 0.0f || v >= (float)height) return false;
+    p2[0] = u; p2[1] = v; dep[0] = zc;
+
+    float Rg[3][3];
+    quat_to_mat_f(qg, Rg);
+    float M[3][3];
+    for (int i = 0; i < 3; i++) {
+        M[i][0] = Rg[i][0] * sc[0];
+        M[i][1] = Rg[i][1] * sc[1];
+        M[i][2] = Rg[i][2] * sc[2];
+    }
+    float S[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = i; j < 3; j++) {
+            S[i][j] = M[i][0] * M[j][0] + M[i][1] * M[j][1] + M[i][2] * M[j][2];
+            if (i != j) S[j][i] = S[i][j];
+        }
+    float Sc[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            Sc[i][j] = 0.0f;
+            for (int k = 0; k < 3; k++)
+                for (int l = 0; l < 3; l++) Sc[i][j] += Rc[k][i] * S[k][l] * Rc[l][j];
+        }
+    float J[2][3];
+    J[0][0] = intr[0] * inv_z; J[0][1] = 0.0f; J[0][2] = -intr[0] * xc * inv_z * inv_z;
+    J[1][0] = 0.0f; J[1][1] = intr[1] * inv_z; J[1][2] = -intr[1] * yc * inv_z * inv_z;
+    float S2[2][2];
+    for (int i = 0; i < 2; i++)
+        for (int j = i; j < 2; j++) {
+            S2[i][j] = 0.0f;
+            for (int k = 0; k < 3; k++)
+                for (int l = 0; l < 3; l++) S2[i][j] += J[i][k] * Sc[k][l] * J[j][l];
+            if (i != j) S2[j][i] = S2[i][j];
+        }
+    cov[0] = S2[0][0]; cov[1] = S2[1][1]; cov[2] = S2[0][1];
+    invert2x2_f(S2[0][0], S2[1][1], S2[0][1], con[0], con[1], con[2]);
+    ph[0] = xc * inv_z * inv_z;
+    ph[1] = yc * inv_z * inv_z;
+    return true;
+}
+
+struct TileRect { int x0, x1, y0, y1; };
+
+// GSCudaKernels.cu:549-593 / 596-640 (shared bbox logic of count and hash kernels), radius 3 sigma.
+static TileRect tile_rect_f(const float* p2, const float* cov, int width, int height, int ntx, int nty) {
+    float sx = std::sqrt(cov[0]);
+    float sy = std::sqrt(cov[1]);
+    float rx = 3.0f * sx, ry = 3.0f * sy;
+    float bminx = std::fmax(0.0f, p2[0] - rx);
+    float bmaxx = std::fmin((float)(width - 1), p2[0] + rx);
+    float bminy = std::fmax(0.0f, p2[1] - ry);
+    float bmaxy = std::fmin((float)(height - 1), p2[1] + ry);
+    TileRect r;
+    r.x0 = (int)(bminx / kTile);
+    r.x1 = (int)(bmaxx / kTile);
+    r.y0 = (int)(bminy / kTile);
+    r.y1 = (int)(bmaxy / kTile);
+    r.x0 = std::max(0, std::min(r.x0, ntx - 1));
+    r.x1 = std::max(0, std::min(r.x1, ntx - 1));
+    r.y0 = std::max(0, std::min(r.y0, nty - 1));
+    r.y1 = std::max(0, std::min(r.y1, nty - 1));
+    return r;
+}
+
+// GSCudaKernels.cu:641-666 — depth at the tile centre, raw float bits in the low word.
+static uint32_t tile_depth_bits_f(const float* p2, float z, const float* ph, int tx, int ty) {
+    float tcx = (tx + 0.5f) * kTile;
+    float tcy = (ty + 0.5f) * kTile;
+    float dx = tcx - p2[0];
+    float dy = tcy - p2[1];
+    float d = z + ph[0] * dx + ph[1] * dy;
+    uint32_t bits;
+    std::memcpy(&bits, &d, 4);
+    return bits;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Value mode helpers: FP64.
+// ---------------------------------------------------------------------------------------------
+static void quat_to_mat_d(const double* q, double R[3][3]) {
+    double x = q[0], y = q[1], z = q[2], w = q[3];
+    R[0][0] = 1.0 - 2.0 * (y * y + z * z);
+    R[0][1] = 2.0 * (x * y - w * z);
+    R[0][2] = 2.0 * (x * z + w * y);
+    R[1][0] = 2.0 * (x * y + w * z);
+    R[1][1] = 1.0 - 2.0 * (x * x + z * z);
+    R[1][2] = 2.0 * (y * z - w * x);
+    R[2][0] = 2.0 * (x * z - w * y);
+    R[2][1] = 2.0 * (y * z + w * x);
+    R[2][2] = 1.0 - 2.0 * (x * x + y * y);
+}
+
+// FP32 thresholds of the reference, widened exactly.
+static const double kAlphaMin = (double)(1.0f / 255.0f);   // GSCudaKernels.cu:2189, 974
+static const double kAlphaMax = (double)0.99f;             // :2190
+static const double kTStop = (double)0.0001f;               // :2210, 1013
+static const double kPass2Stop = (double)0.001f;            // :1340
+static const double kDepthValid = (double)0.1f;             // :953, 1050
+static const double kDepthValid2 = (double)0.5f;            // :1271
+
+// One screen-space record in value mode (the fields the blend kernels read).
+struct Rec {
+    double u, v;        // positions_2d
+    double c0, c1, c2;  // inverse covariance as stored: (inv_xx, inv_yy, inv_xy)
+    double z;           // depths
+    double px, py;      // p_hats
+    double r, g, b;     // colours
+    double o;           // opacity
+    double k0, k1, k2;  // covariance (xx, yy, xy), used by the render-only kernel
+};
+
+static inline Rec load_rec(const double* p2, const double* con, const double* dep, const double* ph,
+                           const double* col, const double* op, const double* cov, uint32_t g) {
+    Rec r;
+    r.u = p2[2 * g]; r.v = p2[2 * g + 1];
+    r.c0 = con[3 * g]; r.c1 = con[3 * g + 1]; r.c2 = con[3 * g + 2];
+    r.z = dep[g]; r.px = ph[2 * g]; r.py = ph[2 * g + 1];
+    r.r = col[3 * g]; r.g = col[3 * g + 1]; r.b = col[3 * g + 2];
+    r.o = op[g];
+    if (cov) { r.k0 = cov[3 * g]; r.k1 = cov[3 * g + 1]; r.k2 = cov[3 * g + 2]; }
+    else { r.k0 = r.k1 = r.k2 = 0.0; }
+    return r;
+}
+
+}  // namespace
+
+// =============================================================================================
+// Binning mode API.
+// =============================================================================================
+
+// A5 in FP32 (GSCudaKernels.cu:345-476). Culled Gaussians get the zero record (Q1).
+ORC_API void orc_project_f32(int n, const float* pos, const float* scale, const float* quat,
+                             const float* cam_pos, const float* cam_quat, const float* intr,
+                             int width, int height, float* pos2d, float* cov2d, float* conic,
+                             float* depth, float* phat) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        float* p2 = pos2d + 2 * i; float* cv = cov2d + 3 * i; float* cn = conic + 3 * i;
+        float* dp = depth + i; float* ph = phat + 2 * i;
+        if (!project_one_f(pos + 3 * i, scale + 3 * i, quat + 4 * i, cam_pos, cam_quat, intr,
+                           width, height, p2, cv, cn, dp, ph)) {
+            p2[0] = p2[1] = 0.0f; cv[0] = cv[1] = cv[2] = 0.0f; cn[0] = cn[1] = cn[2] = 0.0f;
+            dp[0] = 0.0f; ph[0] = ph[1] = 0.0f;
+        }
+    }
+}
+
+// A6 + A7: per-Gaussian tile counts and the instance total (GSCudaKernels.cu:549-593,
+// GSSlam.cu:556-572).
+ORC_API int64_t orc_count_tiles(int n, const float* pos2d, const float* cov2d, int width, int height,
+                                uint32_t* counts) {
+    int ntx = (width + kTile - 1) / kTile, nty = (height + kTile - 1) / kTile;
+    int64_t total = 0;
+#pragma omp parallel for schedule(static) reduction(+ : total)
+    for (int i = 0; i < n; i++) {
+        TileRect r = tile_rect_f(pos2d + 2 * i, cov2d + 3 * i, width, height, ntx, nty);
+        uint32_t c = (uint32_t)((r.x1 - r.x0 + 1) * (r.y1 - r.y0 + 1));
+        counts[i] = c;
+        total += c;
+    }
+    return total;
+}
+
+// A7-A10: exclusive scan, hash emission, stable sort by the 64-bit key, ranges with the
+// intended semantics (Q3 end written when M==1, Q4 all instances scanned).
+// keys/idx have room for m = orc_count_tiles(...) entries; ranges has 2*T entries.
+ORC_API int orc_bin(int n, const float* pos2d, const float* cov2d, const float* depth,
+                    const float* phat, int width, int height, const uint32_t* counts, int64_t m,
+                    uint64_t* keys, uint32_t* idx, uint32_t* ranges) {
+    int ntx = (width + kTile - 1) / kTile, nty = (height + kTile - 1) / kTile;
+    int ntiles = ntx * nty;
+    std::vector<int64_t> off(n);
+    int64_t acc = 0;
+    for (int i = 0; i < n; i++) { off[i] = acc; acc += counts[i]; }
+    if (acc != m) return -1;
+    std::vector<uint64_t> k(m);
+    std::vector<uint32_t> v(m);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        TileRect r = tile_rect_f(pos2d + 2 * i, cov2d + 3 * i, width, height, ntx, nty);
+        int64_t w = off[i];
+        for (int ty = r.y0; ty <= r.y1; ty++)
+            for (int tx = r.x0; tx <= r.x1; tx++) {
+                uint32_t tile = (uint32_t)(ty * ntx + tx);
+                uint32_t bits = tile_depth_bits_f(pos2d + 2 * i, depth[i], phat + 2 * i, tx, ty);
+                k[w] = ((uint64_t)tile << 32) | bits;
+                v[w] = (uint32_t)i;
+                w++;
+            }
+    }
+    // Stable sort by key: equal keys keep emission (Gaussian index) order, as the reference's
+    // radix sort does (GSSlam.cu:624-628).
+    std::vector<int64_t> perm(m);
+    for (int64_t j = 0; j < m; j++) perm[j] = j;
+    std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) { return k[a] < k[b]; });
+    for (int64_t j = 0; j < m; j++) { keys[j] = k[perm[j]]; idx[j] = v[perm[j]]; }
+    std::memset(ranges, 0, sizeof(uint32_t) * 2 * (size_t)ntiles);
+    for (int64_t j = 0; j < m; j++) {
+        uint32_t t = (uint32_t)(keys[j] >> 32);
+        if (j == 0 || (uint32_t)(keys[j - 1] >> 32) != t) ranges[2 * t] = (uint32_t)j;
+        if (j == m - 1 || (uint32_t)(keys[j + 1] >> 32) != t) ranges[2 * t + 1] = (uint32_t)(j + 1);
+    }
+    return 0;
+}
+
+// =============================================================================================
+// Value mode API (FP64).
+// =============================================================================================
+
+// A5 in FP64 over FP32 inputs. Same cull tests as the FP32 mode (decided in FP64).
+ORC_API void orc_project_f64(int n, const float* pos, const float* scale, const float* quat,
+                             const float* cam_pos, const float* cam_quat, const float* intr,
+                             int width, int height, double* pos2d, double* cov2d, double* conic,
+                             double* depth, double* phat) {
+    double cq[4] = {cam_quat[0], cam_quat[1], cam_quat[2], cam_quat[3]};
+    double Rc[3][3];
+    quat_to_mat_d(cq, Rc);
+    double fx = intr[0], fy = intr[1], cx = intr[2], cy = intr[3];
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        double* p2 = pos2d + 2 * i; double* cv = cov2d + 3 * i; double* cn = conic + 3 * i;
+        double* ph = phat + 2 * i;
+        p2[0] = p2[1] = 0; cv[0] = cv[1] = cv[2] = 0; cn[0] = cn[1] = cn[2] = 0;
+        depth[i] = 0; ph[0] = ph[1] = 0;
+        double d[3] = {(double)pos[3 * i] - cam_pos[0], (double)pos[3 * i + 1] - cam_pos[1],
+                       (double)pos[3 * i + 2] - cam_pos[2]};
+        double pc[3];
+        for (int a = 0; a < 3; a++) pc[a] = Rc[0][a] * d[0] + Rc[1][a] * d[1] + Rc[2][a] * d[2];
+        if (pc[2] <= 0.0) continue;
+        double iz = 1.0 / pc[2];
+        double u = fx * pc[0] * iz + cx, v = fy * pc[1] * iz + cy;
+        if (u < 0.0 || u >= width || v < 0.0 || v >= height) continue;
+        p2[0] = u; p2[1] = v; depth[i] = pc[2];
+        double qg[4] = {quat[4 * i], quat[4 * i + 1], quat[4 * i + 2], quat[4 * i + 3]};
+        double Rg[3][3];
+        quat_to_mat_d(qg, Rg);
+        double s[3] = {scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]};
+        double S[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                double acc = 0;
+                for (int k = 0; k < 3; k++) acc += Rg[a][k] * s[k] * s[k] * Rg[b][k];
+                S[a][b] = acc;
+            }
+        double Sc[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                double acc = 0;
+                for (int k = 0; k < 3; k++)
+                    for (int l = 0; l < 3; l++) acc += Rc[k][a] * S[k][l] * Rc[l][b];
+                Sc[a][b] = acc;
+            }
+        double J[2][3] = {{fx * iz, 0, -fx * pc[0] * iz * iz}, {0, fy * iz, -fy * pc[1] * iz * iz}};
+        double S2[2][2];
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) {
+                double acc = 0;
+                for (int k = 0; k < 3; k++)
+                    for (int l = 0; l < 3; l++) acc += J[a][k] * Sc[k][l] * J[b][l];
+                S2[a][b] = acc;
+            }
+        cv[0] = S2[0][0]; cv[1] = S2[1][1]; cv[2] = S2[0][1];
+        double det = S2[0][0] * S2[1][1] - S2[0][1] * S2[0][1];
+        if (det == 0.0) { cn[0] = 1; cn[1] = 1; cn[2] = 0; }
+        else { cn[0] = S2[1][1] / det; cn[1] = S2[0][0] / det; cn[2] = -S2[0][1] / det; }
+        ph[0] = pc[0] * iz * iz; ph[1] = pc[1] * iz * iz;
+    }
+}
+
+// A12: render-only forward (GSCudaKernels.cu:710-859). No background; depth default 1e10;
+// per-pixel inversion of the covariance (det==0 -> identity); stop once alpha >= 0.99.
+// Outputs rgb[3P], depth[P], alpha[P] (alpha = accumulated opacity, the kernel's pixel_alpha).
+ORC_API void orc_render(int width, int height, const uint32_t* ranges, const uint32_t* idx,
+                        const double* pos2d, const double* cov2d, const double* conic,
+                        const double* depth, const double* phat, const float* colors,
+                        const float* opac, double* out_rgb, double* out_depth, double* out_alpha) {
+    int ntx = (width + kTile - 1) / kTile, nty = (height + kTile - 1) / kTile;
+    std::vector<double> col(3 * 0);
+    (void)col;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int t = 0; t < ntx * nty; t++) {
+        int tx = t % ntx, ty = t / ntx;
+        uint32_t s = ranges[2 * t], e = ranges[2 * t + 1];
+        for (int ly = 0; ly < kTile; ly++)
+            for (int lx = 0; lx < kTile; lx++) {
+                int x = tx * kTile + lx, y = ty * kTile + ly;
+                if (x >= width || y >= height) continue;
+                double C[3] = {0, 0, 0}, A = 0, D = 1e10;
+                bool has_med = false;
+                for (uint32_t j = s; j < e; j++) {
+                    uint32_t g = idx[j];
+                    double cxx = cov2d[3 * g], cyy = cov2d[3 * g + 1], cxy = cov2d[3 * g + 2];
+                    double ixx, iyy, ixy;
+                    double det = cxx * cyy - cxy * cxy;
+                    if (det == 0.0) { ixx = 1; iyy = 1; ixy = 0; }
+                    else { ixx = cyy / det; iyy = cxx / det; ixy = -cxy / det; }
+                    double dx = x - pos2d[2 * g], dy = y - pos2d[2 * g + 1];
+                    double gv = std::exp(-0.5 * (ixx * dx * dx + iyy * dy * dy + 2.0 * ixy * dx * dy));
+                    double wa = opac[g] * gv;
+                    double Tb = 1.0 - A;
+                    C[0] += (1.0 - A) * wa * colors[3 * g];
+                    C[1] += (1.0 - A) * wa * colors[3 * g + 1];
+                    C[2] += (1.0 - A) * wa * colors[3 * g + 2];
+                    A += (1.0 - A) * wa;
+                    double Ta = 1.0 - A;
+                    if (!has_med && Tb > 0.5 && Ta <= 0.5) {
+                        D = depth[g] + phat[2 * g] * dx + phat[2 * g + 1] * dy;
+                        has_med = true;
+                    }
+                    if (A >= kAlphaMax) break;
+                }
+                size_t p = (size_t)y * width + x;
+                out_rgb[3 * p] = C[0]; out_rgb[3 * p + 1] = C[1]; out_rgb[3 * p + 2] = C[2];
+                out_depth[p] = D; out_alpha[p] = A;
+            }
+    }
+    (void)conic;
+}
+
+// A14: mapping forward (GSCudaKernels.cu:2092-2263), intended semantics for Q7.
+// Outputs per pixel: color (incl. T*bg), depth, final_T, n_contrib, color_error; per tile
+// max_contrib. Checkpoints (T, colour) at list index b*32 for b < 8 are written to ck_T[T*8*256]
+// and ck_C[3*T*8*256] when those pointers are non-null (consumed by orc_backward).
+ORC_API void orc_forward(int width, int height, const uint32_t* ranges, const uint32_t* idx,
+                         const double* pos2d, const double* conic, const double* depth,
+                         const double* phat, const float* colors, const float* opac,
+                         const float* obs_rgb, const float* bg, double* out_rgb, double* out_depth,
+                         double* final_T, uint32_t* n_contrib, double* color_err,
+                         uint32_t* max_contrib, double* ck_T, double* ck_C) {
+    int ntx = (width + kTile - 1) / kTile, nty = (height + kTile - 1) / kTile;
+    const int nb = kBlock / kBucket;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int t = 0; t < ntx * nty; t++) {
+        int tx = t % ntx, ty = t / ntx;
+        uint32_t s = ranges[2 * t], e = ranges[2 * t + 1];
+        uint32_t n = e - s;
+        uint32_t tile_max = 0;
+        for (int tid = 0; tid < kBlock; tid++) {
+            int x = tx * kTile + tid % kTile, y = ty * kTile + tid / kTile;
+            bool inside = x < width && y < height;
+            double T = 1.0, C[3] = {0, 0, 0}, D = 0.0;
+            uint32_t contributor = 0, last = 0;
+            bool done = !inside;
+            for (uint32_t j = 0; j < n && !done; j++) {
+                if (j % kBucket == 0 && ck_T && j / kBucket < (uint32_t)nb) {
+                    size_t ci = ((size_t)t * nb + j / kBucket) * kBlock + tid;
+                    ck_T[ci] = T; ck_C[3 * ci] = C[0]; ck_C[3 * ci + 1] = C[1]; ck_C[3 * ci + 2] = C[2];
+                }
+                contributor++;
+                uint32_t g = idx[s + j];
+                double dx = x - pos2d[2 * g], dy = y - pos2d[2 * g + 1];
+                const double* c = conic + 3 * g;
+                double v = c[0] * dx * dx + 2.0 * c[1] * dx * dy + c[2] * dy * dy;   // Q14
+                double a = std::min(kAlphaMax, opac[g] * std::exp(-0.5 * v));
+                if (a < kAlphaMin || v <= 0.0) continue;
+                C[0] += T * a * colors[3 * g];
+                C[1] += T * a * colors[3 * g + 1];
+                C[2] += T * a * colors[3 * g + 2];
+                double tT = T * (1.0 - a);
+                if (T > 0.5 && tT <= 0.5) D = depth[g] + phat[2 * g] * dx + phat[2 * g + 1] * dy;
+                T = tT;
+                last = contributor;
+                if (tT < kTStop) done = true;
+            }
+            if (inside) {
+                size_t p = (size_t)y * width + x;
+                final_T[p] = T;
+                n_contrib[p] = last;
+                for (int k = 0; k < 3; k++) {
+                    double ck = C[k] + T * (double)bg[k];
+                    out_rgb[3 * p + k] = ck;
+                    color_err[3 * p + k] = ck - (double)obs_rgb[3 * p + k];
+                }
+                out_depth[p] = D;
+            }
+            tile_max = std::max(tile_max, last);
+        }
+        max_contrib[t] = tile_max;
+    }
+}
+
+// A15: mapping backward (GSCudaKernels.cu:2265-2502). Per tile, per bucket b (b*32 < max_contrib,
+// b < 8 by Q10), per pixel: replay the 32 splats of the bucket from the forward checkpoint with
+// the backward's rules (no v<=0 test, Q8; dx = mean - pixel, Q9), accumulating the 2D deltas.
+// delta layout per Gaussian (13 doubles): meanImg(2) invSigma(3) color(3) depth alpha pHat(2) n.
+ORC_API void orc_backward(int width, int height, int n_gauss, const uint32_t* ranges,
+                          const uint32_t* idx, const double* pos2d, const double* conic,
+                          const double* depth, const double* phat, const float* colors,
+                          const float* opac, const double* out_rgb, const double* out_depth,
+                          const uint32_t* n_contrib, const double* color_err,
+                          const float* obs_depth, const uint32_t* max_contrib, const double* ck_T,
+                          const double* ck_C, double w_depth, double w_dist, double* delta) {
+    int ntx = (width + kTile - 1) / kTile, nty = (height + kTile - 1) / kTile;
+    int ntiles = ntx * nty;
+    const int nb = kBlock / kBucket;
+    // Per-(tile, slot) partials, reduced afterwards in tile order: deterministic.
+    std::vector<double> part((size_t)ntiles * kBlock * 13, 0.0);
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int t = 0; t < ntiles; t++) {
+        int tx = t % ntx, ty = t / ntx;
+        uint32_t s = ranges[2 * t], e = ranges[2 * t + 1];
+        uint32_t n = e - s;
+        uint32_t nsplat = std::min<uint32_t>(n, kBlock);
+        for (int b = 0; b < nb; b++) {
+            if ((uint32_t)(b * kBucket) >= max_contrib[t]) break;
+            for (int pix = 0; pix < kBlock; pix++) {
+                int x = tx * kTile + pix % kTile, y = ty * kTile + pix / kTile;
+                if (x >= width || y >= height) continue;
+                size_t p = (size_t)y * width + x;
+                size_t ci = ((size_t)t * nb + b) * kBlock + pix;
+                double T = ck_T[ci];
+                double acc[3] = {ck_C[3 * ci], ck_C[3 * ci + 1], ck_C[3 * ci + 2]};
+                const double* C = out_rgb + 3 * p;
+                const double* ce = color_err + 3 * p;
+                double D = out_depth[p];
+                double img_depth = obs_depth[p];
+                uint32_t last = n_contrib[p];
+                for (int l = 0; l < kBucket; l++) {
+                    uint32_t sp = (uint32_t)(b * kBucket + l);
+                    if (sp >= nsplat) continue;
+                    if (sp >= last) continue;
+                    uint32_t g = idx[s + sp];
+                    double dx = pos2d[2 * g] - x, dy = pos2d[2 * g + 1] - y;
+                    const double* c = conic + 3 * g;
+                    double v = c[0] * dx * dx + 2.0 * c[1] * dx * dy + c[2] * dy * dy;  // Q14
+                    double a = std::min(kAlphaMax, opac[g] * std::exp(-0.5 * v));
+                    if (a < kAlphaMin) continue;
+                    double col[3] = {colors[3 * g], colors[3 * g + 1], colors[3 * g + 2]};
+                    double da[3], dl = 0.0;
+                    for (int k = 0; k < 3; k++) {
+                        da[k] = col[k] * T;
+                        acc[k] += da[k] * a;
+                    }
+                    double iom = 1.0 / (1.0 - a);
+                    for (int k = 0; k < 3; k++) {
+                        da[k] -= (C[k] - acc[k]) * iom;
+                        da[k] *= -ce[k];
+                        dl += da[k];
+                    }
+                    double aG = a * dl;
+                    double* d = part.data() + ((size_t)t * kBlock + sp) * 13;
+                    d[12] += 1.0;
+                    d[5] -= a * T * ce[0]; d[6] -= a * T * ce[1]; d[7] -= a * T * ce[2];
+                    d[9] -= aG;
+                    d[0] -= aG * (c[0] * dx + c[1] * dy);
+                    d[1] -= aG * (c[1] * dx + c[2] * dy);
+                    d[2] -= 0.5 * aG * dx * dx;
+                    d[3] -= 0.5 * aG * dx * dy;
+                    d[4] -= 0.5 * aG * dy * dy;
+                    double tT = T * (1.0 - a);
+                    double phx = phat[2 * g], phy = phat[2 * g + 1];
+                    if (T > 0.5 && tT <= 0.5) {
+                        double de = (img_depth > kDepthValid) ? (D - img_depth) : 0.0;
+                        d[8] -= w_depth * de;
+                        d[0] -= w_depth * de * phx; d[1] -= w_depth * de * phy;
+                        d[10] -= w_depth * de * dx; d[11] -= w_depth * de * dy;
+                    }
+                    double di = depth[g] + dx * phx + dy * phy;
+                    double dc = w_dist * a * T * (di - D);
+                    d[8] -= dc;
+                    d[0] -= dc * phx; d[1] -= dc * phy;
+                    d[10] -= dc * dx; d[11] -= dc * dy;
+                    T = tT;
+                }
+            }
+        }
+    }
+    std::memset(delta, 0, sizeof(double) * 13 * (size_t)n_gauss);
+    for (int t = 0; t < ntiles; t++) {
+        uint32_t s = ranges[2 * t], e = ranges[2 * t + 1];
+        uint32_t nsplat = std::min<uint32_t>(e - s, kBlock);
+        for (uint32_t sp = 0; sp < nsplat; sp++) {
+            const double* d = part.data() + ((size_t)t * kBlock + sp) * 13;
+            if (d[12] == 0.0) continue;
+            double* o = delta + 13 * (size_t)idx[s + sp];
+            for (int k = 0; k < 13; k++) o[k] += d[k];
+        }
+    }
+}
+
+// A16: 2D deltas -> 3D deltas (GSCudaKernels.cu:2504-2720). Output per Gaussian (16 doubles):
+// position(3) scale(3) orientation(3) color(3) alpha n pad pad. n==0 -> all zero (Q11).
+ORC_API void orc_delta3d(int n, const float* pos, const float* scale, const float* quat,
+                         const float* cam_pos, const float* cam_quat, const float* intr,
+                         double lambda_iso, const double* delta2d, double* out) {
+    double qc[4] = {cam_quat[0], cam_quat[1], cam_quat[2], cam_quat[3]};
+    double Rq[3][3];   // rotation of q_cw (camera-to-world)
+    quat_to_mat_d(qc, Rq);
+    double fx = intr[0], fy = intr[1];
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        const double* d2 = delta2d + 13 * (size_t)i;
+        double* o = out + 16 * (size_t)i;
+        for (int k = 0; k < 16; k++) o[k] = 0.0;
+        if (d2[12] == 0.0) continue;
+        o[13] = d2[12];
+        double eta = 1.0 / (5.0 + d2[12]);
+        o[9] = d2[5] * eta; o[10] = d2[6] * eta; o[11] = d2[7] * eta;
+        o[12] = d2[9] * eta;
+        double dw[3] = {pos[3 * i] - (double)cam_pos[0], pos[3 * i + 1] - (double)cam_pos[1],
+                        pos[3 * i + 2] - (double)cam_pos[2]};
+        double mu[3];  // q_cw^-1 * (mu_w - c_w) = R^T dw
+        for (int a = 0; a < 3; a++) mu[a] = Rq[0][a] * dw[0] + Rq[1][a] * dw[1] + Rq[2][a] * dw[2];
+        double J[3][3] = {{fx / mu[2], 0.0, -fx * mu[0] / (mu[2] * mu[2])},
+                          {0.0, fy / mu[2], -fy * mu[1] / (mu[2] * mu[2])},
+                          {0.0, 0.0, 1.0}};
+        double Rcw[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) Rcw[a][b] = Rq[b][a];
+        double qg[4] = {quat[4 * i], quat[4 * i + 1], quat[4 * i + 2], quat[4 * i + 3]};
+        double Rg[3][3];
+        quat_to_mat_d(qg, Rg);
+        double s[3] = {scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]};
+        double T[3][3], RS[3][3], M[3][3], Sp[3][3], S3[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                T[a][b] = J[a][0] * Rcw[0][b] + J[a][1] * Rcw[1][b] + J[a][2] * Rcw[2][b];
+                RS[a][b] = Rg[a][b] * s[b];
+            }
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                M[a][b] = T[a][0] * RS[0][b] + T[a][1] * RS[1][b] + T[a][2] * RS[2][b];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                Sp[a][b] = M[a][0] * M[b][0] + M[a][1] * M[b][1] + M[a][2] * M[b][2];
+                S3[a][b] = RS[a][0] * RS[b][0] + RS[a][1] * RS[b][1] + RS[a][2] * RS[b][2];
+            }
+        // 3x3 inverse by cofactors.
+        double Si[3][3];
+        {
+            double c00 = Sp[1][1] * Sp[2][2] - Sp[1][2] * Sp[2][1];
+            double c01 = Sp[1][2] * Sp[2][0] - Sp[1][0] * Sp[2][2];
+            double c02 = Sp[1][0] * Sp[2][1] - Sp[1][1] * Sp[2][0];
+            double det = Sp[0][0] * c00 + Sp[0][1] * c01 + Sp[0][2] * c02;
+            double id = 1.0 / det;
+            Si[0][0] = c00 * id;
+            Si[0][1] = (Sp[0][2] * Sp[2][1] - Sp[0][1] * Sp[2][2]) * id;
+            Si[0][2] = (Sp[0][1] * Sp[1][2] - Sp[0][2] * Sp[1][1]) * id;
+            Si[1][0] = c01 * id;
+            Si[1][1] = (Sp[0][0] * Sp[2][2] - Sp[0][2] * Sp[2][0]) * id;
+            Si[1][2] = (Sp[0][2] * Sp[1][0] - Sp[0][0] * Sp[1][2]) * id;
+            Si[2][0] = c02 * id;
+            Si[2][1] = (Sp[0][1] * Sp[2][0] - Sp[0][0] * Sp[2][1]) * id;
+            Si[2][2] = (Sp[0][0] * Sp[1][1] - Sp[0][1] * Sp[1][0]) * id;
+        }
+        // dL/dmu_3d = q_cw * (J^T dL/dmu_2d + depth * (x/z, y/z, 1))
+        double m2[3] = {d2[0], d2[1], 0.0};
+        double tv[3];
+        for (int a = 0; a < 3; a++) tv[a] = J[0][a] * m2[0] + J[1][a] * m2[1] + J[2][a] * m2[2];
+        tv[0] += d2[8] * (mu[0] / mu[2]);
+        tv[1] += d2[8] * (mu[1] / mu[2]);
+        tv[2] += d2[8] * 1.0;
+        double dmu[3];
+        for (int a = 0; a < 3; a++) dmu[a] = Rq[a][0] * tv[0] + Rq[a][1] * tv[1] + Rq[a][2] * tv[2];
+        double phx = d2[10], phy = d2[11];
+        double n2 = mu[0] * mu[0] + mu[1] * mu[1] + mu[2] * mu[2];
+        double nn = std::sqrt(n2);
+        double in3 = 1.0 / (n2 * nn);
+        double dph[3] = {-mu[0] * mu[2] * in3, -mu[1] * mu[2] * in3, (mu[0] * mu[0] + mu[1] * mu[1]) * in3};
+        double fph = (Si[2][0] * phx + Si[2][1] * phy) / Si[2][2];
+        for (int a = 0; a < 3; a++)
+            dmu[a] += Rq[a][0] * dph[0] * fph + Rq[a][1] * dph[1] * fph + Rq[a][2] * dph[2] * fph;
+        // dL/dSigma' from dL/dSigma'^{-1} (2x2, regularised by 0.001).
+        double sxx = Sp[0][0] + (double)0.001f, sxy = Sp[1][0], syy = Sp[1][1] + (double)0.001f;
+        double den = sxx * syy - sxy * sxy;
+        double d2i = 1.0 / ((den * den) + (double)0.0000001f);
+        double gx = d2[2], gy = d2[3], gz = d2[4];
+        double dSxx = d2i * (-syy * syy * gx + 2 * sxy * syy * gy + (den - sxx * syy) * gz);
+        double dSyy = d2i * (-sxx * sxx * gz + 2 * sxx * sxy * gy + (den - sxx * syy) * gx);
+        double dSxy = d2i * 2 * (sxy * syy * gx - (den + 2 * sxy * sxy) * gy + sxx * sxy * gz);
+        double dT[2][3];
+        for (int b = 0; b < 3; b++) {
+            double r0 = T[0][0] * S3[b][0] + T[0][1] * S3[b][1] + T[0][2] * S3[b][2];
+            double r1 = T[1][0] * S3[b][0] + T[1][1] * S3[b][1] + T[1][2] * S3[b][2];
+            dT[0][b] = 2 * r0 * dSxx + r1 * dSxy;
+            dT[1][b] = 2 * r1 * dSyy + r0 * dSxy;
+        }
+        double zt = mu[2] / nn;
+        double s22 = Si[2][2];
+        double dSI[3][3] = {{0, 0, 0.5 * zt * phx / s22},
+                            {0, 0, 0.5 * zt * phy / s22},
+                            {0.5 * zt * phx / s22, 0.5 * zt * phy / s22,
+                             -zt * (phx * Si[2][0] + phy * Si[2][1]) / (s22 * s22)}};
+        double dSP[3][3], tmp[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                tmp[a][b] = Si[a][0] * dSI[0][b] + Si[a][1] * dSI[1][b] + Si[a][2] * dSI[2][b];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                dSP[a][b] = -(tmp[a][0] * Si[0][b] + tmp[a][1] * Si[1][b] + tmp[a][2] * Si[2][b]);
+        double TS[3][3], ST[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                TS[a][b] = T[a][0] * S3[0][b] + T[a][1] * S3[1][b] + T[a][2] * S3[2][b];
+                ST[a][b] = S3[a][0] * T[b][0] + S3[a][1] * T[b][1] + S3[a][2] * T[b][2];
+            }
+        for (int r = 0; r < 2; r++)
+            for (int b = 0; b < 3; b++) {
+                double c1 = TS[0][b] * dSP[0][r] + TS[1][b] * dSP[1][r] + TS[2][b] * dSP[2][r];
+                double c2 = ST[b][0] * dSP[r][0] + ST[b][1] * dSP[r][1] + ST[b][2] * dSP[r][2];
+                dT[r][b] += c1 + c2;
+            }
+        double dJ00 = Rcw[0][0] * dT[0][0] + Rcw[0][1] * dT[0][1] + Rcw[0][2] * dT[0][2];
+        double dJ02 = Rcw[2][0] * dT[0][0] + Rcw[2][1] * dT[0][1] + Rcw[2][2] * dT[0][2];
+        double dJ11 = Rcw[1][0] * dT[1][0] + Rcw[1][1] * dT[1][1] + Rcw[1][2] * dT[1][2];
+        double dJ12 = Rcw[2][0] * dT[1][0] + Rcw[2][1] * dT[1][1] + Rcw[2][2] * dT[1][2];
+        double iz = 1.0 / mu[2], iz2 = iz * iz, iz3 = iz2 * iz;
+        double dmc[3] = {-fx * iz2 * dJ02, -fy * iz2 * dJ12,
+                         -fx * iz2 * dJ00 - fy * iz2 * dJ11 + (2 * fx * mu[0]) * iz3 * dJ02 +
+                             (2 * fy * mu[1]) * iz3 * dJ12};
+        for (int a = 0; a < 3; a++) dmu[a] += Rq[a][0] * dmc[0] + Rq[a][1] * dmc[1] + Rq[a][2] * dmc[2];
+        o[0] = eta * dmu[0]; o[1] = eta * dmu[1]; o[2] = eta * dmu[2];
+        // dL/dSigma_3d = explicit symmetric form + T^T dSP T
+        double dS3[3][3];
+        dS3[0][0] = T[0][0] * T[0][0] * dSxx + T[0][0] * T[1][0] * dSxy + T[1][0] * T[1][0] * dSyy;
+        dS3[1][1] = T[0][1] * T[0][1] * dSxx + T[0][1] * T[1][1] * dSxy + T[1][1] * T[1][1] * dSyy;
+        dS3[2][2] = T[0][2] * T[0][2] * dSxx + T[0][2] * T[1][2] * dSxy + T[1][2] * T[1][2] * dSyy;
+        dS3[1][0] = dS3[0][1] = T[0][0] * T[0][1] * dSxx + 0.5 * (T[0][0] * T[1][1] + T[0][1] * T[1][0]) * dSxy + T[1][0] * T[1][1] * dSyy;
+        dS3[2][0] = dS3[0][2] = T[0][0] * T[0][2] * dSxx + 0.5 * (T[0][0] * T[1][2] + T[0][2] * T[1][0]) * dSxy + T[1][0] * T[1][2] * dSyy;
+        dS3[2][1] = dS3[1][2] = T[0][2] * T[0][1] * dSxx + 0.5 * (T[0][1] * T[1][2] + T[0][2] * T[1][1]) * dSxy + T[1][1] * T[1][2] * dSyy;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                double acc = 0;
+                for (int k = 0; k < 3; k++)
+                    for (int l = 0; l < 3; l++) acc += T[k][a] * dSP[k][l] * T[l][b];
+                dS3[a][b] += acc;
+            }
+        double dM[3][3];   // 2 RS^T dS3
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                dM[a][b] = 2.0 * (RS[0][a] * dS3[0][b] + RS[1][a] * dS3[1][b] + RS[2][a] * dS3[2][b]);
+        double ds[3];
+        for (int a = 0; a < 3; a++) ds[a] = Rg[0][a] * dM[a][0] + Rg[1][a] * dM[a][1] + Rg[2][a] * dM[a][2];
+        double ms = (s[0] + s[1] + s[2]) / 3.0;
+        double li[3] = {s[0] - ms, s[1] - ms, s[2] - ms};
+        double lr = lambda_iso * (1.0 / 3.0);
+        ds[0] -= lr * (2 * li[0] - li[1] - li[2]);
+        ds[1] -= lr * (-li[0] + 2 * li[1] - li[2]);
+        ds[2] -= lr * (-li[0] - li[1] + 2 * li[2]);
+        for (int b = 0; b < 3; b++) { dM[0][b] *= s[0]; dM[1][b] *= s[1]; dM[2][b] *= s[2]; }
+        double th[3] = {0, 0, 0};
+        for (int r = 0; r < 3; r++) {
+            // R_g.row(r) x dL_dM.col(r)
+            double a0 = Rg[r][0], a1 = Rg[r][1], a2 = Rg[r][2];
+            double b0 = dM[0][r], b1 = dM[1][r], b2 = dM[2][r];
+            th[0] += a1 * b2 - a2 * b1;
+            th[1] += a2 * b0 - a0 * b2;
+            th[2] += a0 * b1 - a1 * b0;
+        }
+        o[6] = -eta * th[0]; o[7] = -eta * th[1]; o[8] = -eta * th[2];
+        o[3] = eta * ds[0]; o[4] = eta * ds[1]; o[5] = eta * ds[2];
+    }
+}
+
+// A18: per-tile pose Jacobians (GSCudaKernels.cu:922-1392), intended semantics for Q12.
+// out_tiles[T*27]: Jtr(6) then JtJ upper(21) per tile; out_total[27] summed in tile order.
+// When out_rgb is non-null the pass-1 render (colour incl. T*bg, depth, final_T, n_contrib) is
+// written per pixel too: with obs_depth > 0.1 everywhere it is the self-render used as target.
+// The pose is the kernel-frame pose: position + float4 read as (x, y, z, w).
+ORC_API void orc_pose_jacobians(int width, int height, const uint32_t* ranges, const uint32_t* idx,
+                                const double* pos2d, const double* conic, const double* depth,
+                                const double* phat, const float* colors, const float* opac,
+                                const float* gx, const float* gy, const float* obs_rgb,
+                                const float* obs_depth, const float* cam_quat, const float* intr,
+                                const float* bg, double alpha_thresh, double color_thresh,
+                                double depth_thresh, double depth_weight, double* out_tiles,
+                                double* out_total, double* out_rgb, double* out_depth,
+                                double* out_final_T, uint32_t* out_n_contrib) {
+    int ntx = (width + kTile - 1) / kTile, nty = (height + kTile - 1) / kTile;
+    int ntiles = ntx * nty;
+    double q[4] = {cam_quat[0], cam_quat[1], cam_quat[2], cam_quat[3]};
+    double R[3][3];
+    quat_to_mat_d(q, R);
+    double fx = intr[0], fy = intr[1], cx = intr[2], cy = intr[3];
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int t = 0; t < ntiles; t++) {
+        int tx = t % ntx, ty = t / ntx;
+        uint32_t s = ranges[2 * t], e = ranges[2 * t + 1];
+        int n_total = (int)(e - s);
+        double tJtr[6] = {0}, tJtJ[21] = {0};
+        for (int tid = 0; tid < kBlock; tid++) {
+            int x = tx * kTile + tid % kTile, y = ty * kTile + tid / kTile;
+            bool in_bounds = x < width && y < height;
+            size_t p = in_bounds ? (size_t)y * width + x : 0;
+            double img_depth = in_bounds ? obs_depth[p] : 0.0;
+            bool inside = in_bounds && img_depth > kDepthValid;
+            double C[3] = {0, 0, 0}, D = 0.0, final_T = 1.0;
+            uint32_t n_contrib = 0;
+            if (n_total > 0) {
+                double T = 1.0;
+                uint32_t contributor = 0, last = 0;
+                bool done = !inside;
+                for (int j = 0; j < n_total && !done; j++) {
+                    contributor++;
+                    uint32_t g = idx[s + j];
+                    double dx = pos2d[2 * g] - x, dy = pos2d[2 * g + 1] - y;
+                    const double* c = conic + 3 * g;
+                    double v = c[0] * dx * dx + 2.0 * c[1] * dx * dy + c[2] * dy * dy;  // Q14
+                    double a = std::min(kAlphaMax, opac[g] * std::exp(-0.5 * v));
+                    if (a < kAlphaMin || v <= 0.0) continue;
+                    double tT = T * (1.0 - a);
+                    if (tT < kTStop) { done = true; continue; }
+                    C[0] += colors[3 * g] * a * T;
+                    C[1] += colors[3 * g + 1] * a * T;
+                    C[2] += colors[3 * g + 2] * a * T;
+                    double d = depth[g] + dx * phat[2 * g] + dy * phat[2 * g + 1];
+                    if (T > 0.5 && tT < 0.5) D = d;
+                    T = tT;
+                    last = contributor;
+                }
+                if (inside) {
+                    final_T = T;
+                    n_contrib = last;
+                    C[0] += T * bg[0]; C[1] += T * bg[1]; C[2] += T * bg[2];
+                }
+            }
+            if (in_bounds && out_rgb) {
+                for (int k = 0; k < 3; k++) out_rgb[3 * p + k] = C[k];
+                out_depth[p] = D; out_final_T[p] = final_T; out_n_contrib[p] = n_contrib;
+            }
+            inside = inside && (final_T < alpha_thresh);
+            if (!inside) continue;
+            double ce[3] = {C[0] - obs_rgb[3 * p], C[1] - obs_rgb[3 * p + 1], C[2] - obs_rgb[3 * p + 2]};
+            double de = (img_depth > kDepthValid) ? (D - img_depth) : 0.0;
+            double cl = std::sqrt(ce[0] * ce[0] + ce[1] * ce[1] + ce[2] * ce[2]);
+            double wc = (cl < color_thresh) ? 1.0 : color_thresh / cl;
+            double ray[3] = {(x - cx) / fx, (y - cy) / fy, 1.0};
+            double rc[9] = {0, -ray[2], ray[1], ray[2], 0, -ray[0], -ray[1], ray[0], 0};
+            const float* gX = gx + 3 * p;
+            const float* gY = gy + 3 * p;
+            double iid = 1.0 / img_depth;
+            double Jp[6] = {fx * iid, 0.0, 0.0, fy * iid, -fx * ray[0] * iid, -fy * ray[1] * iid};
+            double A[9];
+            for (int r = 0; r < 3; r++)
+                for (int k = 0; k < 3; k++) A[r * 3 + k] = Jp[2 * r] * gX[k] + Jp[2 * r + 1] * gY[k];
+            double H[9], b3[3];
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++)
+                    H[r * 3 + c] = wc * (A[r * 3] * A[c * 3] + A[r * 3 + 1] * A[c * 3 + 1] + A[r * 3 + 2] * A[c * 3 + 2]);
+            for (int r = 0; r < 3; r++)
+                b3[r] = -wc * (A[r * 3] * ce[0] + A[r * 3 + 1] * ce[1] + A[r * 3 + 2] * ce[2]);
+            double ld = std::fabs(de);
+            double wd = (ld < depth_thresh) ? 1.0 : depth_thresh / ld;
+            wd /= img_depth;
+            wd *= depth_weight;
+            for (int r = 0; r < 3; r++) {
+                for (int c = 0; c < 3; c++) H[r * 3 + c] += wd * ray[r] * ray[c];
+                b3[r] += wd * de * ray[r];
+            }
+            auto accumulate = [&](const double* Hc, const double* bc, double dd, const double* rcx) {
+                double Jpose[18];
+                for (int r = 0; r < 3; r++)
+                    for (int k = 0; k < 3; k++) Jpose[r * 3 + k] = R[r][k];
+                for (int k = 0; k < 9; k++) Jpose[9 + k] = dd * rcx[k];
+                double tmp[18];
+                for (int r = 0; r < 6; r++)
+                    for (int c = 0; c < 3; c++)
+                        tmp[r * 3 + c] = Jpose[r * 3] * Hc[c] + Jpose[r * 3 + 1] * Hc[3 + c] + Jpose[r * 3 + 2] * Hc[6 + c];
+                int id = 0;
+                for (int r = 0; r < 6; r++)
+                    for (int c = r; c < 6; c++)
+                        tJtJ[id++] += tmp[r * 3] * Jpose[c * 3] + tmp[r * 3 + 1] * Jpose[c * 3 + 1] + tmp[r * 3 + 2] * Jpose[c * 3 + 2];
+                for (int r = 0; r < 6; r++)
+                    tJtr[r] += Jpose[r * 3] * bc[0] + Jpose[r * 3 + 1] * bc[1] + Jpose[r * 3 + 2] * bc[2];
+            };
+            accumulate(H, b3, img_depth, rc);
+            // Pass 2: per-Gaussian terms over the first n_contrib entries (no v<=0 test, Q8).
+            double prod = 1.0, T2 = 1.0, acc[3] = {0, 0, 0};
+            for (int j = 0; j < n_total && j < (int)n_contrib; j++) {
+                uint32_t g = idx[s + j];
+                double dx = pos2d[2 * g] - x, dy = pos2d[2 * g + 1] - y;
+                const double* c = conic + 3 * g;
+                double v = c[0] * dx * dx + 2.0 * c[1] * dx * dy + c[2] * dy * dy;  // Q14
+                double a = std::min(kAlphaMax, opac[g] * std::exp(-0.5 * v));
+                if (a < kAlphaMin) continue;
+                double d = depth[g] + dx * phat[2 * g] + dy * phat[2 * g + 1];
+                double da[3] = {colors[3 * g] * prod, colors[3 * g + 1] * prod, colors[3 * g + 2] * prod};
+                for (int k = 0; k < 3; k++) acc[k] += a * da[k];
+                double iom = 1.0 / (1.0 - a);
+                for (int k = 0; k < 3; k++) da[k] -= (C[k] - acc[k]) * iom;
+                double dm[2] = {c[0] * dx + c[1] * dy, c[1] * dx + c[2] * dy};
+                double rg[3] = {(pos2d[2 * g] - cx) / fx, (pos2d[2 * g + 1] - cy) / fy, 1.0};
+                double rcg[9] = {0, -rg[2], rg[1], rg[2], 0, -rg[0], -rg[1], rg[0], 0};
+                double id_ = 1.0 / d;
+                double Jt[6] = {fx * id_, 0.0, 0.0, fy * id_, -fx * rg[0] * id_, -fy * rg[1] * id_};
+                double jd[3] = {Jt[0] * dm[0] + Jt[1] * dm[1], Jt[2] * dm[0] + Jt[3] * dm[1],
+                                Jt[4] * dm[0] + Jt[5] * dm[1]};
+                double Jc[9];
+                for (int r = 0; r < 3; r++)
+                    for (int k = 0; k < 3; k++) Jc[r * 3 + k] = jd[r] * a * da[k];
+                double Hc[9], bc[3];
+                for (int r = 0; r < 3; r++)
+                    for (int cc = 0; cc < 3; cc++)
+                        Hc[r * 3 + cc] = wc * (Jc[r * 3] * Jc[cc * 3] + Jc[r * 3 + 1] * Jc[cc * 3 + 1] + Jc[r * 3 + 2] * Jc[cc * 3 + 2]);
+                for (int r = 0; r < 3; r++)
+                    bc[r] = -wc * (Jc[r * 3] * ce[0] + Jc[r * 3 + 1] * ce[1] + Jc[r * 3 + 2] * ce[2]);
+                prod *= (1.0 - a);
+                if (T2 > 0.5 && prod <= 0.5 && img_depth > kDepthValid2) {
+                    T2 = prod;
+                    double ld2 = std::fabs(de);
+                    double wd2 = (ld2 < depth_thresh) ? 1.0 : depth_thresh / ld2;
+                    wd2 /= img_depth;
+                    wd2 *= depth_weight;
+                    for (int r = 0; r < 3; r++) {
+                        for (int cc = 0; cc < 3; cc++) Hc[r * 3 + cc] += wd2 * rg[r] * rg[cc];
+                        bc[r] += wd2 * de * rg[r];
+                    }
+                }
+                accumulate(Hc, bc, d, rcg);
+                if (prod < kPass2Stop) break;
+            }
+        }
+        double* o = out_tiles + 27 * (size_t)t;
+        for (int k = 0; k < 6; k++) o[k] = tJtr[k];
+        for (int k = 0; k < 21; k++) o[6 + k] = tJtJ[k];
+    }
+    for (int k = 0; k < 27; k++) out_total[k] = 0.0;
+    for (int t = 0; t < ntiles; t++)
+        for (int k = 0; k < 27; k++) out_total[k] += out_tiles[27 * (size_t)t + k];
+}
+
+// A21: FP64 Gauss-Newton step. JtJ from the 21-entry upper triangle (row-major r<=c), Jtr the
+// kernel-convention sum (the host negates it, GSSlam.cu:2225, and Ceres' step is -JtJ^-1 *
+// (-Jtr)). Solves JtJ * delta = Jtr by Cholesky, then PoseLocalParameterization::Plus
+// (PoseLocalParameterization.cpp:22-44): p += dp, q = normalize(q * [1, dtheta/2]).
+// pose7 = (px, py, pz, qx, qy, qz, qw). Returns 0, or -5 when JtJ is not positive definite.
+ORC_API int orc_gn_step(const double* jtj21, const double* jtr6, double* pose7, double* delta6) {
+    double A[6][6];
+    int id = 0;
+    for (int r = 0; r < 6; r++)
+        for (int c = r; c < 6; c++) { A[r][c] = jtj21[id]; A[c][r] = jtj21[id]; id++; }
+    double L[6][6] = {{0}};
+    for (int j = 0; j < 6; j++) {
+        double s = A[j][j];
+        for (int k = 0; k < j; k++) s -= L[j][k] * L[j][k];
+        if (!(s > 0.0)) return -5;
+        L[j][j] = std::sqrt(s);
+        for (int i = j + 1; i < 6; i++) {
+            double t = A[i][j];
+            for (int k = 0; k < j; k++) t -= L[i][k] * L[j][k];
+            L[i][j] = t / L[j][j];
+        }
+    }
+    double y[6], d[6];
+    for (int i = 0; i < 6; i++) {
+        double t = jtr6[i];
+        for (int k = 0; k < i; k++) t -= L[i][k] * y[k];
+        y[i] = t / L[i][i];
+    }
+    for (int i = 5; i >= 0; i--) {
+        double t = y[i];
+        for (int k = i + 1; k < 6; k++) t -= L[k][i] * d[k];
+        d[i] = t / L[i][i];
+    }
+    if (delta6) for (int i = 0; i < 6; i++) delta6[i] = d[i];
+    pose7[0] += d[0]; pose7[1] += d[1]; pose7[2] += d[2];
+    double qx = pose7[3], qy = pose7[4], qz = pose7[5], qw = pose7[6];
+    double px = 0.5 * d[3], py = 0.5 * d[4], pz = 0.5 * d[5], pw = 1.0;
+    double rw = qw * pw - qx * px - qy * py - qz * pz;
+    double rx = qw * px + qx * pw + qy * pz - qz * py;
+    double ry = qw * py - qx * pz + qy * pw + qz * px;
+    double rz = qw * pz + qx * py - qy * px + qz * pw;
+    double nrm = std::sqrt(rw * rw + rx * rx + ry * ry + rz * rz);
+    pose7[3] = rx / nrm; pose7[4] = ry / nrm; pose7[5] = rz / nrm; pose7[6] = rw / nrm;
+    return 0;
+}
+
+// 3x3 Sobel (OpenCV Sobel(src, CV_32F, dx, dy, 3), scale 1, BORDER_REFLECT_101) of an
+// interleaved 3-channel image, as the reference builds its tracking gradients
+// (GSSlam.cu:1959-1962). Outputs gx, gy (3 channels each).
+ORC_API void orc_sobel3(int width, int height, const float* img, float* gx, float* gy) {
+    auto refl = [](int i, int n) {
+        if (n == 1) return 0;
+        while (i < 0 || i >= n) { if (i < 0) i = -i; if (i >= n) i = 2 * n - 2 - i; }
+        return i;
+    };
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < height; y++)
+        for (int x = 0; x < width; x++) {
+            int xm = refl(x - 1, width), xp = refl(x + 1, width);
+            int ym = refl(y - 1, height), yp = refl(y + 1, height);
+            for (int c = 0; c < 3; c++) {
+                auto P = [&](int yy, int xx) { return (double)img[3 * ((size_t)yy * width + xx) + c]; };
+                double sx = (P(ym, xp) - P(ym, xm)) + 2.0 * (P(y, xp) - P(y, xm)) + (P(yp, xp) - P(yp, xm));
+                double sy = (P(yp, xm) - P(ym, xm)) + 2.0 * (P(yp, x) - P(ym, x)) + (P(yp, xp) - P(ym, xp));
+                gx[3 * ((size_t)y * width + x) + c] = (float)sx;
+                gy[3 * ((size_t)y * width + x) + c] = (float)sy;
+            }
+        }
+}
+
+ORC_API int orc_num_threads(void) { return omp_get_max_threads(); }

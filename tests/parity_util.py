@@ -1,0 +1,54 @@
+"""Shared helpers of the parity tests: seeded workloads, oracle runs, tolerance checks."""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle as O
+from f_vigs_slam_amd import scenes as S
+
+IDENTITY7 = np.array([0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0])
+
+
+def workload(n, seed=0, width=640, height=480, pose7=None):
+    scene = S.make_scene(n, seed=seed, width=width, height=height)
+    intr = S.intrinsics(width, height)
+    pose = IDENTITY7 if pose7 is None else np.asarray(pose7, np.float64)
+    return scene, intr, pose
+
+
+def cam(pose7):
+    p = np.asarray(pose7, np.float64)
+    return p[:3].astype(np.float32), p[3:].astype(np.float32)
+
+
+def oracle_view(scene, pose7, intr, width, height):
+    cp, cq = cam(pose7)
+    return O.pipeline(scene, cp, cq, intr, width, height)
+
+
+def mapping_observations(scene, b, rec, width, height, seed=1):
+    """Observed colour/depth = oracle forward render minus N(0, 1e-2) upstream perturbations."""
+    zero = np.zeros((height * width, 3), np.float32)
+    f0 = O.forward(width, height, b, rec, scene["color"], scene["opacity"], zero)
+    up_c, up_d = S.upstream_grads(width, height, seed=seed)
+    obs_rgb = (f0["rgb"].reshape(height, width, 3) - up_c).astype(np.float32)
+    obs_depth = (f0["depth"].reshape(height, width) - up_d).astype(np.float32)
+    return obs_rgb, obs_depth
+
+
+def close_fraction(got, ref, atol, rtol):
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    err = np.abs(got - ref)
+    ok = err <= atol + rtol * np.abs(ref)
+    return float(ok.mean()), float(err.max()) if err.size else 0.0
+
+
+def grad_agreement(got, ref, rel=1e-3):
+    """Fraction of entries with |got - ref| <= rel * (|ref| + 1e-3 * max|ref|): relative error
+    with a floor at a thousandth of the field's largest magnitude."""
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    scale = np.abs(ref).max() if ref.size else 0.0
+    ok = np.abs(got - ref) <= rel * (np.abs(ref) + 1e-3 * scale)
+    return float(ok.mean())

@@ -1,0 +1,271 @@
+"""GPU parity of libfvgs against the oracle, through the C ABI (ctypes).
+
+Bars (stated per output):
+  * binning (screen records that feed keys, keys, sorted indices, ranges): bit-exact;
+  * images: |gpu - oracle| <= 1e-4 * (1 + |oracle|) on >= 99.9 % of pixels (the rest are
+    discrete flips of FP32-vs-FP64 threshold tests: alpha >= 1/255, T < 1e-4, the T = 0.5 depth
+    crossing);
+  * 2D / 3D gradients: |gpu - oracle| <= 1e-3 * (|oracle| + 1e-3 max|oracle|) on >= 99 % of
+    Gaussians per field (float atomics + flips);
+  * pose JtJ / Jtr: <= 1e-3 relative to the largest entry; Gauss-Newton step <= 1e-6 rad / m.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from f_vigs_slam_amd import Context, FvgsError, gn_step
+from f_vigs_slam_amd import scenes as S
+from tests.parity_util import (close_fraction, grad_agreement, mapping_observations,
+                               oracle_view, workload, cam)
+
+pytestmark = pytest.mark.gpu
+
+W, H = 640, 480
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(keep_keys=1)
+    yield c
+    c.close()
+
+
+def _check_binning(ctx, scene, pose, intr, w, h):
+    pf, b, rec = oracle_view(scene, pose, intr, w, h)
+    ctx.set_scene(scene)
+    m = ctx.prepare_rasterization(pose, intr, w, h)
+    print(f"[binning] N={len(scene['pos'])} {w}x{h} M={m} oracle M={b['n_instances']}", flush=True)
+    assert m == b["n_instances"]
+    for f, k in (("POS2D", "pos2d"), ("COV2D", "cov2d"), ("CONIC", "conic"), ("DEPTH_G", "depth"),
+                 ("PHAT", "phat")):
+        np.testing.assert_array_equal(ctx.read(f), pf[k], err_msg=f)
+    np.testing.assert_array_equal(ctx.read("RANGES"), b["ranges"])
+    np.testing.assert_array_equal(ctx.read("KEYS"), b["keys"])
+    np.testing.assert_array_equal(ctx.read("INDICES"), b["idx"])
+    return pf, b, rec
+
+
+def test_binning_bitexact_identity(ctx):
+    scene, intr, pose = workload(50000, seed=0)
+    _check_binning(ctx, scene, pose, intr, W, H)
+
+
+def test_binning_bitexact_perturbed_with_culls(ctx):
+    scene, intr, _ = workload(50000, seed=3)
+    pose = S.perturbed_pose(3, rot_deg=8.0, trans_m=0.2)
+    _, b, _ = _check_binning(ctx, scene, pose, intr, W, H)
+    # the culled zero records lead tile 0 with key 0
+    assert b["keys"][0] == 0
+
+
+def test_binning_long_tile_lists(ctx):
+    # Large Gaussians on a 128x128 view: lists past 4096 (1024-thread LDS sort) and past 16384
+    # (streamed passes through HBM).
+    rng = np.random.default_rng(11)
+    scene = S.make_scene(60000, seed=11, width=128, height=128)
+    scene["scale"] = (scene["scale"] * rng.uniform(2.0, 6.0, (60000, 1))).astype(np.float32)
+    intr = S.intrinsics(128, 128, f=105.0)
+    _, b, _ = _check_binning(ctx, scene, np.array([0, 0, 0, 0, 0, 0, 1.0]), intr, 128, 128)
+    lens = b["ranges"][:, 1] - b["ranges"][:, 0]
+    print(f"[binning] long lists: max {lens.max()} min {lens.min()}", flush=True)
+    assert lens.max() > 16384 and (lens > 4096).any()
+
+
+def test_binning_odd_size_and_single_gaussian(ctx):
+    scene, intr, pose = workload(1, seed=5, width=37, height=23)
+    _check_binning(ctx, scene, pose, S.intrinsics(37, 23), 37, 23)
+    scene, _, pose = workload(3000, seed=6, width=333, height=177)
+    _check_binning(ctx, scene, pose, S.intrinsics(333, 177), 333, 177)
+
+
+def test_render_parity(ctx):
+    scene, intr, pose = workload(50000, seed=0)
+    pf, b, rec = oracle_view(scene, pose, intr, W, H)
+    ref = O.render(W, H, b, rec, scene["color"], scene["opacity"])
+    ctx.set_scene(scene)
+    ctx.prepare_rasterization(pose, intr, W, H)
+    ctx.render()
+    for name, key in (("RENDER_RGB", "rgb"), ("RENDER_ALPHA", "alpha")):
+        frac, mx = close_fraction(ctx.read(name), ref[key], 1e-4, 1e-4)
+        print(f"[render] {name} within tol {frac:.6f} max err {mx:.3g}", flush=True)
+        assert frac >= 0.999
+    frac, mx = close_fraction(ctx.read("RENDER_DEPTH"), ref["depth"], 1e-4, 1e-4)
+    print(f"[render] depth within tol {frac:.6f}", flush=True)
+    assert frac >= 0.999
+
+
+@pytest.fixture(scope="module")
+def mapping_case():
+    scene, intr, pose = workload(50000, seed=0)
+    pf, b, rec = oracle_view(scene, pose, intr, W, H)
+    obs_rgb, obs_depth = mapping_observations(scene, b, rec, W, H)
+    fwd = O.forward(W, H, b, rec, scene["color"], scene["opacity"], obs_rgb)
+    d2 = O.backward(W, H, b, rec, scene["color"], scene["opacity"], fwd, obs_depth)
+    cp, cq = cam(pose)
+    d3 = O.delta3d(scene["pos"], scene["scale"], scene["quat"], cp, cq, intr, d2)
+    return dict(scene=scene, intr=intr, pose=pose, obs_rgb=obs_rgb, obs_depth=obs_depth,
+                fwd=fwd, d2=d2, d3=d3, b=b)
+
+
+def test_forward_parity(ctx, mapping_case):
+    c = mapping_case
+    ctx.set_scene(c["scene"])
+    ctx.set_frame(c["obs_rgb"], c["obs_depth"])
+    ctx.prepare_rasterization(c["pose"], c["intr"], W, H)
+    ctx.forward()
+    f = c["fwd"]
+    np.testing.assert_array_equal(ctx.read("MAX_CONTRIB") > 0, f["max_contrib"] > 0)
+    nc = ctx.read("N_CONTRIB").ravel()
+    same_nc = float((nc == f["n_contrib"]).mean())
+    print(f"[forward] n_contrib identical on {same_nc:.6f} of pixels", flush=True)
+    assert same_nc >= 0.999
+    for name, ref in (("RGB", f["rgb"]), ("FINAL_T", f["final_T"]), ("DEPTH", f["depth"]),
+                      ("COLOR_ERROR", f["color_error"])):
+        frac, mx = close_fraction(ctx.read(name).reshape(ref.shape), ref, 1e-4, 1e-4)
+        print(f"[forward] {name} within tol {frac:.6f} max err {mx:.3g}", flush=True)
+        assert frac >= 0.999
+
+
+def test_backward_parity(ctx, mapping_case):
+    c = mapping_case
+    ctx.set_scene(c["scene"])
+    ctx.set_frame(c["obs_rgb"], c["obs_depth"])
+    ctx.prepare_rasterization(c["pose"], c["intr"], W, H)
+    ctx.backward()
+    g = ctx.read("DELTA2D")
+    r = c["d2"]
+    n_same = float((g["n"] == r[:, 12]).mean())
+    print(f"[backward] n identical on {n_same:.6f} of Gaussians", flush=True)
+    assert n_same >= 0.99
+    cols = dict(meanImg=(0, 2), invSigmaImg=(2, 5), color=(5, 8), depth=(8, 9), alpha=(9, 10),
+                pHat=(10, 12))
+    for f, (a, z) in cols.items():
+        gv = g[f].reshape(len(g), -1)
+        frac = grad_agreement(gv, r[:, a:z])
+        print(f"[backward] {f} agreement {frac:.6f}", flush=True)
+        assert frac >= 0.99
+
+
+def test_gradients_3d_parity(ctx, mapping_case):
+    c = mapping_case
+    ctx.set_scene(c["scene"])
+    ctx.set_frame(c["obs_rgb"], c["obs_depth"])
+    ctx.mapping_step(c["pose"], c["intr"])
+    g = ctx.read("DELTA3D")
+    r = c["d3"]
+    for f, (a, z) in dict(position=(0, 3), scale=(3, 6), orientation=(6, 9), color=(9, 12),
+                          alpha=(12, 13)).items():
+        frac = grad_agreement(g[f].reshape(len(g), -1), r[:, a:z])
+        print(f"[delta3d] {f} agreement {frac:.6f}", flush=True)
+        assert frac >= 0.99
+
+
+@pytest.fixture(scope="module")
+def track_case():
+    """Config-1 style tracking case at reduced N: target = pass-1 self-render at identity."""
+    n = 50000
+    scene, intr, gt = workload(n, seed=21)
+    pf, b, rec = oracle_view(scene, gt, intr, W, H)
+    ones = np.ones((H, W), np.float32)
+    zeros3 = np.zeros((H, W, 3), np.float32)
+    _, _, rd = O.pose_jacobians(W, H, b, rec, scene["color"], scene["opacity"], zeros3, zeros3,
+                                zeros3, ones, gt[3:].astype(np.float32), intr, want_render=True)
+    obs_rgb = rd["rgb"].reshape(H, W, 3).astype(np.float32)
+    obs_depth = rd["depth"].reshape(H, W).astype(np.float32)
+    gx, gy = O.sobel3(obs_rgb)
+    start = S.perturbed_pose(21, rot_deg=2.0, trans_m=0.03)
+    return dict(scene=scene, intr=intr, gt=gt, obs_rgb=obs_rgb, obs_depth=obs_depth, gx=gx, gy=gy,
+                start=start)
+
+
+def _oracle_jacobians(c, pose7, depth_weight=1.0):
+    pf, b, rec = oracle_view(c["scene"], pose7, c["intr"], W, H)
+    _, tot = O.pose_jacobians(W, H, b, rec, c["scene"]["color"], c["scene"]["opacity"], c["gx"],
+                              c["gy"], c["obs_rgb"], c["obs_depth"],
+                              np.asarray(pose7[3:], np.float32), c["intr"],
+                              depth_weight=depth_weight)
+    return tot[6:], tot[:6]
+
+
+def test_pose_jacobians_parity(ctx, track_case):
+    c = track_case
+    ctx.set_scene(c["scene"])
+    ctx.set_frame(c["obs_rgb"], c["obs_depth"], c["gx"], c["gy"])
+    for pose in (c["gt"], c["start"]):
+        jtj, jtr = ctx.pose_jacobians(pose, c["intr"])
+        rjtj, rjtr = _oracle_jacobians(c, pose)
+        ej = np.abs(jtj - rjtj).max() / np.abs(rjtj).max()
+        er = np.abs(jtr - rjtr).max() / max(np.abs(rjtr).max(), 1e-30)
+        print(f"[pose] JtJ rel err {ej:.3g}  Jtr rel err {er:.3g} |Jtr| {np.abs(rjtr).max():.3g}",
+              flush=True)
+        assert ej <= 1e-3
+        if pose is c["start"]:
+            assert er <= 1e-3
+        else:
+            # Jtr vanishes at the ground truth for a self-rendered target
+            assert np.abs(jtr).max() <= 1e-3 * np.abs(jtj).max() ** 0.5
+
+
+def test_tracking_gauss_newton_vs_oracle(ctx, track_case):
+    c = track_case
+    ctx.set_scene(c["scene"])
+    ctx.set_frame(c["obs_rgb"], c["obs_depth"], c["gx"], c["gy"])
+    pose_g, _ = ctx.track(c["intr"], 20, c["start"])
+    pose_o = c["start"].copy()
+    for _ in range(20):
+        jtj, jtr = _oracle_jacobians(c, pose_o)
+        rc, pose_o, _ = O.gn_step(jtj, jtr, pose_o)
+        assert rc == 0
+    rot, tr = S.pose_error(pose_g, pose_o)
+    r0, t0 = S.pose_error(c["start"], c["gt"])
+    rg, tg = S.pose_error(pose_g, c["gt"])
+    print(f"[track] gpu vs oracle: {rot:.3g} rad {tr:.3g} m; start err {r0:.3g} rad {t0:.3g} m; "
+          f"final vs gt {rg:.3g} rad {tg:.3g} m", flush=True)
+    assert rot <= 1e-6 and tr <= 1e-6
+
+
+def test_gn_step_matches_oracle():
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((6, 6))
+    A = A @ A.T + 6 * np.eye(6)
+    jtj = A[np.triu_indices(6)]
+    jtr = rng.standard_normal(6)
+    pose = np.array([0.1, -0.2, 0.3, 0.1, 0.2, -0.1, 0.97])
+    pose[3:] /= np.linalg.norm(pose[3:])
+    pg, dg = gn_step(jtj, jtr, pose)
+    rc, po, do = O.gn_step(jtj, jtr, pose)
+    np.testing.assert_allclose(dg, do, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(pg, po, rtol=1e-12, atol=1e-14)
+
+
+def test_error_paths(ctx):
+    scene, intr, pose = workload(1000, seed=1)
+    with pytest.raises(FvgsError) as e:
+        ctx.prepare_rasterization(pose, intr, 0, 480)
+    assert e.value.code == -1
+    bad = pose.copy()
+    bad[0] = np.nan
+    ctx.set_scene(scene)
+    with pytest.raises(FvgsError) as e:
+        ctx.prepare_rasterization(bad, intr, W, H)
+    assert e.value.code == -1
+    empty = {k: v[:0] for k, v in scene.items()}
+    ctx.set_scene(empty)
+    with pytest.raises(FvgsError) as e:
+        ctx.prepare_rasterization(pose, intr, W, H)
+    assert e.value.code == -2
+    # all culled with cull_emit=0: no instances
+    c2 = Context(cull_emit=0)
+    c2.set_scene(scene)
+    behind = np.array([0, 0, 0, 1.0, 0, 0, 0])   # 180 deg about x: everything behind the camera
+    with pytest.raises(FvgsError) as e:
+        c2.prepare_rasterization(behind, intr, W, H)
+    assert e.value.code == -2
+    c2.close()
+    # a memory budget too small for the map is rejected, not faulted
+    c3 = Context(mem_budget=1 << 16)
+    with pytest.raises(FvgsError) as e:
+        c3.set_scene(scene)
+        c3.prepare_rasterization(pose, intr, W, H)
+    assert e.value.code == -3
+    c3.close()
