@@ -176,7 +176,7 @@ def delta3d(pos, scale, quat, cam_pos, cam_quat, intr, delta2d, lambda_iso=0.01)
 
 def pose_jacobians(width, height, binning, rec, colors, opac, gx, gy, obs_rgb, obs_depth,
                    cam_quat, intr, bg=(0.0, 0.0, 0.0), alpha_thresh=0.2, color_thresh=0.1,
-                   depth_thresh=0.2, depth_weight=1.0, want_render=False):
+                   depth_thresh=0.2, depth_weight=1.0, want_render=False, pass2_weight=1.0):
     """A18 (GSCudaKernels.cu:922-1392) per tile and summed in tile order.
     Returns (tiles (T, 27), total (27,)[, pass-1 render dict]): Jtr(6) then JtJ upper (21)."""
     ntx, nty = tiles(width, height)
@@ -193,7 +193,7 @@ def pose_jacobians(width, height, binning, rec, colors, opac, gx, gy, obs_rgb, o
                              _p(_f32(cam_quat)), _p(_f32(intr)), _p(_f32(bg)),
                              ctypes.c_double(alpha_thresh), ctypes.c_double(color_thresh),
                              ctypes.c_double(depth_thresh), ctypes.c_double(depth_weight),
-                             _p(t), _p(tot), *(
+                             ctypes.c_double(pass2_weight), _p(t), _p(tot), *(
                                  [_p(rd["rgb"]), _p(rd["depth"]), _p(rd["final_T"]),
                                   _p(rd["n_contrib"])] if rd else [None] * 4))
     if rd is not None:

@@ -742,6 +742,8 @@ ORC_API void orc_delta3d(int n, const float* pos, const float* scale, const floa
 
 // A18: per-tile pose Jacobians (GSCudaKernels.cu:922-1392), intended semantics for Q12.
 // out_tiles[T*27]: Jtr(6) then JtJ upper(21) per tile; out_total[27] summed in tile order.
+// pass2_weight scales the per-Gaussian (pass-2) terms: 1 reproduces the reference, 0 keeps only
+// the per-pixel photometric + depth terms.
 // When out_rgb is non-null the pass-1 render (colour incl. T*bg, depth, final_T, n_contrib) is
 // written per pixel too: with obs_depth > 0.1 everywhere it is the self-render used as target.
 // The pose is the kernel-frame pose: position + float4 read as (x, y, z, w).
@@ -751,7 +753,8 @@ ORC_API void orc_pose_jacobians(int width, int height, const uint32_t* ranges, c
                                 const float* gx, const float* gy, const float* obs_rgb,
                                 const float* obs_depth, const float* cam_quat, const float* intr,
                                 const float* bg, double alpha_thresh, double color_thresh,
-                                double depth_thresh, double depth_weight, double* out_tiles,
+                                double depth_thresh, double depth_weight, double pass2_weight,
+                                double* out_tiles,
                                 double* out_total, double* out_rgb, double* out_depth,
                                 double* out_final_T, uint32_t* out_n_contrib) {
     int ntx = (width + kTile - 1) / kTile, nty = (height + kTile - 1) / kTile;
@@ -854,7 +857,7 @@ ORC_API void orc_pose_jacobians(int width, int height, const uint32_t* ranges, c
             accumulate(H, b3, img_depth, rc);
             // Pass 2: per-Gaussian terms over the first n_contrib entries (no v<=0 test, Q8).
             double prod = 1.0, T2 = 1.0, acc[3] = {0, 0, 0};
-            for (int j = 0; j < n_total && j < (int)n_contrib; j++) {
+            for (int j = 0; j < n_total && j < (int)n_contrib && pass2_weight != 0.0; j++) {
                 uint32_t g = idx[s + j];
                 double dx = pos2d[2 * g] - x, dy = pos2d[2 * g + 1] - y;
                 const double* c = conic + 3 * g;
@@ -894,6 +897,8 @@ ORC_API void orc_pose_jacobians(int width, int height, const uint32_t* ranges, c
                         bc[r] += wd2 * de * rg[r];
                     }
                 }
+                for (int k = 0; k < 9; k++) Hc[k] *= pass2_weight;
+                for (int k = 0; k < 3; k++) bc[k] *= pass2_weight;
                 accumulate(Hc, bc, d, rcg);
                 if (prod < kPass2Stop) break;
             }

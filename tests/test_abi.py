@@ -1,0 +1,124 @@
+"""CPU checks of the C ABI: libfvgs.so loads and exports every function include/fvgs/fvgs.h
+declares, the struct layouts seen by ctypes match the C compiler's, the defaults reproduce the
+reference's constants, and the host-only entry points behave (no GPU needed)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from f_vigs_slam_amd import _lib as L
+from f_vigs_slam_amd import context as C
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "fvgs", "fvgs.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(fvgs_\w+)\s*\(", text, re.M)))
+
+
+def test_header_and_binding_agree():
+    assert declared_functions() == sorted(L.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    nm = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True,
+                        text=True, check=True).stdout
+    exported = set(re.findall(r" T (fvgs_\w+)$", nm, re.M))
+    assert set(declared_functions()) <= exported
+
+
+def test_library_embeds_gfx950_code_objects():
+    # the offload bundle of a library built here names its target triple
+    data = open(L.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in data
+
+
+def test_struct_layouts_match_c(tmp_path):
+    src = tmp_path / "layout.c"
+    src.write_text(f'''
+#include <stdio.h>
+#include <stddef.h>
+#include "{HEADER}"
+int main(void) {{
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(fvgs_params), sizeof(fvgs_pose),
+         sizeof(fvgs_intrinsics), sizeof(fvgs_delta2d), sizeof(fvgs_delta3d),
+         sizeof(fvgs_pose_data), offsetof(fvgs_params, cull_emit),
+         offsetof(fvgs_params, mem_budget), offsetof(fvgs_pose, orientation));
+  return 0;
+}}''')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    want = [ctypes.sizeof(L.Params), ctypes.sizeof(L.Pose), ctypes.sizeof(L.Intrinsics),
+            L.DELTA2D_DTYPE.itemsize, L.DELTA3D_DTYPE.itemsize, 108,
+            L.Params.cull_emit.offset, L.Params.mem_budget.offset, L.Pose.orientation.offset]
+    assert got == want
+
+
+def test_default_params_reproduce_reference_constants():
+    p = C.default_params()
+    assert list(p.bg) == [0.0, 0.0, 0.0]                                   # GSSlam.cu:117
+    assert (p.alpha_thresh, p.color_thresh, p.depth_thresh) == pytest.approx((0.2, 0.1, 0.2))
+    assert (p.w_depth, p.w_dist, p.lambda_iso) == pytest.approx((1.0, 0.1, 0.01))
+    assert p.depth_weight == 1.0 and p.cull_emit == 1 and p.device == 0
+
+
+def test_gn_step_host_path_matches_oracle():
+    rng = np.random.default_rng(7)
+    for _ in range(5):
+        A = rng.standard_normal((6, 6))
+        A = A @ A.T + 0.5 * np.eye(6)
+        jtj = A[np.triu_indices(6)]
+        jtr = rng.standard_normal(6) * 10
+        q = rng.standard_normal(4)
+        pose = np.concatenate([rng.standard_normal(3), q / np.linalg.norm(q)])
+        new, d = C.gn_step(jtj, jtr, pose)
+        rc, po, do = O.gn_step(jtj, jtr, pose)
+        assert rc == 0
+        np.testing.assert_allclose(d, do, rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(new, po, rtol=1e-12, atol=1e-14)
+
+
+def test_gn_step_rejects_indefinite_and_nan():
+    A = np.eye(6)
+    A[2, 2] = -1.0
+    with pytest.raises(C.FvgsError) as e:
+        C.gn_step(A[np.triu_indices(6)], np.ones(6), np.array([0, 0, 0, 0, 0, 0, 1.0]))
+    assert e.value.code == L.FVGS_E_NOTPD
+    A = np.eye(6)
+    A[0, 0] = np.nan
+    with pytest.raises(C.FvgsError):
+        C.gn_step(A[np.triu_indices(6)], np.ones(6), np.array([0, 0, 0, 0, 0, 0, 1.0]))
+
+
+def test_null_arguments_are_rejected_without_a_device():
+    lib = L.load()
+    assert lib.fvgs_default_params(None) == L.FVGS_E_ARG
+    assert lib.fvgs_create(None, None) == L.FVGS_E_ARG
+    assert lib.fvgs_destroy(None) == L.FVGS_E_ARG
+    assert lib.fvgs_gn_step(None, None, None, None) == L.FVGS_E_ARG
+    assert lib.fvgs_last_error(None) == b"null context"
+    assert lib.fvgs_abi_version() == 1
+
+
+def test_context_creation_fails_loudly_without_gpu():
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("a GPU is present")
+    with pytest.raises(C.FvgsError) as e:
+        C.Context()
+    assert e.value.code == L.FVGS_E_HIP
