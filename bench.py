@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tracking", action="store_true")
     ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--breakdown", action="store_true",
+                    help="after the timed steps, run the forward-only blend K times (profiling)")
     return ap.parse_args()
 
 
@@ -151,6 +153,10 @@ def main():
     stage /= args.steps
     st = ctx.stats()
     M, K, P = st["M"], st["K"], st["pixels"]
+    if args.breakdown:
+        ctx.prepare_rasterization(pose, intr, w, h)
+        for _ in range(args.steps):
+            ctx.forward()
     log(f"step {t * 1e3:.3f} ms; stages binning {stage[0]:.3f} clear {stage[1]:.3f} "
         f"blend {stage[2]:.3f} 3d {stage[3]:.3f} ms; M={M} K={K}")
     # Algorithmic bytes (compulsory traffic, each counted once):

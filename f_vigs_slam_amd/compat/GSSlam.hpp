@@ -1,0 +1,353 @@
+// GSSlam hot-path shim over the libfvgs C ABI (include/fvgs/fvgs.h).
+//
+// The reference's callers — RgbdPoseCostFunction (RgbdPoseCost.cu:29-109), the marginalization
+// pre-pass (MarginalizationFactor.cpp:37,183-185), GSSlam::compute / optimizationLoop and the
+// viewer (GaussianSplattingViewer.cpp:204-205) — reach the rasterizer through a handful of
+// GSSlam methods (GSSlam.cuh:25-176). This header re-provides those methods, with the same names,
+// argument meaning and error behaviour, on top of libfvgs:
+//
+//   prepareRasterization(pose, intrinsics, w, h)        GSSlam.cu:408-646
+//   renderView(pose, intrinsics, w, h, rgb, depth)       GSSlam.cu:775-799  (device aliases)
+//   optimizeGaussiansKeyframe(keyframe)                  GSSlam.cu:1083-1264 (without Adam)
+//   computeRgbdPoseJacobians(JtJ, Jtr, level, P, Q)      GSSlam.cu:2103-2244
+//
+// It depends on nothing but the C ABI and the C++ standard library; with Eigen on the include
+// path it also offers the reference's Eigen-typed computeRgbdPoseJacobians overload. Like the
+// reference, void methods report failures on std::cerr and return early; renderView returns
+// false. The host-side conventions of the reference are reproduced, including the camera
+// quaternion packing of computeRgbdPoseJacobians (GSSlam.cu:2120-2121: float4(w, x, y, z), which
+// the kernels read as (x, y, z, w)).
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <iostream>
+#include <mutex>
+#include <vector>
+
+#include "../../include/fvgs/fvgs.h"
+
+#if __has_include(<Eigen/Dense>)
+#include <Eigen/Dense>
+#define FVGS_SHIM_HAS_EIGEN 1
+#endif
+
+namespace fvgs_compat {
+
+// Layout-compatible mirrors of CameraPose / IntrinsicParameters (RepresentationClasses.hpp:101-122)
+// without CUDA vector types: orientation is whatever float4 the caller stores; the kernels read
+// it as (x, y, z, w).
+struct CameraPose {
+    float position[3] = {0.f, 0.f, 0.f};
+    float orientation[4] = {1.f, 0.f, 0.f, 0.f};
+};
+struct IntrinsicParameters {
+    float f[2] = {0.f, 0.f};
+    float c[2] = {0.f, 0.f};
+};
+
+// One pyramid level of the current frame (GSSlam::pyr_color_, pyr_depth_, pyr_dx_, pyr_dy_):
+// interleaved float RGB in [0, 1], depth in metres, Sobel gradients (may be null: then computed
+// on the device).
+struct FrameLevel {
+    int width = 0, height = 0;
+    const float* rgb = nullptr;
+    const float* depth = nullptr;
+    const float* gx = nullptr;
+    const float* gy = nullptr;
+    int mem = FVGS_MEM_HOST;
+};
+
+struct KeyframeData {
+    FrameLevel frame;
+    CameraPose pose;
+    IntrinsicParameters intrinsics;
+};
+
+class GSSlam {
+public:
+    explicit GSSlam(const fvgs_params* params = nullptr) {
+        fvgs_params p;
+        fvgs_default_params(&p);
+        if (params) p = *params;
+        if (fvgs_create(&p, &ctx_) != FVGS_OK) {
+            std::cerr << "[GSSlam] fvgs_create failed (no HIP device?)" << std::endl;
+            ctx_ = nullptr;
+        }
+    }
+    ~GSSlam() {
+        if (ctx_) fvgs_destroy(ctx_);
+    }
+    GSSlam(const GSSlam&) = delete;
+    GSSlam& operator=(const GSSlam&) = delete;
+
+    bool ok() const { return ctx_ != nullptr; }
+    fvgs_ctx* context() { return ctx_; }
+
+    void setIntrinsics(const IntrinsicParameters& intr) {
+        intrinsics_ = intr;
+        intrinsics_set_ = true;
+    }
+    // Extrinsics of the camera in the IMU frame: t_imu_cam, q_imu_cam as (x, y, z, w).
+    void setImuToCamExtrinsics(const double t[3], const double q_xyzw[4]) {
+        for (int i = 0; i < 3; i++) t_imu_cam_[i] = t[i];
+        for (int i = 0; i < 4; i++) q_imu_cam_[i] = q_xyzw[i];
+    }
+
+    // Map (GSSlam::gaussians_): copied to HBM, or referenced in place with FVGS_MEM_DEVICE.
+    bool setGaussians(uint32_t n, const float* pos, const float* scale, const float* quat,
+                      const float* color, const float* opacity, int mem = FVGS_MEM_HOST) {
+        std::lock_guard<std::mutex> lk(mutex_);
+        return check(fvgs_set_gaussians(ctx_, n, pos, scale, quat, color, opacity, mem),
+                     "setGaussians");
+    }
+    uint32_t getGaussiansCount() const {
+        uint32_t n = 0;
+        if (ctx_) fvgs_gaussian_count(ctx_, &n);
+        return n;
+    }
+    bool hasGaussians() const { return getGaussiansCount() > 0; }
+
+    // Pyramid of the current frame (GSSlam::initAndCopyImgs output); level 0 first.
+    void setFramePyramid(const std::vector<FrameLevel>& levels) { pyramid_ = levels; }
+
+    void prepareRasterization(const CameraPose& pose, const IntrinsicParameters& intr, int w,
+                              int h) {
+        std::lock_guard<std::mutex> lk(mutex_);
+        fvgs_pose p = to_pose(pose);
+        fvgs_intrinsics i = to_intr(intr);
+        uint32_t m = 0;
+        int r = fvgs_prepare_rasterization(ctx_, &p, &i, w, h, &m);
+        if (r == FVGS_E_EMPTY) return;   // the reference returns early without a message
+        check(r, "prepareRasterization");
+        last_nb_instances_ = m;
+    }
+
+    // Render-only forward; rgb/depth receive device aliases of the library's buffers
+    // (float RGB and float depth, width*height pixels), like the GpuMat aliases of the reference.
+    bool renderView(const CameraPose& pose, const IntrinsicParameters& intr, int w, int h,
+                    float** rgb_dev, float** depth_dev) {
+        std::lock_guard<std::mutex> lk(mutex_);
+        if (!ctx_ || !intrinsics_set_ || getGaussiansCount() == 0 || w <= 0 || h <= 0)
+            return false;
+        fvgs_pose p = to_pose(pose);
+        fvgs_intrinsics i = to_intr(intr);
+        if (fvgs_prepare_rasterization(ctx_, &p, &i, w, h, nullptr) != FVGS_OK) return false;
+        if (fvgs_render(ctx_) != FVGS_OK) return false;
+        size_t b = 0;
+        if (fvgs_device_ptr(ctx_, FVGS_OUT_RENDER_RGB, (void**)rgb_dev, &b) != FVGS_OK) return false;
+        if (fvgs_device_ptr(ctx_, FVGS_OUT_RENDER_DEPTH, (void**)depth_dev, &b) != FVGS_OK)
+            return false;
+        return true;
+    }
+
+    // Forward + backward + 2D->3D gradients for one keyframe; the fvgs_delta3d[N] result stays
+    // on the device (deltas3d_dev). The Adam update of the reference is not part of this path.
+    void optimizeGaussiansKeyframe(const KeyframeData& kf, fvgs_delta3d** deltas3d_dev = nullptr) {
+        std::lock_guard<std::mutex> lk(mutex_);
+        if (!ctx_ || getGaussiansCount() == 0 || !kf.frame.rgb || !kf.frame.depth) return;
+        if (!check(fvgs_set_frame(ctx_, kf.frame.width, kf.frame.height, kf.frame.rgb,
+                                  kf.frame.depth, nullptr, nullptr, kf.frame.mem),
+                   "optimizeGaussiansKeyframe"))
+            return;
+        fvgs_pose p = to_pose(kf.pose);
+        fvgs_intrinsics i = to_intr(kf.intrinsics);
+        if (!check(fvgs_mapping_step(ctx_, &p, &i), "optimizeGaussiansKeyframe")) return;
+        if (deltas3d_dev) {
+            size_t b = 0;
+            fvgs_device_ptr(ctx_, FVGS_OUT_DELTA3D, (void**)deltas3d_dev, &b);
+        }
+    }
+
+    // RGB-D pose Jacobians at the IMU pose (P_imu, Q_imu as (x, y, z, w)) for a pyramid level:
+    // JtJ is a row-major 6x6, Jtr has the reference's sign (the negated kernel sum), both in the
+    // IMU frame (J_cam_imu applied, GSSlam.cu:2233-2243).
+    void computeRgbdPoseJacobians(double JtJ[36], double Jtr[6], int level, const double P_imu[3],
+                                  const double Q_imu[4]) {
+        std::lock_guard<std::mutex> lk(mutex_);
+        for (int k = 0; k < 36; k++) JtJ[k] = 0.0;
+        for (int k = 0; k < 6; k++) Jtr[k] = 0.0;
+        if (!ctx_ || pyramid_.empty() || getGaussiansCount() == 0) return;
+        // IMU -> camera (GSSlam.cu:2113-2121)
+        double R_imu[9];
+        quat_to_mat(Q_imu, R_imu);
+        double P_cam[3];
+        for (int r = 0; r < 3; r++)
+            P_cam[r] = P_imu[r] + R_imu[3 * r] * t_imu_cam_[0] + R_imu[3 * r + 1] * t_imu_cam_[1] +
+                       R_imu[3 * r + 2] * t_imu_cam_[2];
+        double Q_cam[4];
+        quat_mul(Q_imu, q_imu_cam_, Q_cam);
+        fvgs_pose pose;
+        for (int k = 0; k < 3; k++) pose.position[k] = (float)P_cam[k];
+        // float4(w, x, y, z), read by the kernels as (x, y, z, w) — the reference's packing.
+        pose.orientation[0] = (float)Q_cam[3];
+        pose.orientation[1] = (float)Q_cam[0];
+        pose.orientation[2] = (float)Q_cam[1];
+        pose.orientation[3] = (float)Q_cam[2];
+        // pyramid level and scaled intrinsics (GSSlam.cu:2126-2138)
+        const int lv = std::min(level, (int)pyramid_.size() - 1);
+        const FrameLevel& fl = pyramid_[lv];
+        const float s = 1.0f / (float)(1 << lv);
+        fvgs_intrinsics intr = to_intr(intrinsics_);
+        intr.f[0] *= s; intr.f[1] *= s; intr.c[0] *= s; intr.c[1] *= s;
+        if (!check(fvgs_set_frame(ctx_, fl.width, fl.height, fl.rgb, fl.depth, fl.gx, fl.gy,
+                                  fl.mem), "computeRgbdPoseJacobians"))
+            return;
+        if ((!fl.gx || !fl.gy) && !check(fvgs_compute_gradients(ctx_), "computeRgbdPoseJacobians"))
+            return;
+        double jtj21[21], jtr6[6];
+        int r = fvgs_pose_jacobians(ctx_, &pose, &intr, jtj21, jtr6);
+        if (r == FVGS_E_EMPTY) return;
+        if (!check(r, "computeRgbdPoseJacobians")) return;
+        double A[36];
+        int id = 0;
+        for (int i = 0; i < 6; i++)
+            for (int j = i; j < 6; j++) {
+                A[6 * i + j] = jtj21[id];
+                A[6 * j + i] = jtj21[id];
+                id++;
+            }
+        double b[6];
+        for (int i = 0; i < 6; i++) b[i] = -jtr6[i];
+        // J_cam_imu = [[I, -R_imu [t_ic]x], [0, R_ic^T]]; JtJ <- J^T JtJ J, Jtr <- J^T Jtr
+        double R_ic[9];
+        quat_to_mat(q_imu_cam_, R_ic);
+        double sk[9] = {0, -t_imu_cam_[2], t_imu_cam_[1], t_imu_cam_[2], 0, -t_imu_cam_[0],
+                        -t_imu_cam_[1], t_imu_cam_[0], 0};
+        double J[36] = {0};
+        for (int i = 0; i < 3; i++) J[6 * i + i] = 1.0;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double v = 0;
+                for (int k = 0; k < 3; k++) v -= R_imu[3 * i + k] * sk[3 * k + j];
+                J[6 * i + 3 + j] = v;
+                J[6 * (3 + i) + 3 + j] = R_ic[3 * j + i];
+            }
+        double T[36];
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) {
+                double v = 0;
+                for (int k = 0; k < 6; k++) v += A[6 * i + k] * J[6 * k + j];
+                T[6 * i + j] = v;
+            }
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) {
+                double v = 0;
+                for (int k = 0; k < 6; k++) v += J[6 * k + i] * T[6 * k + j];
+                JtJ[6 * i + j] = v;
+            }
+        for (int i = 0; i < 6; i++) {
+            double v = 0;
+            for (int k = 0; k < 6; k++) v += J[6 * k + i] * b[k];
+            Jtr[i] = v;
+        }
+    }
+
+#ifdef FVGS_SHIM_HAS_EIGEN
+    void computeRgbdPoseJacobians(Eigen::Matrix<double, 6, 6>& JtJ, Eigen::Matrix<double, 6, 1>& Jtr,
+                                  int level, const Eigen::Vector3d& P_imu,
+                                  const Eigen::Quaterniond& Q_imu) {
+        double a[36], b[6], p[3] = {P_imu.x(), P_imu.y(), P_imu.z()},
+                            q[4] = {Q_imu.x(), Q_imu.y(), Q_imu.z(), Q_imu.w()};
+        computeRgbdPoseJacobians(a, b, level, p, q);
+        for (int i = 0; i < 6; i++) {
+            Jtr(i) = b[i];
+            for (int j = 0; j < 6; j++) JtJ(i, j) = a[6 * i + j];
+        }
+    }
+#endif
+
+    uint32_t lastInstances() const { return last_nb_instances_; }
+
+private:
+    bool check(int r, const char* what) {
+        if (r == FVGS_OK) return true;
+        std::cerr << "[ERROR " << what << "] " << fvgs_last_error(ctx_) << " (" << r << ")"
+                  << std::endl;
+        return false;
+    }
+    static fvgs_pose to_pose(const CameraPose& c) {
+        fvgs_pose p;
+        for (int k = 0; k < 3; k++) p.position[k] = c.position[k];
+        for (int k = 0; k < 4; k++) p.orientation[k] = c.orientation[k];
+        return p;
+    }
+    static fvgs_intrinsics to_intr(const IntrinsicParameters& c) {
+        fvgs_intrinsics i;
+        i.f[0] = c.f[0]; i.f[1] = c.f[1]; i.c[0] = c.c[0]; i.c[1] = c.c[1];
+        return i;
+    }
+    // (x, y, z, w) unit quaternion -> row-major rotation matrix
+    static void quat_to_mat(const double q[4], double R[9]) {
+        const double x = q[0], y = q[1], z = q[2], w = q[3];
+        R[0] = 1 - 2 * (y * y + z * z); R[1] = 2 * (x * y - w * z); R[2] = 2 * (x * z + w * y);
+        R[3] = 2 * (x * y + w * z); R[4] = 1 - 2 * (x * x + z * z); R[5] = 2 * (y * z - w * x);
+        R[6] = 2 * (x * z - w * y); R[7] = 2 * (y * z + w * x); R[8] = 1 - 2 * (x * x + y * y);
+    }
+    static void quat_mul(const double a[4], const double b[4], double o[4]) {
+        o[0] = a[3] * b[0] + a[0] * b[3] + a[1] * b[2] - a[2] * b[1];
+        o[1] = a[3] * b[1] - a[0] * b[2] + a[1] * b[3] + a[2] * b[0];
+        o[2] = a[3] * b[2] + a[0] * b[1] - a[1] * b[0] + a[2] * b[3];
+        o[3] = a[3] * b[3] - a[0] * b[0] - a[1] * b[1] - a[2] * b[2];
+    }
+
+    fvgs_ctx* ctx_ = nullptr;
+    std::mutex mutex_;   // the reference's optimization_mutex_
+    IntrinsicParameters intrinsics_;
+    bool intrinsics_set_ = false;
+    double t_imu_cam_[3] = {0, 0, 0};
+    double q_imu_cam_[4] = {0, 0, 0, 1};
+    std::vector<FrameLevel> pyramid_;
+    uint32_t last_nb_instances_ = 0;
+};
+
+// RgbdPoseCostFunction::EvaluateNonConst's conditioning (RgbdPoseCost.cu:57-102) without Eigen:
+// eigen-decompose JtJ (cyclic Jacobi, FP64), zero eigenvalues <= 1e-4, and return
+// r* = sqrt(L+)^-1 U^T Jtr and J* = sqrt(L) U^T (6x7 row-major, last column zero).
+inline void rgbdPoseCost(const double JtJ[36], const double Jtr[6], double r[6], double J67[42]) {
+    double A[36], V[36];
+    for (int i = 0; i < 36; i++) { A[i] = JtJ[i]; V[i] = (i % 7 == 0) ? 1.0 : 0.0; }
+    for (int sweep = 0; sweep < 64; sweep++) {
+        double off = 0;
+        for (int p = 0; p < 6; p++)
+            for (int q = p + 1; q < 6; q++) off += A[6 * p + q] * A[6 * p + q];
+        if (off < 1e-30) break;
+        for (int p = 0; p < 6; p++)
+            for (int q = p + 1; q < 6; q++) {
+                const double apq = A[6 * p + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                const double th = (A[6 * q + q] - A[6 * p + p]) / (2 * apq);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1));
+                const double c = 1 / std::sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < 6; k++) {
+                    const double akp = A[6 * k + p], akq = A[6 * k + q];
+                    A[6 * k + p] = c * akp - s * akq;
+                    A[6 * k + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 6; k++) {
+                    const double apk = A[6 * p + k], aqk = A[6 * q + k];
+                    A[6 * p + k] = c * apk - s * aqk;
+                    A[6 * q + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 6; k++) {
+                    const double vkp = V[6 * k + p], vkq = V[6 * k + q];
+                    V[6 * k + p] = c * vkp - s * vkq;
+                    V[6 * k + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    const double eps = 1e-4;
+    for (int i = 0; i < 42; i++) J67[i] = 0.0;
+    for (int e = 0; e < 6; e++) {
+        const double lam = A[6 * e + e];
+        const double ss = lam > eps ? std::sqrt(lam) : 0.0;
+        const double si = lam > eps ? std::sqrt(1.0 / lam) : 0.0;
+        double ut = 0;
+        for (int k = 0; k < 6; k++) ut += V[6 * k + e] * Jtr[k];
+        r[e] = si * ut;
+        for (int k = 0; k < 6; k++) J67[7 * e + k] = ss * V[6 * k + e];
+    }
+}
+
+}  // namespace fvgs_compat

@@ -220,10 +220,12 @@ __global__ __launch_bounds__(256) void k_colscan1(uint32_t* __restrict__ blk_cou
 }
 
 // Tile totals, their exclusive scan (ranges), and chunk bases folded with the tile offset.
-// One block of 1024 threads.
+// One block of 1024 threads. When the instance count exceeds the capacity of the instance
+// buffers (cap), every range is written empty so that the downstream kernels do no work and
+// touch nothing; the host sees M > cap, grows the buffers and runs the view again.
 __global__ __launch_bounds__(1024) void k_colscan2(uint32_t* __restrict__ chunk_pre, int nchunk,
                                                    int T, uint32_t* __restrict__ ranges,
-                                                   uint32_t* __restrict__ d_m) {
+                                                   uint32_t* __restrict__ d_m, uint32_t cap) {
     extern __shared__ uint32_t tot[];   // T entries, then 1024 + 32 scratch
     uint32_t* part = tot + T;
     const int tid = threadIdx.x;
@@ -251,10 +253,13 @@ __global__ __launch_bounds__(1024) void k_colscan2(uint32_t* __restrict__ chunk_
         __syncthreads();
     }
     uint32_t run = part[tid] - sum;   // exclusive prefix of this thread's segment
+    const bool overflow = part[1023] > cap;
     for (int i = a; i < b; ++i) {
         uint32_t x = tot[i];
-        ranges[2 * i] = run;
-        ranges[2 * i + 1] = run + x;
+        // tiles without instances keep (0, 0), the reference's memset value (GSSlam.cu:470-471)
+        const bool empty = overflow || x == 0;
+        ranges[2 * i] = empty ? 0u : run;
+        ranges[2 * i + 1] = empty ? 0u : run + x;
         tot[i] = run;
         run += x;
     }
@@ -294,14 +299,23 @@ __global__ __launch_bounds__(kBinThreads) void k_emit(
 
 // ------------------------------------------------------------------------------------------
 // Per-tile LSD radix sort.
+//
+// One workgroup per tile sorts the unique 64-bit key ((depth bits - tile min) << idx_bits | idx)
+// on its significant bits, 8 bits per pass. Ranks are stable: within a wave, item by item, the
+// lanes holding the same digit are found with 8 ballots ("match"), ranked by lane order and
+// offset by a per-wave digit counter in LDS; per-digit offsets across waves come from a prefix
+// over the per-wave counters. Elements are laid out w*64*ipt + e*64 + lane with ipt =
+// ceil(n / NT) items per thread, so every wave works on valid data.
+// Lists of up to CHUNK keys stay in LDS (ping-pong); longer lists run each pass as a histogram
+// sweep plus chunk-by-chunk stable scatter through HBM (pairs <-> tmp).
 // ------------------------------------------------------------------------------------------
 
 __device__ __forceinline__ uint64_t match8(uint32_t d, uint64_t active) {
     uint64_t peers = active;
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-        int bit = (d >> b) & 1;
-        uint64_t m = __ballot(bit);
+        const int bit = (d >> b) & 1;
+        const uint64_t m = __ballot(bit);
         peers &= bit ? m : ~m;
     }
     return peers;
@@ -315,49 +329,56 @@ __device__ __forceinline__ void block_minmax(uint32_t& mn, uint32_t& mx, uint32_
         mn = min(mn, (uint32_t)__shfl_xor((int)mn, o));
         mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
     }
-    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) { red[w] = mn; red[NW + w] = mx; }
     __syncthreads();
     mn = red[0]; mx = red[NW];
+#pragma unroll
     for (int i = 1; i < NW; ++i) { mn = min(mn, red[i]); mx = max(mx, red[NW + i]); }
     __syncthreads();
 }
 
-// Exclusive scan of 256 LDS counters in place (NT >= 256 threads, all call).
-template <int NT>
-__device__ __forceinline__ void scan256(uint32_t* a, uint32_t* tmp) {
-    const int tid = threadIdx.x;
-    uint32_t x = (tid < 256) ? a[tid] : 0u;
-    if (tid < 256) tmp[tid] = x;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        uint32_t y = (tid < 256 && tid >= off) ? tmp[tid - off] : 0u;
-        __syncthreads();
-        if (tid < 256) tmp[tid] += y;
-        __syncthreads();
+// In-place exclusive scan of a[0..256) by the first four waves (wave shuffles + one barrier).
+// All threads of the block call it. wsum: 4 entries of scratch.
+__device__ __forceinline__ void scan256(uint32_t* a, uint32_t* wsum) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    uint32_t x = 0, inc = 0;
+    if (tid < 256) {
+        x = a[tid];
+        inc = x;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)inc, o);
+            if (lane >= o) inc += y;
+        }
+        if (lane == 63) wsum[w] = inc;
     }
-    if (tid < 256) a[tid] = tmp[tid] - x;
+    __syncthreads();
+    if (tid < 256) {
+        uint32_t pre = 0;
+        for (int i = 0; i < w; ++i) pre += wsum[i];
+        a[tid] = pre + inc - x;
+    }
     __syncthreads();
 }
 
 template <int NT, int E>
-struct SortLds {
+struct SortCfg {
     static constexpr int NW = NT / 64;
     static constexpr int CHUNK = NT * E;
-    // whist[NW*256] + dbase[256] + tmp[512] (the reduction scratch lives in tmp's tail)
-    static constexpr size_t kHeader = (size_t)(NW * 256 + 256 + 512) * 4;
-    static constexpr size_t bytes(bool resident) {
-        return kHeader + (resident ? (size_t)CHUNK * 8 : 0);
-    }
+    // whist[NW*256] | dtot[256] | dbase[256] | misc[64] | keysA[CHUNK] | keysB[CHUNK]
+    static constexpr size_t kHeader = (size_t)(NW * 256 + 256 + 256 + 64) * 4;
+    static constexpr size_t kLds = kHeader + (size_t)CHUNK * 16;
 };
 
-// One pass over one chunk: stable ranks per wave, per-digit bases, scatter.
-// resident: digit bases are the chunk's own exclusive digit scan; otherwise dbase[] holds the
-// running global bases of the segment and is advanced by the chunk's digit totals.
+// One stable counting pass over cnt keys held in LDS (src). Destinations: the LDS buffer dst_lds
+// (resident: bases = this chunk's own digit scan) or HBM dst_glob (streaming: bases = running
+// per-digit offsets in dbase, advanced by this chunk's digit totals).
 template <int NT, int E>
-__device__ __forceinline__ void radix_chunk(uint64_t (&k)[E], int cnt, int shift,
-                                            uint32_t* whist, uint32_t* dbase, uint32_t* tmp,
-                                            bool resident, uint64_t* dst, uint32_t limit) {
+__device__ __forceinline__ void radix_pass(const uint64_t* src, int cnt, int ipt, int shift,
+                                           uint32_t* whist, uint32_t* dtot, uint32_t* dbase,
+                                           uint32_t* misc, uint64_t* dst_lds,
+                                           uint64_t* dst_glob, uint32_t limit) {
     constexpr int NW = NT / 64;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
@@ -366,53 +387,60 @@ __device__ __forceinline__ void radix_chunk(uint64_t (&k)[E], int cnt, int shift
     uint32_t rk[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        int j = w * 64 * E + e * 64 + lane;
-        bool valid = j < cnt;
-        uint32_t d = valid ? (uint32_t)(k[e] >> shift) & 255u : 0u;
-        uint64_t act = __ballot(valid);
-        uint64_t peers = match8(d, act);
-        uint32_t lr = __popcll(peers & lt);
-        uint32_t base = 0;
-        if (valid) base = whist[w * 256 + d];
-        if (valid && lr == 0) whist[w * 256 + d] = base + (uint32_t)__popcll(peers);
-        rk[e] = base + lr;
+        rk[e] = 0;
+        if (e < ipt) {
+            const int j = w * 64 * ipt + e * 64 + lane;
+            const bool valid = j < cnt;
+            const uint32_t d = valid ? (uint32_t)(src[j] >> shift) & 255u : 0u;
+            const uint64_t peers = match8(d, __ballot(valid));
+            const uint32_t lr = __popcll(peers & lt);
+            uint32_t base = 0;
+            if (valid) base = whist[w * 256 + d];
+            if (valid && lr == 0) whist[w * 256 + d] = base + (uint32_t)__popcll(peers);
+            rk[e] = base + lr;
+        }
     }
     __syncthreads();
     if (tid < 256) {
         uint32_t run = 0;
+#pragma unroll
         for (int ww = 0; ww < NW; ++ww) {
-            uint32_t h = whist[ww * 256 + tid];
+            const uint32_t h = whist[ww * 256 + tid];
             whist[ww * 256 + tid] = run;
             run += h;
         }
-        tmp[256 + tid] = run;  // digit total of this chunk
+        dtot[tid] = run;
+        if (dst_lds) dbase[tid] = run;
     }
     __syncthreads();
-    if (resident) {
-        if (tid < 256) dbase[tid] = tmp[256 + tid];
-        __syncthreads();
-        scan256<NT>(dbase, tmp);
-    }
+    if (dst_lds) scan256(dbase, misc);
     if (tid < 256) {
-        uint32_t b = dbase[tid];
+        const uint32_t b = dbase[tid];
+#pragma unroll
         for (int ww = 0; ww < NW; ++ww) whist[ww * 256 + tid] += b;
     }
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        int j = w * 64 * E + e * 64 + lane;
-        if (j < cnt) {
-            uint32_t d = (uint32_t)(k[e] >> shift) & 255u;
-            uint32_t pos = whist[w * 256 + d] + rk[e];
-            if (pos < limit) dst[pos] = k[e];   // always true for a consistent ranking
+        if (e < ipt) {
+            const int j = w * 64 * ipt + e * 64 + lane;
+            if (j < cnt) {
+                const uint64_t key = src[j];
+                const uint32_t d = (uint32_t)(key >> shift) & 255u;
+                const uint32_t pos = whist[w * 256 + d] + rk[e];
+                if (pos < limit) {   // always true for a consistent ranking
+                    if (dst_lds) dst_lds[pos] = key;
+                    else dst_glob[pos] = key;
+                }
+            }
         }
     }
-    if (!resident && tid < 256) dbase[tid] += tmp[256 + tid];
+    __syncthreads();
+    if (!dst_lds && tid < 256) dbase[tid] += dtot[tid];
     __syncthreads();
 }
 
-// Sorts the tiles whose list length n satisfies lo < n <= hi (the other tiles are left to a
-// launch with a different configuration). Lists up to CHUNK stay resident in LDS.
+// Sorts the tiles whose list length n satisfies lo < n <= hi.
 template <int NT, int E>
 __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ ranges,
                                                   int idx_bits, uint64_t* __restrict__ pairs,
@@ -420,115 +448,90 @@ __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ r
                                                   uint32_t* __restrict__ sorted_idx,
                                                   uint64_t* __restrict__ keys_out, uint32_t lo,
                                                   uint32_t hi) {
-    using L = SortLds<NT, E>;
-    constexpr int NW = L::NW, CHUNK = L::CHUNK;
+    using C = SortCfg<NT, E>;
+    constexpr int NW = C::NW, CHUNK = C::CHUNK;
     extern __shared__ __align__(16) uint8_t smem[];
     uint32_t* whist = reinterpret_cast<uint32_t*>(smem);
-    uint32_t* dbase = whist + NW * 256;
-    uint32_t* tmp = dbase + 256;          // 512 entries
-    uint32_t* red = tmp + 512 - 64;       // reuse tail for reductions (NW*2 <= 64)
-    uint64_t* xbuf = reinterpret_cast<uint64_t*>(smem + L::kHeader);
-    const int tile = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    uint32_t* dtot = whist + NW * 256;
+    uint32_t* dbase = dtot + 256;
+    uint32_t* misc = dbase + 256;
+    uint64_t* keysA = reinterpret_cast<uint64_t*>(smem + C::kHeader);
+    uint64_t* keysB = keysA + CHUNK;
+    const int tile = blockIdx.x, tid = threadIdx.x;
     const uint32_t s = ranges[2 * tile], n = ranges[2 * tile + 1] - s;
     if (n == 0 || n <= lo || n > hi) return;
-    const bool resident = n <= (uint32_t)CHUNK;
     const uint64_t idx_mask = (1ull << idx_bits) - 1ull;
     if (n == 1) {
         if (tid == 0) {
-            uint64_t p = pairs[s];
+            const uint64_t p = pairs[s];
             sorted_idx[s] = (uint32_t)p;
             if (keys_out) keys_out[s] = ((uint64_t)tile << 32) | (p >> 32);
         }
         return;
     }
-    uint64_t k[E];
+    const bool resident = n <= (uint32_t)CHUNK;
     uint32_t mn = 0xffffffffu, mx = 0u;
-    if (resident) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            int j = w * 64 * E + e * 64 + lane;
-            k[e] = (j < (int)n) ? pairs[s + j] : 0ull;
-            if (j < (int)n) {
-                uint32_t d = (uint32_t)(k[e] >> 32);
-                mn = min(mn, d);
-                mx = max(mx, d);
-            }
-        }
-    } else {
-        for (uint32_t j = tid; j < n; j += NT) {
-            uint32_t d = (uint32_t)(pairs[s + j] >> 32);
-            mn = min(mn, d);
-            mx = max(mx, d);
-        }
+    for (uint32_t j = tid; j < n; j += NT) {
+        const uint64_t p = pairs[s + j];
+        const uint32_t d = (uint32_t)(p >> 32);
+        mn = min(mn, d);
+        mx = max(mx, d);
+        if (resident) keysA[j] = p;
     }
-    block_minmax<NT>(mn, mx, red);
+    block_minmax<NT>(mn, mx, misc);
     const uint32_t span = mx - mn;
-    const int dep_bits = span ? 32 - __clz(span) : 0;
-    const int total_bits = dep_bits + idx_bits;
+    const int total_bits = (span ? 32 - __clz(span) : 0) + idx_bits;
     const int passes = (total_bits + 7) / 8;
     auto xform = [&](uint64_t p) -> uint64_t {
         return ((uint64_t)((uint32_t)(p >> 32) - mn) << idx_bits) | (p & 0xffffffffull);
     };
     if (resident) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            int j = w * 64 * E + e * 64 + lane;
-            if (j < (int)n) k[e] = xform(k[e]);
-        }
+        for (uint32_t j = tid; j < n; j += NT) keysA[j] = xform(keysA[j]);
+        __syncthreads();
+        const int ipt = (int)((n + NT - 1) / NT);
+        uint64_t* src = keysA;
+        uint64_t* dst = keysB;
         for (int p = 0; p < passes; ++p) {
-            radix_chunk<NT, E>(k, (int)n, 8 * p, whist, dbase, tmp, true, xbuf, n);
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                int j = w * 64 * E + e * 64 + lane;
-                if (j < (int)n) k[e] = xbuf[j];
-            }
-            __syncthreads();
+            radix_pass<NT, E>(src, (int)n, ipt, 8 * p, whist, dtot, dbase, misc, dst, nullptr, n);
+            uint64_t* t = src; src = dst; dst = t;
         }
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            int j = w * 64 * E + e * 64 + lane;
-            if (j < (int)n) {
-                sorted_idx[s + j] = (uint32_t)(k[e] & idx_mask);
-                if (keys_out)
-                    keys_out[s + j] = ((uint64_t)tile << 32) | (uint32_t)((k[e] >> idx_bits) + mn);
-            }
+        for (uint32_t j = tid; j < n; j += NT) {
+            const uint64_t key = src[j];
+            sorted_idx[s + j] = (uint32_t)(key & idx_mask);
+            if (keys_out)
+                keys_out[s + j] = ((uint64_t)tile << 32) | (uint32_t)((key >> idx_bits) + mn);
         }
         return;
     }
-    // Streaming: ping-pong between pairs and tmpbuf within [s, s + n).
-    uint64_t* src = pairs + s;
-    uint64_t* dst = tmpbuf + s;
+    // Streaming: each pass = global digit histogram, then chunks in order, stable scatter.
+    uint64_t* gsrc = pairs + s;
+    uint64_t* gdst = tmpbuf + s;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
         if (tid < 256) dbase[tid] = 0;
         __syncthreads();
         for (uint32_t j = tid; j < n; j += NT) {
-            uint64_t key = (p == 0) ? xform(src[j]) : src[j];
+            const uint64_t key = (p == 0) ? xform(gsrc[j]) : gsrc[j];
             atomicAdd(&dbase[(uint32_t)(key >> shift) & 255u], 1u);
         }
         __syncthreads();
-        scan256<NT>(dbase, tmp);
+        scan256(dbase, misc);
         for (uint32_t c0 = 0; c0 < n; c0 += CHUNK) {
-            int cnt = (int)min((uint32_t)CHUNK, n - c0);
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                int j = w * 64 * E + e * 64 + lane;
-                if (j < cnt) {
-                    uint64_t key = src[c0 + j];
-                    k[e] = (p == 0) ? xform(key) : key;
-                } else {
-                    k[e] = 0ull;
-                }
+            const int cnt = (int)min((uint32_t)CHUNK, n - c0);
+            for (int j = tid; j < cnt; j += NT) {
+                const uint64_t key = gsrc[c0 + j];
+                keysA[j] = (p == 0) ? xform(key) : key;
             }
-            radix_chunk<NT, E>(k, cnt, shift, whist, dbase, tmp, false, dst, n);
+            __syncthreads();
+            radix_pass<NT, E>(keysA, cnt, (cnt + NT - 1) / NT, shift, whist, dtot, dbase, misc,
+                              nullptr, gdst, n);
         }
-        // make this pass's global writes visible to the next pass's loads
         __threadfence_block();
         __syncthreads();
-        uint64_t* t2 = src; src = dst; dst = t2;
+        uint64_t* t = gsrc; gsrc = gdst; gdst = t;
     }
     for (uint32_t j = tid; j < n; j += NT) {
-        uint64_t key = src[j];
+        const uint64_t key = gsrc[j];
         sorted_idx[s + j] = (uint32_t)(key & idx_mask);
         if (keys_out) keys_out[s + j] = ((uint64_t)tile << 32) | (uint32_t)((key >> idx_bits) + mn);
     }
@@ -548,7 +551,7 @@ hipError_t launch_project(const float* pos, const float* scale, const float* qua
 }
 
 hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* chunk_pre, int nblk, int ntiles,
-                          uint32_t* ranges, uint32_t* d_m, hipStream_t s) {
+                          uint32_t* ranges, uint32_t* d_m, uint32_t cap, hipStream_t s) {
     int nchunk = (nblk + kScanRows - 1) / kScanRows;
     hipLaunchKernelGGL(k_colscan1, dim3((ntiles + 255) / 256, nchunk), dim3(256), 0, s,
                        blk_counts, chunk_pre, nblk, ntiles);
@@ -556,7 +559,7 @@ hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* chunk_pre, int nblk, i
     if (e != hipSuccess) return e;
     size_t lds = ((size_t)ntiles + 1024 + 32) * 4;
     hipLaunchKernelGGL(k_colscan2, dim3(1), dim3(1024), lds, s, chunk_pre, nchunk, ntiles,
-                       ranges, d_m);
+                       ranges, d_m, cap);
     return hipGetLastError();
 }
 
@@ -575,31 +578,24 @@ hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uin
                             size_t lds_limit, hipStream_t s) {
     int idx_bits = 1;
     while (idx_bits < 32 && ((uint64_t)1 << idx_bits) < (uint64_t)n_gauss) ++idx_bits;
-    // Small tiles: 256 threads x 16 keys resident in LDS; larger ones: 1024 x 16 when the
-    // device allows that much LDS per workgroup, else stream.
-    using Small = SortLds<256, 16>;
-    using Large = SortLds<1024, 16>;
-    size_t small_lds = Small::bytes(true);
-    size_t large_lds = Large::bytes(true);
-    if (large_lds <= lds_limit) {
-        static bool attr_set = false;
-        if (!attr_set) {
-            hipFuncSetAttribute((const void*)k_tile_sort<1024, 16>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)large_lds);
-            attr_set = true;
-        }
-        const uint32_t small_cap = (uint32_t)Small::CHUNK;
-        hipLaunchKernelGGL((k_tile_sort<256, 16>), dim3(ntiles), dim3(256), small_lds, s, ranges,
-                           idx_bits, pairs, tmp, sorted_idx, keys_out, 0u, small_cap);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_tile_sort<1024, 16>), dim3(ntiles), dim3(1024), large_lds, s,
-                           ranges, idx_bits, pairs, tmp, sorted_idx, keys_out, small_cap,
-                           0xffffffffu);
-        return hipGetLastError();
+    // Short lists: 256 threads, 2048 keys resident (~36 KB LDS, several workgroups per CU).
+    // Longer: 512 threads, 8192 keys resident (~137 KB), streaming beyond that.
+    using Small = SortCfg<256, 8>;
+    using Large = SortCfg<512, 16>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void*)k_tile_sort<512, 16>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)Large::kLds);
+        attr_set = true;
     }
-    hipLaunchKernelGGL((k_tile_sort<256, 16>), dim3(ntiles), dim3(256), small_lds, s, ranges,
-                       idx_bits, pairs, tmp, sorted_idx, keys_out, 0u, 0xffffffffu);
+    if (Large::kLds > lds_limit) return hipErrorInvalidConfiguration;
+    const uint32_t small_cap = (uint32_t)Small::CHUNK;
+    hipLaunchKernelGGL((k_tile_sort<256, 8>), dim3(ntiles), dim3(256), Small::kLds, s, ranges,
+                       idx_bits, pairs, tmp, sorted_idx, keys_out, 0u, small_cap);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_tile_sort<512, 16>), dim3(ntiles), dim3(512), Large::kLds, s, ranges,
+                       idx_bits, pairs, tmp, sorted_idx, keys_out, small_cap, 0xffffffffu);
     return hipGetLastError();
 }
 

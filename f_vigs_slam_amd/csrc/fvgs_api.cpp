@@ -51,7 +51,8 @@ struct fvgs_ctx {
     // view
     bool prepared = false, forwarded = false, backwarded = false;
     View view{};
-    uint32_t m = 0;
+    uint32_t m = 0, inst_cap = 0;
+    bool pending_m = false;
     // screen + binning
     DevBuf rec, rect, blk_counts, chunk_pre, ranges, d_m, pairs, tmp, sorted_idx, keys;
     // outputs
@@ -161,8 +162,31 @@ void collect_timing(fvgs_ctx* c, int nmarks) {
     }
 }
 
-// Projection + binning for one view (body of fvgs_prepare_rasterization, mutex held).
-int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int w, int h) {
+constexpr int kOverflow = -100;   // internal: instance buffers too small for this view
+
+// Instance buffers sized for at least m entries; inst_cap = entries every one of them holds.
+int ensure_instances(fvgs_ctx* c, uint64_t m) {
+    TRY(ensure(c, c->pairs, 8ull * m));
+    TRY(ensure(c, c->tmp, 8ull * m));
+    TRY(ensure(c, c->sorted_idx, 4ull * m));
+    uint64_t cap = std::min<uint64_t>(c->pairs.bytes / 8, c->tmp.bytes / 8);
+    cap = std::min<uint64_t>(cap, c->sorted_idx.bytes / 4);
+    if (c->prm.keep_keys) {
+        TRY(ensure(c, c->keys, 8ull * m));
+        cap = std::min<uint64_t>(cap, c->keys.bytes / 8);
+    }
+    c->inst_cap = (uint32_t)std::min<uint64_t>(cap, 0xffffffffull);
+    return FVGS_OK;
+}
+
+// Projection + binning for one view (mutex held).
+//   async = false: project, scan, read M on the host, size the instance buffers, emit, sort —
+//                  the reference's order (one host round trip, like its nb_instances read).
+//   async = true:  enqueue everything with the current instance capacity and no host wait; the
+//                  scan empties every range if M exceeds the capacity, and the caller checks
+//                  overflowed() after its own final synchronisation and re-runs synchronously.
+int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int w, int h,
+            bool async) {
     if (!pose || !intr) return fail(c, FVGS_E_ARG, "null pose or intrinsics");
     if (w <= 0 || h <= 0) return fail(c, FVGS_E_ARG, "invalid image size %d x %d", w, h);
     if (!finite_pose(pose)) return fail(c, FVGS_E_ARG, "non-finite pose");
@@ -190,35 +214,45 @@ int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int
     TRY(ensure(c, c->chunk_pre, 4ull * nchunk * T));
     TRY(ensure(c, c->ranges, 8ull * T));
     TRY(ensure(c, c->d_m, 16));
+    if (async && c->inst_cap == 0) async = false;
     hipStream_t s = c->stream;
+    c->view = v;
     mark(c, 0);
     HIPCHK(c, launch_project(c->pos, c->scale, c->quat, c->color, c->opac, n, v,
                              c->prm.cull_emit, as<Rec>(c->rec), as<uint2>(c->rect),
                              as<uint32_t>(c->blk_counts), s));
     HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->chunk_pre), nblk, T,
-                             as<uint32_t>(c->ranges), as<uint32_t>(c->d_m), s));
+                             as<uint32_t>(c->ranges), as<uint32_t>(c->d_m),
+                             async ? c->inst_cap : 0xffffffffu, s));
     HIPCHK(c, hipMemcpyAsync(c->h_m, c->d_m.p, 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipStreamSynchronize(s));
-    const uint32_t m = *c->h_m;
-    c->view = v;
-    c->m = m;
-    if (m == 0) return fail(c, FVGS_E_EMPTY, "no tile instances");
-    TRY(ensure(c, c->pairs, 8ull * m));
-    TRY(ensure(c, c->tmp, 8ull * m));
-    TRY(ensure(c, c->sorted_idx, 4ull * m));
-    uint64_t* keys = nullptr;
-    if (c->prm.keep_keys) {
-        TRY(ensure(c, c->keys, 8ull * m));
-        keys = as<uint64_t>(c->keys);
+    if (!async) {
+        HIPCHK(c, hipStreamSynchronize(s));
+        c->m = *c->h_m;
+        if (c->m == 0) return fail(c, FVGS_E_EMPTY, "no tile instances");
+        if (c->m > c->inst_cap) TRY(ensure_instances(c, c->m));
     }
+    uint64_t* keys = c->prm.keep_keys ? as<uint64_t>(c->keys) : nullptr;
     HIPCHK(c, launch_emit(as<Rec>(c->rec), as<uint2>(c->rect), as<uint32_t>(c->blk_counts),
-                          as<uint32_t>(c->chunk_pre), n, v, as<uint64_t>(c->pairs),
-                          c->pairs.bytes / 8, s));
+                          as<uint32_t>(c->chunk_pre), n, v, as<uint64_t>(c->pairs), c->inst_cap,
+                          s));
     HIPCHK(c, launch_tile_sort(as<uint32_t>(c->ranges), T, n, as<uint64_t>(c->pairs),
                                as<uint64_t>(c->tmp), as<uint32_t>(c->sorted_idx), keys,
                                c->lds_limit, s));
     mark(c, 1);
     c->prepared = true;
+    c->pending_m = async;
+    return FVGS_OK;
+}
+
+// After the stream has been synchronised: resolve M of an asynchronous prepare. Returns
+// kOverflow when the instance buffers overflowed (the caller re-runs synchronously) and
+// FVGS_E_EMPTY when the view has no instances.
+int resolve_m(fvgs_ctx* c) {
+    if (!c->pending_m) return FVGS_OK;
+    c->pending_m = false;
+    c->m = *c->h_m;
+    if (c->m > c->inst_cap) return kOverflow;
+    if (c->m == 0) return fail(c, FVGS_E_EMPTY, "no tile instances");
     return FVGS_OK;
 }
 
@@ -277,7 +311,7 @@ int run_pose(fvgs_ctx* c, bool want_render) {
     if (!c->gx || !c->gy) return fail(c, FVGS_E_STATE, "frame has no image gradients");
     const int T = c->view.ntx * c->view.nty;
     const size_t P = (size_t)c->view.W * c->view.H;
-    TRY(ensure(c, c->pose_tiles, 27 * 4ull * T));
+    TRY(ensure(c, c->pose_tiles, 27 * 8ull * T));
     TRY(ensure(c, c->pose_sum, 27 * 8));
     float *trgb = nullptr, *tdep = nullptr;
     if (want_render) {
@@ -288,8 +322,8 @@ int run_pose(fvgs_ctx* c, bool want_render) {
     }
     HIPCHK(c, launch_pose(c->view, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
                           as<Rec>(c->rec), c->obs_rgb, c->obs_depth, c->gx, c->gy,
-                          blend_params(c->prm), as<float>(c->pose_tiles), trgb, tdep, c->stream));
-    HIPCHK(c, launch_pose_reduce(as<float>(c->pose_tiles), T, as<double>(c->pose_sum), c->stream));
+                          blend_params(c->prm), as<double>(c->pose_tiles), trgb, tdep, c->stream));
+    HIPCHK(c, launch_pose_reduce(as<double>(c->pose_tiles), T, as<double>(c->pose_sum), c->stream));
     HIPCHK(c, hipMemcpyAsync(c->h_sum, c->pose_sum.p, 27 * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return FVGS_OK;
@@ -331,7 +365,7 @@ OutDesc describe(fvgs_ctx* c, int what) {
         case FVGS_OUT_MAX_CONTRIB: return {c->forwarded ? c->o_maxc.p : nullptr, 4 * T};
         case FVGS_OUT_DELTA2D: return {c->backwarded ? c->d2.p : nullptr, sizeof(Delta2D) * n};
         case FVGS_OUT_DELTA3D: return {c->d3.p, sizeof(Delta3D) * n};
-        case FVGS_OUT_POSE_TILES: return {c->pose_tiles.p, 108 * T};
+        case FVGS_OUT_POSE_TILES: return {c->pose_tiles.p, 216 * T};
         case FVGS_OUT_TRACK_RGB: return {c->t_rgb.p, 12 * P};
         case FVGS_OUT_TRACK_DEPTH: return {c->t_depth.p, 4 * P};
         default: return {nullptr, 0};
@@ -480,7 +514,7 @@ int fvgs_prepare_rasterization(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_in
     if (!c) return FVGS_E_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
     (void)hipSetDevice(c->prm.device);
-    int r = prepare(c, pose, intr, w, h);
+    int r = prepare(c, pose, intr, w, h, false);
     if (n_instances) *n_instances = (r == FVGS_OK || r == FVGS_E_EMPTY) ? c->m : 0;
     if (r == FVGS_OK && c->timing) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -534,12 +568,18 @@ int fvgs_mapping_step(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics*
     std::lock_guard<std::mutex> lk(c->mu);
     (void)hipSetDevice(c->prm.device);
     if (!c->obs_rgb) return fail(c, FVGS_E_STATE, "no frame set");
-    TRY(prepare(c, pose, intr, c->fw, c->fh));
-    TRY(run_forward(c, true));
-    mark(c, 3);
-    TRY(run_delta3d(c));
-    mark(c, 4);
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    auto attempt = [&](bool async) -> int {
+        TRY(prepare(c, pose, intr, c->fw, c->fh, async));
+        TRY(run_forward(c, true));
+        mark(c, 3);
+        TRY(run_delta3d(c));
+        mark(c, 4);
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return resolve_m(c);
+    };
+    int r = attempt(true);
+    if (r == kOverflow) r = attempt(false);
+    if (r != FVGS_OK) return r;
     collect_timing(c, 5);
     return FVGS_OK;
 }
@@ -550,8 +590,14 @@ int fvgs_pose_jacobians(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsic
     std::lock_guard<std::mutex> lk(c->mu);
     (void)hipSetDevice(c->prm.device);
     if (!c->obs_rgb) return fail(c, FVGS_E_STATE, "no frame set");
-    TRY(prepare(c, pose, intr, c->fw, c->fh));
-    TRY(run_pose(c, false));
+    auto attempt = [&](bool async) -> int {
+        TRY(prepare(c, pose, intr, c->fw, c->fh, async));
+        TRY(run_pose(c, false));   // synchronises
+        return resolve_m(c);
+    };
+    int r = attempt(true);
+    if (r == kOverflow) r = attempt(false);
+    if (r != FVGS_OK) return r;
     for (int k = 0; k < 6; k++) jtr6[k] = c->h_sum[k];
     for (int k = 0; k < 21; k++) jtj21[k] = c->h_sum[6 + k];
     return FVGS_OK;
@@ -562,15 +608,15 @@ int fvgs_render_tracking(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsi
     if (!c) return FVGS_E_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
     (void)hipSetDevice(c->prm.device);
-    TRY(prepare(c, pose, intr, w, h));
+    TRY(prepare(c, pose, intr, w, h, false));
     const int T = c->view.ntx * c->view.nty;
     const size_t P = (size_t)w * h;
-    TRY(ensure(c, c->pose_tiles, 27 * 4ull * T));
+    TRY(ensure(c, c->pose_tiles, 27 * 8ull * T));
     TRY(ensure(c, c->t_rgb, 12 * P));
     TRY(ensure(c, c->t_depth, 4 * P));
     HIPCHK(c, launch_pose(c->view, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
                           as<Rec>(c->rec), nullptr, nullptr, nullptr, nullptr,
-                          blend_params(c->prm), as<float>(c->pose_tiles), as<float>(c->t_rgb),
+                          blend_params(c->prm), as<double>(c->pose_tiles), as<float>(c->t_rgb),
                           as<float>(c->t_depth), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return FVGS_OK;

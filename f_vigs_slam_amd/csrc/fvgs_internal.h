@@ -55,7 +55,7 @@ hipError_t launch_project(const float* pos, const float* scale, const float* qua
                           int cull_emit, Rec* rec, uint2* rect, uint32_t* blk_counts,
                           hipStream_t s);
 hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* chunk_pre, int nblk, int ntiles,
-                          uint32_t* ranges, uint32_t* d_m, hipStream_t s);
+                          uint32_t* ranges, uint32_t* d_m, uint32_t cap, hipStream_t s);
 hipError_t launch_emit(const Rec* rec, const uint2* rect, const uint32_t* blk_counts,
                        const uint32_t* chunk_pre, int n, const View& v, uint64_t* pairs,
                        uint64_t cap, hipStream_t s);
@@ -79,8 +79,8 @@ hipError_t launch_delta3d(const float* pos, const float* scale, const float* qua
 hipError_t launch_pose(const View& v, const uint32_t* ranges, const uint32_t* idx,
                        const Rec* rec, const float* obs_rgb, const float* obs_depth,
                        const float* gx, const float* gy, const BlendParams& bp,
-                       float* tiles27, float* out_rgb, float* out_depth, hipStream_t s);
-hipError_t launch_pose_reduce(const float* tiles27, int ntiles, double* out27, hipStream_t s);
+                       double* tiles27, float* out_rgb, float* out_depth, hipStream_t s);
+hipError_t launch_pose_reduce(const double* tiles27, int ntiles, double* out27, hipStream_t s);
 hipError_t launch_sobel(const float* img, int W, int H, float* gx, float* gy, hipStream_t s);
 
 }  // namespace fvgs

@@ -12,7 +12,8 @@
 //  * backward: each 64-lane wave replays two 32-entry buckets (lanes 0-31 and 32-63); lane l owns
 //    entry b*32 + (l & 31) and the per-pixel transmittance/colour state flows lane to lane, so
 //    step i of lane l handles pixel i - (l & 31) — the reference's warp pipeline widened to
-//    wave64. Per-pixel forward results are read from LDS; one atomicAdd per field and entry.
+//    wave64, with the state moved by DPP wave shifts. Per-pixel forward results are read from
+//    LDS; one atomicAdd per field and entry.
 // Reference defects Q8, Q9, Q10 and Q14 are transcribed (see the oracle header).
 #include "fvgs_internal.h"
 
@@ -21,6 +22,13 @@ namespace fvgs {
 namespace {
 
 constexpr float kAlphaMin = 1.0f / 255.0f;
+
+// Lane l receives lane l-1's value across the whole wave64 (DPP wave_shr:1, one VALU op);
+// lane 0 keeps its own value, as __shfl_up(v, 1) does.
+__device__ __forceinline__ float lane_shift_up(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                      0x138 /* wave_shr:1 */, 0xf, 0xf, false));
+}
 
 // ---- A12 ------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_render(View v, const uint32_t* __restrict__ ranges,
@@ -179,10 +187,10 @@ __global__ __launch_bounds__(256) void k_mapping(
     float bT = 0.f, ar = 0.f, ag = 0.f, ab = 0.f;
     const int wlim = v.W - tx0, hlim = v.H - ty0;
     for (int i = 0; i < 256 + 31; ++i) {
-        bT = __shfl_up(bT, 1);
-        ar = __shfl_up(ar, 1);
-        ag = __shfl_up(ag, 1);
-        ab = __shfl_up(ab, 1);
+        bT = lane_shift_up(bT);
+        ar = lane_shift_up(ar);
+        ag = lane_shift_up(ag);
+        ab = lane_shift_up(ab);
         const int p = i - sl;
         const bool pix_ok = p >= 0 && p < 256 && (p & 15) < wlim && (p >> 4) < hlim;
         if (!(valid && pix_ok)) continue;

@@ -31,8 +31,11 @@ __device__ __forceinline__ void qmat_t(float x, float y, float z, float w, float
 }
 
 // acc[0..5] += Jpose * b ; acc[6..26] += upper(Jpose * H * Jpose^T) with
-// Jpose = [R ; d * [ray]x] (6x3) (GSCudaKernels.cu:1116-1165, 1293-1336).
-__device__ __forceinline__ void accumulate_pose(float acc[27], const float H[9], const float b3[3],
+// Jpose = [R ; d * [ray]x] (6x3) (GSCudaKernels.cu:1116-1165, 1293-1336). Each term is formed in
+// FP32 as in the reference; the running sums are FP64: JtJ sums terms spanning ~10 orders of
+// magnitude (pass-2 terms carry 1/(1 - alpha)), and FP32 running sums lose positive
+// definiteness.
+__device__ __forceinline__ void accumulate_pose(double acc[27], const float H[9], const float b3[3],
                                                 const float R[3][3], float d, const float ray[3]) {
     float Jp[18];
 #pragma unroll
@@ -54,11 +57,11 @@ __device__ __forceinline__ void accumulate_pose(float acc[27], const float H[9],
     for (int r = 0; r < 6; ++r)
 #pragma unroll
         for (int c = r; c < 6; ++c)
-            acc[id++] += tmp[r * 3] * Jp[c * 3] + tmp[r * 3 + 1] * Jp[c * 3 + 1] +
-                         tmp[r * 3 + 2] * Jp[c * 3 + 2];
+            acc[id++] += (double)(tmp[r * 3] * Jp[c * 3] + tmp[r * 3 + 1] * Jp[c * 3 + 1] +
+                                  tmp[r * 3 + 2] * Jp[c * 3 + 2]);
 #pragma unroll
     for (int r = 0; r < 6; ++r)
-        acc[r] += Jp[r * 3] * b3[0] + Jp[r * 3 + 1] * b3[1] + Jp[r * 3 + 2] * b3[2];
+        acc[r] += (double)(Jp[r * 3] * b3[0] + Jp[r * 3 + 1] * b3[1] + Jp[r * 3 + 2] * b3[2]);
 }
 
 __global__ __launch_bounds__(256) void k_pose(View v, const uint32_t* __restrict__ ranges,
@@ -68,11 +71,11 @@ __global__ __launch_bounds__(256) void k_pose(View v, const uint32_t* __restrict
                                               const float* __restrict__ obs_depth,
                                               const float* __restrict__ gxi,
                                               const float* __restrict__ gyi, BlendParams bp,
-                                              float* __restrict__ tiles27,
+                                              double* __restrict__ tiles27,
                                               float* __restrict__ out_rgb,
                                               float* __restrict__ out_depth) {
     __shared__ Rec recs[256];
-    __shared__ float red[4][27];
+    __shared__ double red[4][27];
     const int tile = blockIdx.x, tid = threadIdx.x;
     const int x = (tile % v.ntx) * kTile + (tid & 15), y = (tile / v.ntx) * kTile + (tid >> 4);
     const bool in_bounds = x < v.W && y < v.H;
@@ -84,9 +87,9 @@ __global__ __launch_bounds__(256) void k_pose(View v, const uint32_t* __restrict
     const uint32_t s = ranges[2 * tile], n = ranges[2 * tile + 1] - s;
     const float px = (float)x, py = (float)y;
 
-    float acc[27];
+    double acc[27];
 #pragma unroll
-    for (int k = 0; k < 27; ++k) acc[k] = 0.f;
+    for (int k = 0; k < 27; ++k) acc[k] = 0.0;
 
     // ---------------- pass 1: render (GSCudaKernels.cu:978-1036) ----------------
     float C0 = 0.f, C1 = 0.f, C2 = 0.f, D = 0.f, final_T = 1.0f;
@@ -243,7 +246,7 @@ __global__ __launch_bounds__(256) void k_pose(View v, const uint32_t* __restrict
     const int w = tid >> 6, lane = tid & 63;
 #pragma unroll
     for (int k = 0; k < 27; ++k) {
-        float vk = acc[k];
+        double vk = acc[k];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) vk += __shfl_xor(vk, o);
         if (lane == 0) red[w][k] = vk;
@@ -253,12 +256,12 @@ __global__ __launch_bounds__(256) void k_pose(View v, const uint32_t* __restrict
 }
 
 // FP64 sum of the per-tile partials: block k sums component k over tiles in a fixed order.
-__global__ __launch_bounds__(256) void k_pose_reduce(const float* __restrict__ tiles27, int ntiles,
+__global__ __launch_bounds__(256) void k_pose_reduce(const double* __restrict__ tiles27, int ntiles,
                                                      double* __restrict__ out27) {
     __shared__ double part[256];
     const int k = blockIdx.x, tid = threadIdx.x;
     double sum = 0.0;
-    for (int t = tid; t < ntiles; t += 256) sum += (double)tiles27[(size_t)t * 27 + k];
+    for (int t = tid; t < ntiles; t += 256) sum += tiles27[(size_t)t * 27 + k];
     part[tid] = sum;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
@@ -305,14 +308,14 @@ hipError_t launch_sobel(const float* img, int W, int H, float* gx, float* gy, hi
 
 hipError_t launch_pose(const View& v, const uint32_t* ranges, const uint32_t* idx,
                        const Rec* rec, const float* obs_rgb, const float* obs_depth,
-                       const float* gx, const float* gy, const BlendParams& bp, float* tiles27,
+                       const float* gx, const float* gy, const BlendParams& bp, double* tiles27,
                        float* out_rgb, float* out_depth, hipStream_t s) {
     hipLaunchKernelGGL(k_pose, dim3(v.ntx * v.nty), dim3(256), 0, s, v, ranges, idx, rec, obs_rgb,
                        obs_depth, gx, gy, bp, tiles27, out_rgb, out_depth);
     return hipGetLastError();
 }
 
-hipError_t launch_pose_reduce(const float* tiles27, int ntiles, double* out27, hipStream_t s) {
+hipError_t launch_pose_reduce(const double* tiles27, int ntiles, double* out27, hipStream_t s) {
     hipLaunchKernelGGL(k_pose_reduce, dim3(27), dim3(256), 0, s, tiles27, ntiles, out27);
     return hipGetLastError();
 }

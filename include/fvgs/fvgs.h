@@ -135,7 +135,8 @@ enum fvgs_out {
     FVGS_OUT_MAX_CONTRIB = 25, /* uint32[T]  per-tile max of n_contrib */
     FVGS_OUT_DELTA2D = 30,     /* fvgs_delta2d[N] */
     FVGS_OUT_DELTA3D = 31,     /* fvgs_delta3d[N] */
-    FVGS_OUT_POSE_TILES = 40,  /* fvgs_pose_data[T] per-tile partials of the last pose call */
+    FVGS_OUT_POSE_TILES = 40,  /* double[T][27] per-tile partials of the last pose call, in the
+                                  order of fvgs_pose_data (Jtr[6], then the JtJ upper triangle) */
     FVGS_OUT_TRACK_RGB = 41,   /* float3[P]  pass-1 render of the last pose call */
     FVGS_OUT_TRACK_DEPTH = 42  /* float[P]   pass-1 median depth of the last pose call */
 };

@@ -52,3 +52,14 @@ def grad_agreement(got, ref, rel=1e-3):
     scale = np.abs(ref).max() if ref.size else 0.0
     ok = np.abs(got - ref) <= rel * (np.abs(ref) + 1e-3 * scale)
     return float(ok.mean())
+
+
+def grad_report(got, ref):
+    """Error summary of a gradient field: max-norm relative error |g - r|_inf / |r|_inf and
+    quantiles of the per-entry error normalised the same way."""
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    scale = max(np.abs(ref).max(), 1e-30)
+    e = np.abs(got - ref).max(axis=-1) / scale if got.ndim > 1 else np.abs(got - ref) / scale
+    return dict(maxnorm=float(e.max()), q50=float(np.quantile(e, 0.5)),
+                q99=float(np.quantile(e, 0.99)), q999=float(np.quantile(e, 0.999)))
