@@ -2,12 +2,13 @@
 """Benchmark of the hot path on one MI355X: fwd+bwd Mpix/s and plain tracking-step latency.
 
 Workload (one "step"): the 1M-Gaussian, 640x480 mapping view — projection, (tile, depth)
-binning, fused forward + backward blend and the 2D->3D gradients (fvgs_mapping_step) — against a
+binning, blend forward and backward and the 2D->3D gradients (fvgs_mapping_step) — against a
 synthetic observation (the GPU's own forward render minus N(0, 1e-2) perturbations), FP32. The
 map and the frame are resident in HBM before the timed region.
 
 Printed JSON line: value = pixels / step time (Mpix/s); `roofline` for the dominant kernel (the
-fused blend), measured with the library's HIP events on its own stream over the timed steps;
+slower of the blend forward and backward), measured with the library's HIP events on its own
+stream over the timed steps;
 `cpu_baseline` = the FP64 oracle (OpenMP) timed on the host cores over the same view; `tracking`
 = plain Gauss-Newton tracking steps (500k Gaussians, 640x480, 20 iterations from a 2 deg / 3 cm
 perturbation), p50/p99 step latency.
@@ -141,7 +142,7 @@ def main():
     for _ in range(args.warmup):
         ctx.mapping_step(pose, intr)
     ctx.set_timing(True)
-    stage = np.zeros(4)
+    stage = np.zeros(5)
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -158,14 +159,21 @@ def main():
         for _ in range(args.steps):
             ctx.forward()
     log(f"step {t * 1e3:.3f} ms; stages binning {stage[0]:.3f} clear {stage[1]:.3f} "
-        f"blend {stage[2]:.3f} 3d {stage[3]:.3f} ms; M={M} K={K}")
-    # Algorithmic bytes (compulsory traffic, each counted once):
-    #   fused blend kernel: 52 B per walked entry in the forward and again in the backward
-    #   (index 4 + record 48) + 36 B per pixel (16 B observed in, 20 B rendered out);
+        f"forward {stage[2]:.3f} backward {stage[3]:.3f} 3d {stage[4]:.3f} ms; M={M} K={K}")
+    # Algorithmic bytes (compulsory traffic, each counted once), K = walked entries, P = pixels:
+    #   k_forward:  52 B per walked entry (index 4 + 48 B of record) + 48 B per pixel
+    #               (12 B observed colour in; colour 12, depth 4, T 4, n_contrib 4, error 12 out);
+    #   k_backward: 52 B per walked entry + 52 B per pixel (colour 12, depth 4, n_contrib 4,
+    #               error 12, observed depth 4 in; one 16 B checkpoint per pixel and bucket);
     #   whole step: 108 N + 44 M + 104 K + 36 P.
-    blend_bytes = 104.0 * K + 36.0 * P
-    blend_s = stage[2] * 1e-3
-    achieved = blend_bytes / blend_s / 1e9
+    # The roofline line is for the slower blend kernel, each timed by its own events.
+    fwd_bytes = 52.0 * K + 48.0 * P
+    bwd_bytes = 52.0 * K + 52.0 * P
+    if stage[3] >= stage[2]:
+        kname, kbytes, kms = "k_backward (blend backward, A15)", bwd_bytes, stage[3]
+    else:
+        kname, kbytes, kms = "k_forward (blend forward, A14)", fwd_bytes, stage[2]
+    achieved = kbytes / (kms * 1e-3) / 1e9
     step_bytes = 108.0 * n + 44.0 * M + 104.0 * K + 36.0 * P
     result = {
         "metric": "fwd+bwd Mpix/s @1M Gaussians 640x480; p99 tracking-step ms, served vs plain",
@@ -180,16 +188,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded config-0 distributions; observation = own render - N(0,1e-2))",
-        "config": {"workload": "plain fwd+bwd view: projection, binning, fused blend "
-                               "forward+backward, 2D->3D gradients",
+        "config": {"workload": "plain fwd+bwd view: projection, binning, blend forward, blend "
+                               "backward, 2D->3D gradients",
                    "n_gaussians": n, "width": w, "height": h, "instances_M": M,
                    "walked_K": K, "pixels_P": P,
                    "stage_ms": {"binning": round(stage[0], 4), "clear": round(stage[1], 4),
-                                "blend": round(stage[2], 4), "grad3d": round(stage[3], 4)},
+                                "forward": round(stage[2], 4), "backward": round(stage[3], 4),
+                                "grad3d": round(stage[4], 4)},
                    "step_algorithmic_GBs": round(step_bytes / t / 1e9, 2)},
-        "roofline": {"kernel": "k_mapping (fused forward+backward blend)", "bound": "hbm",
+        "roofline": {"kernel": kname, "bound": "hbm",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "algorithmic_bytes": int(kbytes), "avg_ms": round(kms, 4)},
     }
     ctx.close()
     if not args.no_tracking:

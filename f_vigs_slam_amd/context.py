@@ -191,7 +191,8 @@ class Context:
         self._check(self._lib.fvgs_set_timing(self._h, 1 if on else 0))
 
     def last_timing(self):
-        ms = (ctypes.c_float * 4)()
+        """Stage times of the last mapping step (ms): binning, clear, forward, backward, 2D->3D."""
+        ms = (ctypes.c_float * 5)()
         self._check(self._lib.fvgs_last_timing(self._h, ms))
         return list(ms)
 

@@ -36,8 +36,8 @@ struct fvgs_ctx {
     size_t lds_limit = 65536;
     size_t allocated = 0;
     bool timing = false;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    float last_ms[4] = {0, 0, 0, 0};
+    hipEvent_t ev[FVGS_TIMING_STAGES + 1] = {};
+    float last_ms[FVGS_TIMING_STAGES] = {};
 
     // map
     uint32_t n = 0;
@@ -58,7 +58,7 @@ struct fvgs_ctx {
     // outputs
     DevBuf r_rgb, r_depth, r_alpha;
     DevBuf o_rgb, o_depth, o_T, o_nc, o_err, o_maxc;
-    DevBuf d2, d3;
+    DevBuf d2, d3, ckpt;
     DevBuf pose_tiles, pose_sum, t_rgb, t_depth;
     uint32_t* h_m = nullptr;      // pinned
     double* h_sum = nullptr;      // pinned, 27
@@ -276,24 +276,38 @@ int check_frame(fvgs_ctx* c) {
     return FVGS_OK;
 }
 
-int run_forward(fvgs_ctx* c, bool with_backward) {
+int run_forward(fvgs_ctx* c) {
     if (!c->prepared) return fail(c, FVGS_E_STATE, "call fvgs_prepare_rasterization first");
     TRY(check_frame(c));
     TRY(alloc_pixel_outputs(c));
-    Delta2D* d2 = nullptr;
-    if (with_backward) {
-        TRY(ensure(c, c->d2, sizeof(Delta2D) * (size_t)c->n));
-        d2 = as<Delta2D>(c->d2);
-        HIPCHK(c, hipMemsetAsync(d2, 0, sizeof(Delta2D) * (size_t)c->n, c->stream));
-    }
-    mark(c, 2);
-    HIPCHK(c, launch_mapping(c->view, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
-                             as<Rec>(c->rec), c->obs_rgb, c->obs_depth, blend_params(c->prm),
+    const int T = c->view.ntx * c->view.nty;
+    TRY(ensure(c, c->ckpt, sizeof(float4) * kMaxBuckets * 256ull * T));
+    HIPCHK(c, launch_forward(c->view, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
+                             as<Rec>(c->rec), c->obs_rgb, blend_params(c->prm),
                              as<float>(c->o_rgb), as<float>(c->o_depth), as<float>(c->o_T),
-                             as<uint32_t>(c->o_nc), as<float>(c->o_err),
-                             as<uint32_t>(c->o_maxc), d2, with_backward ? 1 : 0, c->stream));
+                             as<uint32_t>(c->o_nc), as<float>(c->o_err), as<uint32_t>(c->o_maxc),
+                             as<float4>(c->ckpt), c->stream));
     c->forwarded = true;
-    c->backwarded = with_backward;
+    c->backwarded = false;
+    return FVGS_OK;
+}
+
+// Clears the 2D gradient accumulators (before the forward, so that the timing marks bracket the
+// clear) and runs the backward over the forward's outputs.
+int clear_delta2d(fvgs_ctx* c) {
+    TRY(ensure(c, c->d2, sizeof(Delta2D) * (size_t)c->n));
+    HIPCHK(c, hipMemsetAsync(c->d2.p, 0, sizeof(Delta2D) * (size_t)c->n, c->stream));
+    return FVGS_OK;
+}
+
+int run_backward(fvgs_ctx* c) {
+    if (!c->forwarded) return fail(c, FVGS_E_STATE, "call fvgs_forward first");
+    HIPCHK(c, launch_backward(c->view, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
+                              as<Rec>(c->rec), c->obs_depth, blend_params(c->prm),
+                              as<float>(c->o_rgb), as<float>(c->o_depth), as<uint32_t>(c->o_nc),
+                              as<float>(c->o_err), as<uint32_t>(c->o_maxc), as<float4>(c->ckpt),
+                              as<Delta2D>(c->d2), c->stream));
+    c->backwarded = true;
     return FVGS_OK;
 }
 
@@ -445,7 +459,7 @@ int fvgs_destroy(fvgs_ctx* c) {
                           &c->blk_counts, &c->chunk_pre, &c->ranges, &c->d_m, &c->pairs,
                           &c->tmp, &c->sorted_idx, &c->keys, &c->r_rgb, &c->r_depth, &c->r_alpha,
                           &c->o_rgb, &c->o_depth, &c->o_T, &c->o_nc, &c->o_err, &c->o_maxc,
-                          &c->d2, &c->d3, &c->pose_tiles, &c->pose_sum, &c->t_rgb, &c->t_depth};
+                          &c->d2, &c->d3, &c->ckpt, &c->pose_tiles, &c->pose_sum, &c->t_rgb, &c->t_depth};
         for (DevBuf* b : bufs) release(c, *b);
         for (auto& e : c->ev)
             if (e) (void)hipEventDestroy(e);
@@ -497,6 +511,7 @@ int fvgs_set_frame(fvgs_ctx* c, int w, int h, const float* rgb3, const float* de
     (void)hipSetDevice(c->prm.device);
     const size_t P = (size_t)w * h;
     c->obs_rgb = c->obs_depth = c->gx = c->gy = nullptr;
+    c->forwarded = c->backwarded = false;   // the blend outputs were against the old frame
     TRY(upload(c, c->own_rgb, c->obs_rgb, rgb3, 12 * P, mem));
     TRY(upload(c, c->own_depth, c->obs_depth, depth, 4 * P, mem));
     if (gx3 && gy3) {
@@ -541,7 +556,7 @@ int fvgs_render(fvgs_ctx* c) {
 int fvgs_forward(fvgs_ctx* c) {
     if (!c) return FVGS_E_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
-    TRY(run_forward(c, false));
+    TRY(run_forward(c));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return FVGS_OK;
 }
@@ -549,8 +564,11 @@ int fvgs_forward(fvgs_ctx* c) {
 int fvgs_backward(fvgs_ctx* c) {
     if (!c) return FVGS_E_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
-    // The fused kernel recomputes the forward; its outputs are identical to fvgs_forward's.
-    TRY(run_forward(c, true));
+    // Uses the outputs of the last fvgs_forward on this view (the reference's order,
+    // GSSlam.cu:1192-1240); runs the forward first when it has not run.
+    if (!c->forwarded) TRY(run_forward(c));
+    TRY(clear_delta2d(c));
+    TRY(run_backward(c));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return FVGS_OK;
 }
@@ -570,17 +588,21 @@ int fvgs_mapping_step(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics*
     if (!c->obs_rgb) return fail(c, FVGS_E_STATE, "no frame set");
     auto attempt = [&](bool async) -> int {
         TRY(prepare(c, pose, intr, c->fw, c->fh, async));
-        TRY(run_forward(c, true));
+        TRY(clear_delta2d(c));
+        mark(c, 2);
+        TRY(run_forward(c));
         mark(c, 3);
-        TRY(run_delta3d(c));
+        TRY(run_backward(c));
         mark(c, 4);
+        TRY(run_delta3d(c));
+        mark(c, 5);
         HIPCHK(c, hipStreamSynchronize(c->stream));
         return resolve_m(c);
     };
     int r = attempt(true);
     if (r == kOverflow) r = attempt(false);
     if (r != FVGS_OK) return r;
-    collect_timing(c, 5);
+    collect_timing(c, 6);
     return FVGS_OK;
 }
 
@@ -796,10 +818,10 @@ int fvgs_set_timing(fvgs_ctx* c, int enable) {
     return FVGS_OK;
 }
 
-int fvgs_last_timing(fvgs_ctx* c, float ms[4]) {
+int fvgs_last_timing(fvgs_ctx* c, float ms[FVGS_TIMING_STAGES]) {
     if (!c || !ms) return FVGS_E_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
-    for (int i = 0; i < 4; i++) ms[i] = c->last_ms[i];
+    for (int i = 0; i < FVGS_TIMING_STAGES; i++) ms[i] = c->last_ms[i];
     return FVGS_OK;
 }
 

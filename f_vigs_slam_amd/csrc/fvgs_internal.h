@@ -66,11 +66,15 @@ hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uin
 // ---- blending (raster.hip) ----
 hipError_t launch_render(const View& v, const uint32_t* ranges, const uint32_t* idx,
                          const Rec* rec, float* rgb, float* depth, float* alpha, hipStream_t s);
-hipError_t launch_mapping(const View& v, const uint32_t* ranges, const uint32_t* idx,
-                          const Rec* rec, const float* obs_rgb, const float* obs_depth,
-                          const BlendParams& bp, float* rgb, float* depth, float* final_T,
-                          uint32_t* n_contrib, float* color_err, uint32_t* max_contrib,
-                          Delta2D* delta, int do_backward, hipStream_t s);
+hipError_t launch_forward(const View& v, const uint32_t* ranges, const uint32_t* idx,
+                          const Rec* rec, const float* obs_rgb, const BlendParams& bp, float* rgb,
+                          float* depth, float* final_T, uint32_t* n_contrib, float* color_err,
+                          uint32_t* max_contrib, float4* ckpt, hipStream_t s);
+hipError_t launch_backward(const View& v, const uint32_t* ranges, const uint32_t* idx,
+                           const Rec* rec, const float* obs_depth, const BlendParams& bp,
+                           const float* rgb, const float* depth, const uint32_t* n_contrib,
+                           const float* color_err, const uint32_t* max_contrib,
+                           const float4* ckpt, Delta2D* delta, hipStream_t s);
 hipError_t launch_delta3d(const float* pos, const float* scale, const float* quat, int n,
                           const View& v, float lambda_iso, const Delta2D* d2, Delta3D* d3,
                           hipStream_t s);

@@ -4,16 +4,10 @@
 // (2092-2263), optimizeGaussiansPerGaussianPass (2265-2502) and computeDeltaGaussians3D_kernel
 // (2504-2720).
 //
-// k_mapping fuses the reference's forward and per-bucket backward into one workgroup per tile:
-//  * forward: one thread per pixel walks the depth-sorted tile list in LDS-staged batches of
-//    256 records (64 B each); the (T, colour) checkpoints at list indices 0, 32, ..., 224 stay in
-//    LDS instead of a per-call [buckets x 256] HBM buffer, since only the first 256 entries of a
-//    tile receive gradients (GSCudaKernels.cu:2329-2330);
-//  * backward: each 64-lane wave replays two 32-entry buckets (lanes 0-31 and 32-63); lane l owns
-//    entry b*32 + (l & 31) and the per-pixel transmittance/colour state flows lane to lane, so
-//    step i of lane l handles pixel i - (l & 31) — the reference's warp pipeline widened to
-//    wave64, with the state moved by DPP wave shifts. Per-pixel forward results are read from
-//    LDS; one atomicAdd per field and entry.
+// k_forward: one thread per pixel walks the depth-sorted tile list in LDS-staged batches of 256
+// records (64 B each), with a wave-uniform early exit; checkpoints only for the 8 buckets that
+// the backward can use. k_backward: the per-bucket replay of the reference as a wave64 lane
+// pipeline.
 // Reference defects Q8, Q9, Q10 and Q14 are transcribed (see the oracle header).
 #include "fvgs_internal.h"
 
@@ -82,105 +76,134 @@ __global__ __launch_bounds__(256) void k_render(View v, const uint32_t* __restri
     }
 }
 
-// ---- A14 + A15 ------------------------------------------------------------------------------
-struct MapLds {
-    Rec recs[256];
-    float ckT[kMaxBuckets][256];
-    float ckR[kMaxBuckets][256], ckG[kMaxBuckets][256], ckB[kMaxBuckets][256];
-    float pR[256], pG[256], pB[256], pD[256], pER[256], pEG[256], pEB[256], pObs[256];
-    uint32_t pLast[256];
-    uint32_t maxc;
-};
-
-__global__ __launch_bounds__(256) void k_mapping(
+// ---- A14 ------------------------------------------------------------------------------------
+// One workgroup per tile, one thread per pixel; LDS holds one batch of 256 records (16 KB), so
+// several workgroups share a CU. The (T, colour) checkpoints at list indices 0, 32, ..., 224 go
+// to HBM as float4 per (tile, bucket, pixel) — only the first 256 entries of a tile ever get
+// gradients (GSCudaKernels.cu:2329-2330), so no other bucket needs one.
+__global__ __launch_bounds__(256) void k_forward(
     View v, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ idx,
-    const Rec* __restrict__ rec, const float* __restrict__ obs_rgb,
-    const float* __restrict__ obs_depth, BlendParams bp, float* __restrict__ out_rgb,
-    float* __restrict__ out_depth, float* __restrict__ out_T, uint32_t* __restrict__ out_nc,
-    float* __restrict__ out_err, uint32_t* __restrict__ max_contrib,
-    Delta2D* __restrict__ delta, int do_backward) {
-    __shared__ MapLds L;
+    const Rec* __restrict__ rec, const float* __restrict__ obs_rgb, BlendParams bp,
+    float* __restrict__ out_rgb, float* __restrict__ out_depth, float* __restrict__ out_T,
+    uint32_t* __restrict__ out_nc, float* __restrict__ out_err, uint32_t* __restrict__ max_contrib,
+    float4* __restrict__ ckpt) {
+    __shared__ Rec recs[256];
+    __shared__ uint32_t smax;
     const int tile = blockIdx.x, tid = threadIdx.x;
-    const int tx0 = (tile % v.ntx) * kTile, ty0 = (tile / v.ntx) * kTile;
-    const int x = tx0 + (tid & 15), y = ty0 + (tid >> 4);
+    const int x = (tile % v.ntx) * kTile + (tid & 15), y = (tile / v.ntx) * kTile + (tid >> 4);
     const bool inside = x < v.W && y < v.H;
     const uint32_t s = ranges[2 * tile], n = ranges[2 * tile + 1] - s;
     const float px = (float)x, py = (float)y;
-    if (tid == 0) L.maxc = 0;
+    if (tid == 0) smax = 0;
 
-    // ---------------- forward (GSCudaKernels.cu:2150-2232) ----------------
     float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f, D = 0.f;
-    uint32_t contributor = 0, last = 0;
+    uint32_t last = 0;
     bool done = !inside;
+    // One list entry (GSCudaKernels.cu:2176-2228); returns true once T < 1e-4.
+    auto step = [&](int i, uint32_t j) -> bool {
+        const float4 r0 = recs[i].r0, r1 = recs[i].r1;
+        const float dx = px - r0.x, dy = py - r0.y;
+        const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;   // Q14
+        const float4 r2 = recs[i].r2;
+        const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));
+        if (a < kAlphaMin || q <= 0.0f) return false;
+        const float ta = T * a;
+        cr += ta * r2.x; cg += ta * r2.y; cb += ta * r2.z;
+        const float tT = T * (1.0f - a);
+        if (T > 0.5f && tT <= 0.5f) D = r1.y + r1.z * dx + r1.w * dy;
+        T = tT;
+        last = j + 1;
+        return tT < 0.0001f;
+    };
     for (uint32_t base = 0; base < n; base += 256) {
         if (__syncthreads_count(!done) == 0) break;
-        if (base + tid < n) L.recs[tid] = rec[idx[s + base + tid]];
+        if (base + tid < n) recs[tid] = rec[idx[s + base + tid]];
         __syncthreads();
         const int cnt = (int)min(256u, n - base);
-        if (!done) {
-            for (int i = 0; i < cnt; ++i) {
-                const uint32_t j = base + i;
-                if ((j & 31u) == 0u && j < 256u) {
-                    const int b = (int)(j >> 5);
-                    L.ckT[b][tid] = T; L.ckR[b][tid] = cr; L.ckG[b][tid] = cg; L.ckB[b][tid] = cb;
-                }
-                ++contributor;
-                const float4 r0 = L.recs[i].r0, r1 = L.recs[i].r1;
-                const float dx = px - r0.x, dy = py - r0.y;
-                // Q14: conic slots read as (xx, xy, yy).
-                const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;
-                const float4 r2 = L.recs[i].r2;
-                const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));
-                if (a < kAlphaMin || q <= 0.0f) continue;
-                const float ta = T * a;
-                cr += ta * r2.x; cg += ta * r2.y; cb += ta * r2.z;
-                const float tT = T * (1.0f - a);
-                if (T > 0.5f && tT <= 0.5f) D = r1.y + r1.z * dx + r1.w * dy;
-                T = tT;
-                last = contributor;
-                if (tT < 0.0001f) { done = true; break; }
+        if (done) continue;
+        if (base == 0) {
+            for (int b = 0; b * kBucket < cnt && !done; ++b) {
+                ckpt[((size_t)tile * kMaxBuckets + b) * 256 + tid] = make_float4(T, cr, cg, cb);
+                const int i1 = min(cnt, (b + 1) * kBucket);
+                for (int i = b * kBucket; i < i1; ++i)
+                    if (step(i, (uint32_t)i)) { done = true; break; }
             }
+        } else {
+            for (int i = 0; i < cnt; ++i)
+                if (step(i, base + i)) { done = true; break; }
         }
     }
-    const float fr = cr + T * bp.bg[0], fg = cg + T * bp.bg[1], fb = cb + T * bp.bg[2];
-    float er = 0.f, eg = 0.f, eb = 0.f, od = 0.f;
     if (inside) {
         const size_t p = (size_t)y * v.W + x;
-        er = fr - obs_rgb[3 * p]; eg = fg - obs_rgb[3 * p + 1]; eb = fb - obs_rgb[3 * p + 2];
-        od = obs_depth[p];
+        const float fr = cr + T * bp.bg[0], fg = cg + T * bp.bg[1], fb = cb + T * bp.bg[2];
         out_T[p] = T;
         out_nc[p] = last;
         out_rgb[3 * p] = fr; out_rgb[3 * p + 1] = fg; out_rgb[3 * p + 2] = fb;
         out_depth[p] = D;
-        out_err[3 * p] = er; out_err[3 * p + 1] = eg; out_err[3 * p + 2] = eb;
+        out_err[3 * p] = fr - obs_rgb[3 * p];
+        out_err[3 * p + 1] = fg - obs_rgb[3 * p + 1];
+        out_err[3 * p + 2] = fb - obs_rgb[3 * p + 2];
     }
     // Block max of n_contrib (cub::BlockReduce Max in the reference).
     uint32_t m = last;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
     __syncthreads();
-    if ((tid & 63) == 0) atomicMax(&L.maxc, m);
-    L.pR[tid] = fr; L.pG[tid] = fg; L.pB[tid] = fb; L.pD[tid] = D;
-    L.pER[tid] = er; L.pEG[tid] = eg; L.pEB[tid] = eb; L.pObs[tid] = od;
-    L.pLast[tid] = inside ? last : 0u;
+    if ((tid & 63) == 0) atomicMax(&smax, m);
     __syncthreads();
-    const uint32_t maxc = L.maxc;
-    if (tid == 0) max_contrib[tile] = maxc;
-    if (!do_backward || maxc == 0) return;
+    if (tid == 0) max_contrib[tile] = smax;
+}
 
-    // ---------------- backward (GSCudaKernels.cu:2304-2501) ----------------
+// ---- A15 ------------------------------------------------------------------------------------
+// One workgroup per tile; each 64-lane wave replays two 32-entry buckets (lanes 0-31 and 32-63):
+// lane l owns entry b*32 + (l & 31), and at step i handles pixel i - (l & 31) with the
+// transmittance and accumulated colour received from lane l-1 (DPP wave shift), lanes 0 and 32
+// starting each pixel from the forward checkpoint of their bucket — the reference's warp
+// pipeline widened to wave64. Per-pixel forward results sit in LDS; one atomicAdd per field and
+// (entry, tile).
+struct BwdLds {
+    Rec recs[256];
+    float pR[256], pG[256], pB[256], pD[256], pER[256], pEG[256], pEB[256], pObs[256];
+    uint32_t pLast[256];
+};
+
+__global__ __launch_bounds__(256) void k_backward(
+    View v, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ idx,
+    const Rec* __restrict__ rec, const float* __restrict__ obs_depth, BlendParams bp,
+    const float* __restrict__ o_rgb, const float* __restrict__ o_depth,
+    const uint32_t* __restrict__ o_nc, const float* __restrict__ o_err,
+    const uint32_t* __restrict__ max_contrib, const float4* __restrict__ ckpt,
+    Delta2D* __restrict__ delta) {
+    __shared__ BwdLds L;
+    const int tile = blockIdx.x, tid = threadIdx.x;
+    const uint32_t maxc = max_contrib[tile];
+    if (maxc == 0) return;
+    const int tx0 = (tile % v.ntx) * kTile, ty0 = (tile / v.ntx) * kTile;
+    const uint32_t s = ranges[2 * tile], n = ranges[2 * tile + 1] - s;
     const uint32_t nsplat = min(n, 256u);
-    if (n > 256u) {  // batch 0 was overwritten by later batches
-        if (tid < (int)nsplat) L.recs[tid] = rec[idx[s + tid]];
-        __syncthreads();
+    if (tid < (int)nsplat) L.recs[tid] = rec[idx[s + tid]];
+    {
+        const int x = tx0 + (tid & 15), y = ty0 + (tid >> 4);
+        if (x < v.W && y < v.H) {
+            const size_t p = (size_t)y * v.W + x;
+            L.pR[tid] = o_rgb[3 * p]; L.pG[tid] = o_rgb[3 * p + 1]; L.pB[tid] = o_rgb[3 * p + 2];
+            L.pER[tid] = o_err[3 * p]; L.pEG[tid] = o_err[3 * p + 1]; L.pEB[tid] = o_err[3 * p + 2];
+            L.pD[tid] = o_depth[p];
+            L.pObs[tid] = obs_depth[p];
+            L.pLast[tid] = o_nc[p];
+        } else {
+            L.pLast[tid] = 0;
+        }
     }
+    __syncthreads();
     const int w = tid >> 6, lane = tid & 63, sl = lane & 31;
     const int b = 2 * w + (lane >> 5);
-    if ((uint32_t)(2 * w * kBucket) >= maxc) return;  // both buckets of this wave are idle
+    if ((uint32_t)(2 * w * kBucket) >= maxc) return;   // both buckets of this wave are idle
     const uint32_t sidx = (uint32_t)(b * kBucket + sl);
     const bool valid = (uint32_t)(b * kBucket) < maxc && sidx < nsplat;
     float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
     if (valid) { r0 = L.recs[sidx].r0; r1 = L.recs[sidx].r1; r2 = L.recs[sidx].r2; }
+    const float4* ck = ckpt + ((size_t)tile * kMaxBuckets + b) * 256;
     float dmx = 0.f, dmy = 0.f, dsx = 0.f, dsy = 0.f, dsz = 0.f, dcr = 0.f, dcg = 0.f, dcb = 0.f,
           ddep = 0.f, dal = 0.f, dphx = 0.f, dphy = 0.f;
     uint32_t dn = 0;
@@ -194,7 +217,10 @@ __global__ __launch_bounds__(256) void k_mapping(
         const int p = i - sl;
         const bool pix_ok = p >= 0 && p < 256 && (p & 15) < wlim && (p >> 4) < hlim;
         if (!(valid && pix_ok)) continue;
-        if (sl == 0) { bT = L.ckT[b][p]; ar = L.ckR[b][p]; ag = L.ckG[b][p]; ab = L.ckB[b][p]; }
+        if (sl == 0) {
+            const float4 c = ck[p];
+            bT = c.x; ar = c.y; ag = c.z; ab = c.w;
+        }
         if (sidx >= L.pLast[p]) continue;
         const float dx = r0.x - (float)(tx0 + (p & 15));   // Q9: mean - pixel
         const float dy = r0.y - (float)(ty0 + (p >> 4));
@@ -485,14 +511,22 @@ hipError_t launch_render(const View& v, const uint32_t* ranges, const uint32_t* 
     return hipGetLastError();
 }
 
-hipError_t launch_mapping(const View& v, const uint32_t* ranges, const uint32_t* idx,
-                          const Rec* rec, const float* obs_rgb, const float* obs_depth,
-                          const BlendParams& bp, float* rgb, float* depth, float* final_T,
-                          uint32_t* n_contrib, float* color_err, uint32_t* max_contrib,
-                          Delta2D* delta, int do_backward, hipStream_t s) {
-    hipLaunchKernelGGL(k_mapping, dim3(v.ntx * v.nty), dim3(256), 0, s, v, ranges, idx, rec,
-                       obs_rgb, obs_depth, bp, rgb, depth, final_T, n_contrib, color_err,
-                       max_contrib, delta, do_backward);
+hipError_t launch_forward(const View& v, const uint32_t* ranges, const uint32_t* idx,
+                          const Rec* rec, const float* obs_rgb, const BlendParams& bp, float* rgb,
+                          float* depth, float* final_T, uint32_t* n_contrib, float* color_err,
+                          uint32_t* max_contrib, float4* ckpt, hipStream_t s) {
+    hipLaunchKernelGGL(k_forward, dim3(v.ntx * v.nty), dim3(256), 0, s, v, ranges, idx, rec,
+                       obs_rgb, bp, rgb, depth, final_T, n_contrib, color_err, max_contrib, ckpt);
+    return hipGetLastError();
+}
+
+hipError_t launch_backward(const View& v, const uint32_t* ranges, const uint32_t* idx,
+                           const Rec* rec, const float* obs_depth, const BlendParams& bp,
+                           const float* rgb, const float* depth, const uint32_t* n_contrib,
+                           const float* color_err, const uint32_t* max_contrib,
+                           const float4* ckpt, Delta2D* delta, hipStream_t s) {
+    hipLaunchKernelGGL(k_backward, dim3(v.ntx * v.nty), dim3(256), 0, s, v, ranges, idx, rec,
+                       obs_depth, bp, rgb, depth, n_contrib, color_err, max_contrib, ckpt, delta);
     return hipGetLastError();
 }
 

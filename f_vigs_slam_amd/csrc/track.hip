@@ -192,7 +192,8 @@ __global__ __launch_bounds__(256) void k_pose(View v, const uint32_t* __restrict
             const float dx = r0.x - px, dy = r0.y - py;
             const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;
             const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));   // Q8: no q <= 0 test
-            if (a < kAlphaMin) continue;
+            // Q15: a culled Gaussian (zero record, depth 0) would divide by zero depth below.
+            if (a < kAlphaMin || r1.y == 0.0f) continue;
             const float d = r1.y + dx * r1.z + dy * r1.w;
             float da0 = r2.x * prod, da1 = r2.y * prod, da2 = r2.z * prod;
             a0 += a * da0; a1 += a * da1; a2 += a * da2;

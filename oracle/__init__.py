@@ -163,6 +163,43 @@ def backward(width, height, binning, rec, colors, opac, fwd, obs_depth, w_depth=
     return delta
 
 
+def forward_f32(width, height, binning, rec32, colors, opac, obs_rgb, bg=(0.0, 0.0, 0.0)):
+    """A14 evaluated in FP32 (same operation order) over FP32 records (project_f32's output):
+    the size of FP32 rounding effects on a view, for comparison with the FP64 forward."""
+    p = width * height
+    ntx, nty = tiles(width, height)
+    nt = ntx * nty
+    f = np.float32
+    out = dict(rgb=np.zeros((p, 3), f), depth=np.zeros(p, f), final_T=np.zeros(p, f),
+               n_contrib=np.zeros(p, np.uint32), color_error=np.zeros((p, 3), f),
+               max_contrib=np.zeros(nt, np.uint32),
+               ck_T=np.zeros(nt * 8 * BLOCK, f), ck_C=np.zeros(nt * 8 * BLOCK * 3, f))
+    r = {k: _f32(rec32[k]) for k in ("pos2d", "conic", "depth", "phat")}
+    lib().orc_forward_f32(ctypes.c_int(width), ctypes.c_int(height), _p(binning["ranges"]),
+                          _p(binning["idx"]), _p(r["pos2d"]), _p(r["conic"]), _p(r["depth"]),
+                          _p(r["phat"]), _p(_f32(colors)), _p(_f32(opac)), _p(_f32(obs_rgb)),
+                          _p(_f32(bg)), _p(out["rgb"]), _p(out["depth"]), _p(out["final_T"]),
+                          _p(out["n_contrib"]), _p(out["color_error"]), _p(out["max_contrib"]),
+                          _p(out["ck_T"]), _p(out["ck_C"]))
+    return out
+
+
+def backward_f32(width, height, binning, rec32, colors, opac, fwd32, obs_depth, w_depth=1.0,
+                 w_dist=0.1):
+    """A15 evaluated in FP32 over forward_f32's outputs. Returns (N, 13) float32 deltas."""
+    n = len(opac)
+    delta = np.zeros((n, 13), np.float32)
+    r = {k: _f32(rec32[k]) for k in ("pos2d", "conic", "depth", "phat")}
+    lib().orc_backward_f32(ctypes.c_int(width), ctypes.c_int(height), ctypes.c_int(n),
+                           _p(binning["ranges"]), _p(binning["idx"]), _p(r["pos2d"]),
+                           _p(r["conic"]), _p(r["depth"]), _p(r["phat"]), _p(_f32(colors)),
+                           _p(_f32(opac)), _p(fwd32["rgb"]), _p(fwd32["depth"]),
+                           _p(fwd32["n_contrib"]), _p(fwd32["color_error"]), _p(_f32(obs_depth)),
+                           _p(fwd32["max_contrib"]), _p(fwd32["ck_T"]), _p(fwd32["ck_C"]),
+                           ctypes.c_float(w_depth), ctypes.c_float(w_dist), _p(delta))
+    return delta
+
+
 def delta3d(pos, scale, quat, cam_pos, cam_quat, intr, delta2d, lambda_iso=0.01):
     """A16 (GSCudaKernels.cu:2504-2720). Returns (N, 16): position(3) scale(3) orientation(3)
     color(3) alpha n pad pad."""

@@ -24,6 +24,9 @@
 //   Q14 the projection stores the conic as (inv_xx, inv_yy, inv_xy) but the mapping forward,
 //       backward and pose kernels read it as (xx, xy, yy): v = c0*dx*dx + 2*c1*dx*dy + c2*dy*dy
 //       (GSCudaKernels.cu:448 vs 1003, 2188, 2414). Transcribed.
+//   Q15 pose pass 2 skips culled Gaussians (zero record): with Q1's zero depth the reference's
+//       1/d (GSCudaKernels.cu:1236-1240) is infinite and its J^T J turns NaN on a first call;
+//       with a stale record (later calls) the term is an artefact. Neither is a contribution.
 //
 // Parity status of each function is recorded in the oracle section of the design notes; the
 // functions pinned only by transcription say so at their definition.
@@ -409,12 +412,13 @@ ORC_API void orc_render(int width, int height, const uint32_t* ranges, const uin
 // Outputs per pixel: color (incl. T*bg), depth, final_T, n_contrib, color_error; per tile
 // max_contrib. Checkpoints (T, colour) at list index b*32 for b < 8 are written to ck_T[T*8*256]
 // and ck_C[3*T*8*256] when those pointers are non-null (consumed by orc_backward).
-ORC_API void orc_forward(int width, int height, const uint32_t* ranges, const uint32_t* idx,
-                         const double* pos2d, const double* conic, const double* depth,
-                         const double* phat, const float* colors, const float* opac,
-                         const float* obs_rgb, const float* bg, double* out_rgb, double* out_depth,
-                         double* final_T, uint32_t* n_contrib, double* color_err,
-                         uint32_t* max_contrib, double* ck_T, double* ck_C) {
+template <typename V>
+static void orc_forward_t(int width, int height, const uint32_t* ranges, const uint32_t* idx,
+                         const V* pos2d, const V* conic, const V* depth,
+                         const V* phat, const float* colors, const float* opac,
+                         const float* obs_rgb, const float* bg, V* out_rgb, V* out_depth,
+                         V* final_T, uint32_t* n_contrib, V* color_err,
+                         uint32_t* max_contrib, V* ck_T, V* ck_C) {
     int ntx = (width + kTile - 1) / kTile, nty = (height + kTile - 1) / kTile;
     const int nb = kBlock / kBucket;
 #pragma omp parallel for schedule(dynamic, 4)
@@ -426,7 +430,7 @@ ORC_API void orc_forward(int width, int height, const uint32_t* ranges, const ui
         for (int tid = 0; tid < kBlock; tid++) {
             int x = tx * kTile + tid % kTile, y = ty * kTile + tid / kTile;
             bool inside = x < width && y < height;
-            double T = 1.0, C[3] = {0, 0, 0}, D = 0.0;
+            V T = 1.0, C[3] = {0, 0, 0}, D = 0.0;
             uint32_t contributor = 0, last = 0;
             bool done = !inside;
             for (uint32_t j = 0; j < n && !done; j++) {
@@ -436,15 +440,15 @@ ORC_API void orc_forward(int width, int height, const uint32_t* ranges, const ui
                 }
                 contributor++;
                 uint32_t g = idx[s + j];
-                double dx = x - pos2d[2 * g], dy = y - pos2d[2 * g + 1];
-                const double* c = conic + 3 * g;
-                double v = c[0] * dx * dx + 2.0 * c[1] * dx * dy + c[2] * dy * dy;   // Q14
-                double a = std::min(kAlphaMax, opac[g] * std::exp(-0.5 * v));
+                V dx = x - pos2d[2 * g], dy = y - pos2d[2 * g + 1];
+                const V* c = conic + 3 * g;
+                V v = c[0] * dx * dx + (V)2 * c[1] * dx * dy + c[2] * dy * dy;   // Q14
+                V a = std::min((V)kAlphaMax, opac[g] * std::exp((V)-0.5 * v));
                 if (a < kAlphaMin || v <= 0.0) continue;
                 C[0] += T * a * colors[3 * g];
                 C[1] += T * a * colors[3 * g + 1];
                 C[2] += T * a * colors[3 * g + 2];
-                double tT = T * (1.0 - a);
+                V tT = T * ((V)1 - a);
                 if (T > 0.5 && tT <= 0.5) D = depth[g] + phat[2 * g] * dx + phat[2 * g + 1] * dy;
                 T = tT;
                 last = contributor;
@@ -455,9 +459,9 @@ ORC_API void orc_forward(int width, int height, const uint32_t* ranges, const ui
                 final_T[p] = T;
                 n_contrib[p] = last;
                 for (int k = 0; k < 3; k++) {
-                    double ck = C[k] + T * (double)bg[k];
+                    V ck = C[k] + T * (V)bg[k];
                     out_rgb[3 * p + k] = ck;
-                    color_err[3 * p + k] = ck - (double)obs_rgb[3 * p + k];
+                    color_err[3 * p + k] = ck - (V)obs_rgb[3 * p + k];
                 }
                 out_depth[p] = D;
             }
@@ -467,22 +471,47 @@ ORC_API void orc_forward(int width, int height, const uint32_t* ranges, const ui
     }
 }
 
+ORC_API void orc_forward(int width, int height, const uint32_t* ranges, const uint32_t* idx,
+                         const double* pos2d, const double* conic, const double* depth,
+                         const double* phat, const float* colors, const float* opac,
+                         const float* obs_rgb, const float* bg, double* out_rgb, double* out_depth,
+                         double* final_T, uint32_t* n_contrib, double* color_err,
+                         uint32_t* max_contrib, double* ck_T, double* ck_C) {
+    orc_forward_t<double>(width, height, ranges, idx, pos2d, conic, depth, phat, colors, opac,
+                          obs_rgb, bg, out_rgb, out_depth, final_T, n_contrib, color_err,
+                          max_contrib, ck_T, ck_C);
+}
+
+// A14 with FP32 arithmetic and storage (same operation order): measures how far an FP32
+// evaluation of the blend sits from the FP64 one on a given view.
+ORC_API void orc_forward_f32(int width, int height, const uint32_t* ranges, const uint32_t* idx,
+                             const float* pos2d, const float* conic, const float* depth,
+                             const float* phat, const float* colors, const float* opac,
+                             const float* obs_rgb, const float* bg, float* out_rgb,
+                             float* out_depth, float* final_T, uint32_t* n_contrib,
+                             float* color_err, uint32_t* max_contrib, float* ck_T, float* ck_C) {
+    orc_forward_t<float>(width, height, ranges, idx, pos2d, conic, depth, phat, colors, opac,
+                         obs_rgb, bg, out_rgb, out_depth, final_T, n_contrib, color_err,
+                         max_contrib, ck_T, ck_C);
+}
+
 // A15: mapping backward (GSCudaKernels.cu:2265-2502). Per tile, per bucket b (b*32 < max_contrib,
 // b < 8 by Q10), per pixel: replay the 32 splats of the bucket from the forward checkpoint with
 // the backward's rules (no v<=0 test, Q8; dx = mean - pixel, Q9), accumulating the 2D deltas.
 // delta layout per Gaussian (13 doubles): meanImg(2) invSigma(3) color(3) depth alpha pHat(2) n.
-ORC_API void orc_backward(int width, int height, int n_gauss, const uint32_t* ranges,
-                          const uint32_t* idx, const double* pos2d, const double* conic,
-                          const double* depth, const double* phat, const float* colors,
-                          const float* opac, const double* out_rgb, const double* out_depth,
-                          const uint32_t* n_contrib, const double* color_err,
-                          const float* obs_depth, const uint32_t* max_contrib, const double* ck_T,
-                          const double* ck_C, double w_depth, double w_dist, double* delta) {
+template <typename V>
+static void orc_backward_t(int width, int height, int n_gauss, const uint32_t* ranges,
+                          const uint32_t* idx, const V* pos2d, const V* conic,
+                          const V* depth, const V* phat, const float* colors,
+                          const float* opac, const V* out_rgb, const V* out_depth,
+                          const uint32_t* n_contrib, const V* color_err,
+                          const float* obs_depth, const uint32_t* max_contrib, const V* ck_T,
+                          const V* ck_C, V w_depth, V w_dist, V* delta) {
     int ntx = (width + kTile - 1) / kTile, nty = (height + kTile - 1) / kTile;
     int ntiles = ntx * nty;
     const int nb = kBlock / kBucket;
     // Per-(tile, slot) partials, reduced afterwards in tile order: deterministic.
-    std::vector<double> part((size_t)ntiles * kBlock * 13, 0.0);
+    std::vector<V> part((size_t)ntiles * kBlock * 13, 0.0);
 #pragma omp parallel for schedule(dynamic, 2)
     for (int t = 0; t < ntiles; t++) {
         int tx = t % ntx, ty = t / ntx;
@@ -496,55 +525,55 @@ ORC_API void orc_backward(int width, int height, int n_gauss, const uint32_t* ra
                 if (x >= width || y >= height) continue;
                 size_t p = (size_t)y * width + x;
                 size_t ci = ((size_t)t * nb + b) * kBlock + pix;
-                double T = ck_T[ci];
-                double acc[3] = {ck_C[3 * ci], ck_C[3 * ci + 1], ck_C[3 * ci + 2]};
-                const double* C = out_rgb + 3 * p;
-                const double* ce = color_err + 3 * p;
-                double D = out_depth[p];
-                double img_depth = obs_depth[p];
+                V T = ck_T[ci];
+                V acc[3] = {ck_C[3 * ci], ck_C[3 * ci + 1], ck_C[3 * ci + 2]};
+                const V* C = out_rgb + 3 * p;
+                const V* ce = color_err + 3 * p;
+                V D = out_depth[p];
+                V img_depth = obs_depth[p];
                 uint32_t last = n_contrib[p];
                 for (int l = 0; l < kBucket; l++) {
                     uint32_t sp = (uint32_t)(b * kBucket + l);
                     if (sp >= nsplat) continue;
                     if (sp >= last) continue;
                     uint32_t g = idx[s + sp];
-                    double dx = pos2d[2 * g] - x, dy = pos2d[2 * g + 1] - y;
-                    const double* c = conic + 3 * g;
-                    double v = c[0] * dx * dx + 2.0 * c[1] * dx * dy + c[2] * dy * dy;  // Q14
-                    double a = std::min(kAlphaMax, opac[g] * std::exp(-0.5 * v));
+                    V dx = pos2d[2 * g] - x, dy = pos2d[2 * g + 1] - y;
+                    const V* c = conic + 3 * g;
+                    V v = c[0] * dx * dx + (V)2 * c[1] * dx * dy + c[2] * dy * dy;  // Q14
+                    V a = std::min((V)kAlphaMax, opac[g] * std::exp((V)-0.5 * v));
                     if (a < kAlphaMin) continue;
-                    double col[3] = {colors[3 * g], colors[3 * g + 1], colors[3 * g + 2]};
-                    double da[3], dl = 0.0;
+                    V col[3] = {colors[3 * g], colors[3 * g + 1], colors[3 * g + 2]};
+                    V da[3], dl = 0.0;
                     for (int k = 0; k < 3; k++) {
                         da[k] = col[k] * T;
                         acc[k] += da[k] * a;
                     }
-                    double iom = 1.0 / (1.0 - a);
+                    V iom = (V)1 / ((V)1 - a);
                     for (int k = 0; k < 3; k++) {
                         da[k] -= (C[k] - acc[k]) * iom;
                         da[k] *= -ce[k];
                         dl += da[k];
                     }
-                    double aG = a * dl;
-                    double* d = part.data() + ((size_t)t * kBlock + sp) * 13;
+                    V aG = a * dl;
+                    V* d = part.data() + ((size_t)t * kBlock + sp) * 13;
                     d[12] += 1.0;
                     d[5] -= a * T * ce[0]; d[6] -= a * T * ce[1]; d[7] -= a * T * ce[2];
                     d[9] -= aG;
                     d[0] -= aG * (c[0] * dx + c[1] * dy);
                     d[1] -= aG * (c[1] * dx + c[2] * dy);
-                    d[2] -= 0.5 * aG * dx * dx;
-                    d[3] -= 0.5 * aG * dx * dy;
-                    d[4] -= 0.5 * aG * dy * dy;
-                    double tT = T * (1.0 - a);
-                    double phx = phat[2 * g], phy = phat[2 * g + 1];
+                    d[2] -= (V)0.5 * aG * dx * dx;
+                    d[3] -= (V)0.5 * aG * dx * dy;
+                    d[4] -= (V)0.5 * aG * dy * dy;
+                    V tT = T * ((V)1 - a);
+                    V phx = phat[2 * g], phy = phat[2 * g + 1];
                     if (T > 0.5 && tT <= 0.5) {
-                        double de = (img_depth > kDepthValid) ? (D - img_depth) : 0.0;
+                        V de = (img_depth > kDepthValid) ? (D - img_depth) : 0.0;
                         d[8] -= w_depth * de;
                         d[0] -= w_depth * de * phx; d[1] -= w_depth * de * phy;
                         d[10] -= w_depth * de * dx; d[11] -= w_depth * de * dy;
                     }
-                    double di = depth[g] + dx * phx + dy * phy;
-                    double dc = w_dist * a * T * (di - D);
+                    V di = depth[g] + dx * phx + dy * phy;
+                    V dc = w_dist * a * T * (di - D);
                     d[8] -= dc;
                     d[0] -= dc * phx; d[1] -= dc * phy;
                     d[10] -= dc * dx; d[11] -= dc * dy;
@@ -553,17 +582,43 @@ ORC_API void orc_backward(int width, int height, int n_gauss, const uint32_t* ra
             }
         }
     }
-    std::memset(delta, 0, sizeof(double) * 13 * (size_t)n_gauss);
+    std::memset(delta, 0, sizeof(V) * 13 * (size_t)n_gauss);
     for (int t = 0; t < ntiles; t++) {
         uint32_t s = ranges[2 * t], e = ranges[2 * t + 1];
         uint32_t nsplat = std::min<uint32_t>(e - s, kBlock);
         for (uint32_t sp = 0; sp < nsplat; sp++) {
-            const double* d = part.data() + ((size_t)t * kBlock + sp) * 13;
+            const V* d = part.data() + ((size_t)t * kBlock + sp) * 13;
             if (d[12] == 0.0) continue;
-            double* o = delta + 13 * (size_t)idx[s + sp];
+            V* o = delta + 13 * (size_t)idx[s + sp];
             for (int k = 0; k < 13; k++) o[k] += d[k];
         }
     }
+}
+
+ORC_API void orc_backward(int width, int height, int n_gauss, const uint32_t* ranges,
+                          const uint32_t* idx, const double* pos2d, const double* conic,
+                          const double* depth, const double* phat, const float* colors,
+                          const float* opac, const double* out_rgb, const double* out_depth,
+                          const uint32_t* n_contrib, const double* color_err,
+                          const float* obs_depth, const uint32_t* max_contrib, const double* ck_T,
+                          const double* ck_C, double w_depth, double w_dist, double* delta) {
+    orc_backward_t<double>(width, height, n_gauss, ranges, idx, pos2d, conic, depth, phat, colors,
+                           opac, out_rgb, out_depth, n_contrib, color_err, obs_depth, max_contrib,
+                           ck_T, ck_C, w_depth, w_dist, delta);
+}
+
+// A15 with FP32 arithmetic and storage (same operation order), companion of orc_forward_f32.
+ORC_API void orc_backward_f32(int width, int height, int n_gauss, const uint32_t* ranges,
+                              const uint32_t* idx, const float* pos2d, const float* conic,
+                              const float* depth, const float* phat, const float* colors,
+                              const float* opac, const float* out_rgb, const float* out_depth,
+                              const uint32_t* n_contrib, const float* color_err,
+                              const float* obs_depth, const uint32_t* max_contrib,
+                              const float* ck_T, const float* ck_C, float w_depth, float w_dist,
+                              float* delta) {
+    orc_backward_t<float>(width, height, n_gauss, ranges, idx, pos2d, conic, depth, phat, colors,
+                          opac, out_rgb, out_depth, n_contrib, color_err, obs_depth, max_contrib,
+                          ck_T, ck_C, w_depth, w_dist, delta);
 }
 
 // A16: 2D deltas -> 3D deltas (GSCudaKernels.cu:2504-2720). Output per Gaussian (16 doubles):
@@ -863,7 +918,7 @@ ORC_API void orc_pose_jacobians(int width, int height, const uint32_t* ranges, c
                 const double* c = conic + 3 * g;
                 double v = c[0] * dx * dx + 2.0 * c[1] * dx * dy + c[2] * dy * dy;  // Q14
                 double a = std::min(kAlphaMax, opac[g] * std::exp(-0.5 * v));
-                if (a < kAlphaMin) continue;
+                if (a < kAlphaMin || depth[g] == 0.0) continue;   // Q15: culled record
                 double d = depth[g] + dx * phat[2 * g] + dy * phat[2 * g + 1];
                 double da[3] = {colors[3 * g] * prod, colors[3 * g + 1] * prod, colors[3 * g + 2] * prod};
                 for (int k = 0; k < 3; k++) acc[k] += a * da[k];

@@ -63,3 +63,10 @@ def grad_report(got, ref):
     e = np.abs(got - ref).max(axis=-1) / scale if got.ndim > 1 else np.abs(got - ref) / scale
     return dict(maxnorm=float(e.max()), q50=float(np.quantile(e, 0.5)),
                 q99=float(np.quantile(e, 0.99)), q999=float(np.quantile(e, 0.999)))
+
+
+def rel_l2(got, ref):
+    """||got - ref||_2 / ||ref||_2 over the whole field."""
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300))

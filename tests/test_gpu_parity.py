@@ -5,8 +5,14 @@ Bars (stated per output):
   * images: |gpu - oracle| <= 1e-4 * (1 + |oracle|) on >= 99.9 % of pixels (the rest are
     discrete flips of FP32-vs-FP64 threshold tests: alpha >= 1/255, T < 1e-4, the T = 0.5 depth
     crossing);
-  * 2D / 3D gradients: |gpu - oracle| <= 1e-3 * (|oracle| + 1e-3 max|oracle|) on >= 99 % of
-    Gaussians per field (float atomics + flips);
+  * 2D / 3D gradients, per field, errors normalised by the field's largest |oracle| entry:
+    99th percentile <= 1e-3, and the 99.9th percentile and the relative L2 error at most twice
+    those of the oracle's own FP32 restatement of the reference's arithmetic (forward_f32 /
+    backward_f32) on the same view, with floors 1e-3. The FP32 restatement itself sits at
+    q999 ~ 6e-3 and max-norm ~ 0.1 on meanImg: the reference's blend has discontinuities
+    (alpha >= 1/255, the v <= 0 test that Q14's indefinite form puts at full opacity, the
+    T = 0.5 depth crossing, the T < 1e-4 stop) that FP32 rounding flips on a few pixels, so an
+    FP32 implementation cannot meet 1e-3 on every Gaussian against FP64;
   * pose JtJ / Jtr: <= 1e-3 relative to the largest entry; Gauss-Newton step <= 1e-6 rad / m.
 """
 import numpy as np
@@ -15,8 +21,8 @@ import pytest
 import oracle as O
 from f_vigs_slam_amd import Context, FvgsError, gn_step
 from f_vigs_slam_amd import scenes as S
-from tests.parity_util import (cam, close_fraction, grad_agreement, grad_report,
-                               mapping_observations, oracle_view, workload)
+from tests.parity_util import (cam, close_fraction, grad_report, mapping_observations,
+                               oracle_view, rel_l2, workload)
 
 pytestmark = pytest.mark.gpu
 
@@ -103,8 +109,26 @@ def mapping_case():
     d2 = O.backward(W, H, b, rec, scene["color"], scene["opacity"], fwd, obs_depth)
     cp, cq = cam(pose)
     d3 = O.delta3d(scene["pos"], scene["scale"], scene["quat"], cp, cq, intr, d2)
+    # FP32 restatement over the bit-identical FP32 records: the reference-precision floor.
+    f32 = O.forward_f32(W, H, b, pf, scene["color"], scene["opacity"], obs_rgb)
+    g32 = O.backward_f32(W, H, b, pf, scene["color"], scene["opacity"], f32, obs_depth)
+    d3_32 = O.delta3d(scene["pos"], scene["scale"], scene["quat"], cp, cq, intr, g32)
     return dict(scene=scene, intr=intr, pose=pose, obs_rgb=obs_rgb, obs_depth=obs_depth,
-                fwd=fwd, d2=d2, d3=d3, b=b)
+                fwd=fwd, d2=d2, d3=d3, b=b, g32=g32, d3_32=d3_32)
+
+
+def _check_gradient_field(tag, name, got, ref, ref32):
+    rep = grad_report(got, ref)
+    rep32 = grad_report(ref32, ref)
+    l2 = rel_l2(got, ref)
+    l2_32 = rel_l2(ref32, ref)
+    print(f"[{tag}] {name}: gpu q50 {rep['q50']:.2e} q99 {rep['q99']:.2e} q999 {rep['q999']:.2e} "
+          f"max {rep['maxnorm']:.2e} relL2 {l2:.2e} | fp32 restatement q99 {rep32['q99']:.2e} "
+          f"q999 {rep32['q999']:.2e} max {rep32['maxnorm']:.2e} relL2 {l2_32:.2e} | "
+          f"gpu vs fp32 q50 {grad_report(got, ref32)['q50']:.2e}", flush=True)
+    assert rep["q99"] <= 1e-3
+    assert rep["q999"] <= max(1e-3, 2.0 * rep32["q999"])
+    assert l2 <= max(1e-3, 2.0 * l2_32)
 
 
 def test_forward_parity(ctx, mapping_case):
@@ -140,11 +164,8 @@ def test_backward_parity(ctx, mapping_case):
     cols = dict(meanImg=(0, 2), invSigmaImg=(2, 5), color=(5, 8), depth=(8, 9), alpha=(9, 10),
                 pHat=(10, 12))
     for f, (a, z) in cols.items():
-        gv = g[f].reshape(len(g), -1)
-        frac = grad_agreement(gv, r[:, a:z])
-        rep = grad_report(gv, r[:, a:z])
-        print(f"[backward] {f} per-entry agreement {frac:.6f} {rep}", flush=True)
-        assert rep["q999"] <= 1e-3 and rep["maxnorm"] <= 2e-2
+        _check_gradient_field("backward", f, g[f].reshape(len(g), -1), r[:, a:z],
+                              c["g32"][:, a:z])
 
 
 def test_instance_capacity_growth_reruns_the_view(mapping_case):
@@ -182,10 +203,8 @@ def test_gradients_3d_parity(ctx, mapping_case):
     r = c["d3"]
     for f, (a, z) in dict(position=(0, 3), scale=(3, 6), orientation=(6, 9), color=(9, 12),
                           alpha=(12, 13)).items():
-        frac = grad_agreement(g[f].reshape(len(g), -1), r[:, a:z])
-        rep = grad_report(g[f].reshape(len(g), -1), r[:, a:z])
-        print(f"[delta3d] {f} per-entry agreement {frac:.6f} {rep}", flush=True)
-        assert rep["q999"] <= 1e-3 and rep["maxnorm"] <= 2e-2
+        _check_gradient_field("delta3d", f, g[f].reshape(len(g), -1), r[:, a:z],
+                              c["d3_32"][:, a:z])
 
 
 @pytest.fixture(scope="module")

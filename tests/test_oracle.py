@@ -379,6 +379,58 @@ def test_backward_only_first_256_entries_get_gradients():
     assert (d[256:, 12] == 0).all() and (d[:256, 12] > 0).any()
 
 
+def test_fp32_restatement_follows_the_fp64_blend():
+    """forward_f32 / backward_f32 (the reference's arithmetic precision, used by the GPU tests
+    to size their tolerance) agree with the FP64 forward/backward away from threshold flips."""
+    w, h = 160, 120
+    sc = S.make_scene(6000, seed=4, width=w, height=h)
+    it = S.intrinsics(w, h)
+    pf, b, rec = O.pipeline(sc, np.zeros(3, np.float32), ID_Q, it, w, h)
+    zero = np.zeros((h * w, 3), np.float32)
+    f0 = O.forward(w, h, b, rec, sc["color"], sc["opacity"], zero)
+    up_c, up_d = S.upstream_grads(w, h, seed=2)
+    obs = (f0["rgb"].reshape(h, w, 3) - up_c).astype(np.float32)
+    dep = (f0["depth"].reshape(h, w) - up_d).astype(np.float32)
+    f64 = O.forward(w, h, b, rec, sc["color"], sc["opacity"], obs)
+    f32 = O.forward_f32(w, h, b, pf, sc["color"], sc["opacity"], obs)
+    assert (f32["n_contrib"] == f64["n_contrib"]).mean() > 0.999
+    for k in ("rgb", "final_T", "depth"):
+        err = np.abs(f32[k].reshape(f64[k].shape) - f64[k])
+        assert (err <= 1e-4 * (1 + np.abs(f64[k]))).mean() > 0.999, k
+    d64 = O.backward(w, h, b, rec, sc["color"], sc["opacity"], f64, dep)
+    d32 = O.backward_f32(w, h, b, pf, sc["color"], sc["opacity"], f32, dep)
+    assert d32.dtype == np.float32
+    for a, z in ((0, 2), (2, 5), (5, 8), (8, 9), (9, 10), (10, 12)):
+        scale = np.abs(d64[:, a:z]).max()
+        e = np.abs(d32[:, a:z] - d64[:, a:z]).max(axis=1) / scale
+        assert np.quantile(e, 0.99) <= 1e-3, (a, z)
+
+
+def test_pose_pass2_skips_culled_records():
+    """Q15: culled Gaussians lead tile 0 with the zero record (depth 0); pass 2 leaves them out,
+    so J^T J stays finite when the view culls part of the map (a 2 deg / 3 cm perturbation of a
+    map that fills the frustum culls a few thousand Gaussians; with them, tile 0 divides by 0)."""
+    w, h = 320, 240
+    sc = S.make_scene(12000, seed=21, width=w, height=h)
+    it = S.intrinsics(w, h)
+    gt = np.array([0, 0, 0, 0, 0, 0, 1.0])
+    pf, b, rec = O.pipeline(sc, gt[:3].astype(np.float32), ID_Q, it, w, h)
+    z3 = np.zeros((h, w, 3), np.float32)
+    _, _, rd = O.pose_jacobians(w, h, b, rec, sc["color"], sc["opacity"], z3, z3, z3,
+                                np.ones((h, w), np.float32), ID_Q, it, want_render=True)
+    obs = rd["rgb"].reshape(h, w, 3).astype(np.float32)
+    dep = rd["depth"].reshape(h, w).astype(np.float32)
+    gx, gy = O.sobel3(obs)
+    pose = S.perturbed_pose(21, rot_deg=2.0, trans_m=0.03)
+    cp, cq = pose[:3].astype(np.float32), pose[3:].astype(np.float32)
+    pf, b, rec = O.pipeline(sc, cp, cq, it, w, h)
+    assert b["keys"][0] == 0 and (pf["depth"] == 0).sum() > 100
+    tiles, tot = O.pose_jacobians(w, h, b, rec, sc["color"], sc["opacity"], gx, gy, obs, dep,
+                                  cq, it)
+    assert np.isfinite(tiles).all() and np.isfinite(tot).all()
+    assert np.abs(tiles[0]).max() > 0
+
+
 # ------------------------------------------------------------------------------------------
 # A16: 2D -> 3D, finite differences of the projection chain and closed forms
 # ------------------------------------------------------------------------------------------
