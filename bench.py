@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--breakdown", action="store_true",
                     help="after the timed steps, run the forward-only blend K times (profiling)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run 3 device-to-device copies of this many MiB: the reference "
+                         "kernel that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
 
 
@@ -85,6 +88,17 @@ def cpu_baseline(scene, intr, pose, obs_rgb, obs_depth, w, h, runs):
                 sample=f"the full {len(scene['pos'])}-Gaussian {w}x{h} view (binning FP32 + "
                        f"forward/backward/2D->3D FP64), median of {runs} runs after 1 warm-up, "
                        f"{t:.3f} s per view")
+
+
+def copy_probe(mib):
+    """3 plain device copies of `mib` MiB (reads and writes exactly that many bytes each)."""
+    import torch
+    src = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy probe: 3 x {mib} MiB")
 
 
 def tracking(args, log_fn):
@@ -131,6 +145,8 @@ def main():
     from f_vigs_slam_amd import scenes as S
 
     w, h, n = args.width, args.height, args.n
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
     log(f"scene {n} Gaussians {w}x{h}")
     scene = S.make_scene(n, seed=0, width=w, height=h)
     intr = S.intrinsics(w, h)
