@@ -185,8 +185,9 @@ int ensure_instances(fvgs_ctx* c, uint64_t m) {
 //   async = true:  enqueue everything with the current instance capacity and no host wait; the
 //                  scan empties every range if M exceeds the capacity, and the caller checks
 //                  overflowed() after its own final synchronisation and re-runs synchronously.
+//   cull_emit:     whether culled Gaussians emit their zero-record tile-0 instance.
 int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int w, int h,
-            bool async) {
+            bool async, int cull_emit) {
     if (!pose || !intr) return fail(c, FVGS_E_ARG, "null pose or intrinsics");
     if (w <= 0 || h <= 0) return fail(c, FVGS_E_ARG, "invalid image size %d x %d", w, h);
     if (!finite_pose(pose)) return fail(c, FVGS_E_ARG, "non-finite pose");
@@ -219,7 +220,7 @@ int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int
     c->view = v;
     mark(c, 0);
     HIPCHK(c, launch_project(c->pos, c->scale, c->quat, c->color, c->opac, n, v,
-                             c->prm.cull_emit, as<Rec>(c->rec), as<uint2>(c->rect),
+                             cull_emit, as<Rec>(c->rec), as<uint2>(c->rect),
                              as<uint32_t>(c->blk_counts), s));
     HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->chunk_pre), nblk, T,
                              as<uint32_t>(c->ranges), as<uint32_t>(c->d_m),
@@ -529,7 +530,7 @@ int fvgs_prepare_rasterization(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_in
     if (!c) return FVGS_E_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
     (void)hipSetDevice(c->prm.device);
-    int r = prepare(c, pose, intr, w, h, false);
+    int r = prepare(c, pose, intr, w, h, false, c->prm.cull_emit);
     if (n_instances) *n_instances = (r == FVGS_OK || r == FVGS_E_EMPTY) ? c->m : 0;
     if (r == FVGS_OK && c->timing) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -587,7 +588,7 @@ int fvgs_mapping_step(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics*
     (void)hipSetDevice(c->prm.device);
     if (!c->obs_rgb) return fail(c, FVGS_E_STATE, "no frame set");
     auto attempt = [&](bool async) -> int {
-        TRY(prepare(c, pose, intr, c->fw, c->fh, async));
+        TRY(prepare(c, pose, intr, c->fw, c->fh, async, c->prm.cull_emit));
         TRY(clear_delta2d(c));
         mark(c, 2);
         TRY(run_forward(c));
@@ -613,7 +614,10 @@ int fvgs_pose_jacobians(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsic
     (void)hipSetDevice(c->prm.device);
     if (!c->obs_rgb) return fail(c, FVGS_E_STATE, "no frame set");
     auto attempt = [&](bool async) -> int {
-        TRY(prepare(c, pose, intr, c->fw, c->fh, async));
+        // Culled Gaussians are never emitted here: pass 1 skips their zero records (v <= 0) and
+        // pass 2 skips them (Q15), so the sums are the same without them, and tile 0 does not
+        // carry one list entry per culled Gaussian.
+        TRY(prepare(c, pose, intr, c->fw, c->fh, async, 0));
         TRY(run_pose(c, false));   // synchronises
         return resolve_m(c);
     };
@@ -630,7 +634,7 @@ int fvgs_render_tracking(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsi
     if (!c) return FVGS_E_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
     (void)hipSetDevice(c->prm.device);
-    TRY(prepare(c, pose, intr, w, h, false));
+    TRY(prepare(c, pose, intr, w, h, false, 0));   // culled: skipped by pass 1 anyway
     const int T = c->view.ntx * c->view.nty;
     const size_t P = (size_t)w * h;
     TRY(ensure(c, c->pose_tiles, 27 * 8ull * T));

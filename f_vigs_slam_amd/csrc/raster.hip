@@ -24,6 +24,12 @@ __device__ __forceinline__ float lane_shift_up(float v) {
                                                       0x138 /* wave_shr:1 */, 0xf, 0xf, false));
 }
 
+// Lane l receives lane l+1's value (lane 63 keeps its own).
+__device__ __forceinline__ float lane_shift_down(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                      0x130 /* wave_shl:1 */, 0xf, 0xf, false));
+}
+
 // ---- A12 ------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_render(View v, const uint32_t* __restrict__ ranges,
                                                 const uint32_t* __restrict__ idx,
@@ -77,10 +83,45 @@ __global__ __launch_bounds__(256) void k_render(View v, const uint32_t* __restri
 }
 
 // ---- A14 ------------------------------------------------------------------------------------
-// One workgroup per tile, one thread per pixel; LDS holds one batch of 256 records (16 KB), so
-// several workgroups share a CU. The (T, colour) checkpoints at list indices 0, 32, ..., 224 go
+// Wave-level culling for the blend walks. A wave owns an 8x8 pixel quadrant of the tile; an entry
+// can change a pixel's state only where alpha >= 1/255, i.e. q <= 2 ln(255 opacity), and (in the
+// forward) q > 0. Interval arithmetic bounds q = c0 dx^2 + 2 c1 dx dy + c2 dy^2 over the quadrant's
+// dx, dy ranges; an entry whose bounds exclude every pixel of a quadrant (with a relative slack
+// far above FP32 rounding of either evaluation) is skipped by that wave. Skipped entries leave
+// every pixel of the wave untouched, so results are those of the full walk. NaN bounds keep the
+// entry. Returns bit w set for each quadrant w whose wave must evaluate the entry.
+__device__ __forceinline__ uint32_t quadrant_mask(const Rec& r, int tx0, int ty0, bool need_pos_q) {
+    const float u = r.r0.x, vv = r.r0.y, c0 = r.r0.z, c1x2 = 2.0f * r.r0.w, c2 = r.r1.x;
+    const float qcut = 2.0f * __logf(255.0f * r.r2.w) + 1e-3f;
+    uint32_t mask = 0;
+#pragma unroll 1
+    for (int w = 0; w < 4; ++w) {
+        const float ax0 = (float)(tx0 + 8 * (w & 1)) - u, ax1 = (float)(tx0 + 8 * (w & 1) + 7) - u;
+        const float ay0 = (float)(ty0 + 8 * (w >> 1)) - vv, ay1 = (float)(ty0 + 8 * (w >> 1) + 7) - vv;
+        const float xa = ax0 * ax0, xb = ax1 * ax1, ya = ay0 * ay0, yb = ay1 * ay1;
+        const float xxlo = (ax0 <= 0.0f && ax1 >= 0.0f) ? 0.0f : fminf(xa, xb), xxhi = fmaxf(xa, xb);
+        const float yylo = (ay0 <= 0.0f && ay1 >= 0.0f) ? 0.0f : fminf(ya, yb), yyhi = fmaxf(ya, yb);
+        const float p0 = ax0 * ay0, p1 = ax0 * ay1, p2 = ax1 * ay0, p3 = ax1 * ay1;
+        const float xylo = fminf(fminf(p0, p1), fminf(p2, p3)), xyhi = fmaxf(fmaxf(p0, p1), fmaxf(p2, p3));
+        const float t0lo = c0 >= 0.0f ? c0 * xxlo : c0 * xxhi, t0hi = c0 >= 0.0f ? c0 * xxhi : c0 * xxlo;
+        const float t1lo = c1x2 >= 0.0f ? c1x2 * xylo : c1x2 * xyhi,
+                    t1hi = c1x2 >= 0.0f ? c1x2 * xyhi : c1x2 * xylo;
+        const float t2lo = c2 >= 0.0f ? c2 * yylo : c2 * yyhi, t2hi = c2 >= 0.0f ? c2 * yyhi : c2 * yylo;
+        const float qlo = t0lo + t1lo + t2lo, qhi = t0hi + t1hi + t2hi;
+        const float slack = 1e-5f * (fabsf(t0lo) + fabsf(t0hi) + fabsf(t1lo) + fabsf(t1hi) +
+                                     fabsf(t2lo) + fabsf(t2hi)) + 1e-30f;
+        const bool skip = (need_pos_q && qhi + slack <= 0.0f) || (qlo - slack > qcut);
+        if (!skip) mask |= 1u << w;
+    }
+    return mask;
+}
+
+// One workgroup per tile, one wave per 8x8 quadrant, one thread per pixel. Each batch of 256
+// records is staged in LDS (16 KB) with its quadrant masks; each wave walks the compacted list of
+// the entries its quadrant needs. The (T, colour) checkpoints at list indices 0, 32, ..., 224 go
 // to HBM as float4 per (tile, bucket, pixel) — only the first 256 entries of a tile ever get
-// gradients (GSCudaKernels.cu:2329-2330), so no other bucket needs one.
+// gradients (GSCudaKernels.cu:2329-2330), so no other bucket needs one. A checkpoint is written
+// when the walk reaches or passes its index; skipped entries do not change the state.
 __global__ __launch_bounds__(256) void k_forward(
     View v, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ idx,
     const Rec* __restrict__ rec, const float* __restrict__ obs_rgb, BlendParams bp,
@@ -88,9 +129,13 @@ __global__ __launch_bounds__(256) void k_forward(
     uint32_t* __restrict__ out_nc, float* __restrict__ out_err, uint32_t* __restrict__ max_contrib,
     float4* __restrict__ ckpt) {
     __shared__ Rec recs[256];
+    __shared__ uint8_t qmask[256];
+    __shared__ uint8_t wlist[4][256];
     __shared__ uint32_t smax;
-    const int tile = blockIdx.x, tid = threadIdx.x;
-    const int x = (tile % v.ntx) * kTile + (tid & 15), y = (tile / v.ntx) * kTile + (tid >> 4);
+    const int tile = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int tx0 = (tile % v.ntx) * kTile, ty0 = (tile / v.ntx) * kTile;
+    const int lx = 8 * (w & 1) + (lane & 7), ly = 8 * (w >> 1) + (lane >> 3);
+    const int x = tx0 + lx, y = ty0 + ly, pix = ly * kTile + lx;
     const bool inside = x < v.W && y < v.H;
     const uint32_t s = ranges[2 * tile], n = ranges[2 * tile + 1] - s;
     const float px = (float)x, py = (float)y;
@@ -115,23 +160,37 @@ __global__ __launch_bounds__(256) void k_forward(
         last = j + 1;
         return tT < 0.0001f;
     };
+    float4* ck = ckpt + (size_t)tile * kMaxBuckets * 256 + pix;
     for (uint32_t base = 0; base < n; base += 256) {
         if (__syncthreads_count(!done) == 0) break;
-        if (base + tid < n) recs[tid] = rec[idx[s + base + tid]];
-        __syncthreads();
         const int cnt = (int)min(256u, n - base);
-        if (done) continue;
-        if (base == 0) {
-            for (int b = 0; b * kBucket < cnt && !done; ++b) {
-                ckpt[((size_t)tile * kMaxBuckets + b) * 256 + tid] = make_float4(T, cr, cg, cb);
-                const int i1 = min(cnt, (b + 1) * kBucket);
-                for (int i = b * kBucket; i < i1; ++i)
-                    if (step(i, (uint32_t)i)) { done = true; break; }
-            }
-        } else {
-            for (int i = 0; i < cnt; ++i)
-                if (step(i, base + i)) { done = true; break; }
+        if (tid < cnt) {
+            const Rec r = rec[idx[s + base + tid]];
+            recs[tid] = r;
+            qmask[tid] = (uint8_t)quadrant_mask(r, tx0, ty0, true);
         }
+        __syncthreads();
+        int wc = 0;
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4) {
+            const int e = r4 * 64 + lane;
+            const bool bit = e < cnt && ((qmask[e] >> w) & 1u);
+            const uint64_t bal = __ballot(bit);
+            if (bit) wlist[w][wc + __popcll(bal & ((1ull << lane) - 1ull))] = (uint8_t)e;
+            wc += __popcll(bal);
+        }
+        __syncthreads();
+        if (done) continue;
+        const int nb = base == 0 ? (cnt + kBucket - 1) / kBucket : 0;
+        int nextb = 0;
+        for (int k = 0; k < wc; ++k) {
+            const int i = wlist[w][k];
+            for (; nextb < nb && nextb * kBucket <= i; ++nextb)
+                ck[nextb * 256] = make_float4(T, cr, cg, cb);
+            if (step(i, base + (uint32_t)i)) { done = true; break; }
+        }
+        if (!done)
+            for (; nextb < nb; ++nextb) ck[nextb * 256] = make_float4(T, cr, cg, cb);
     }
     if (inside) {
         const size_t p = (size_t)y * v.W + x;
@@ -149,7 +208,7 @@ __global__ __launch_bounds__(256) void k_forward(
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
     __syncthreads();
-    if ((tid & 63) == 0) atomicMax(&smax, m);
+    if (lane == 0) atomicMax(&smax, m);
     __syncthreads();
     if (tid == 0) max_contrib[tile] = smax;
 }
@@ -203,13 +262,31 @@ __global__ __launch_bounds__(256) void k_backward(
     const bool valid = (uint32_t)(b * kBucket) < maxc && sidx < nsplat;
     float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
     if (valid) { r0 = L.recs[sidx].r0; r1 = L.recs[sidx].r1; r2 = L.recs[sidx].r2; }
+    // Checkpoints of this half's bucket: lane sl holds pixel 32c + sl of chunk c and the values
+    // move one lane down per step, so the half's first lane holds pixel i at step i. The next
+    // chunk is loaded 32 steps ahead.
     const float4* ck = ckpt + ((size_t)tile * kMaxBuckets + b) * 256;
+    // Loads are unconditional and in bounds (b < 8), so that no wait is forced at the load;
+    // values of idle halves are never used.
+    float4 ck_next = ck[sl];
+    float4 ckv = ck_next;
     float dmx = 0.f, dmy = 0.f, dsx = 0.f, dsy = 0.f, dsz = 0.f, dcr = 0.f, dcg = 0.f, dcb = 0.f,
           ddep = 0.f, dal = 0.f, dphx = 0.f, dphy = 0.f;
     uint32_t dn = 0;
     float bT = 0.f, ar = 0.f, ag = 0.f, ab = 0.f;
     const int wlim = v.W - tx0, hlim = v.H - ty0;
-    for (int i = 0; i < 256 + 31; ++i) {
+    for (int c = 0; c < 9; ++c) {   // 9 chunks of 32 steps cover steps 0 .. 256 + 30
+    ckv = ck_next;
+    ck_next = ck[min(32 * c + 32, 224) + sl];
+    const int tend = c < 8 ? 32 : 31;
+    for (int t = 0; t < tend; ++t) {
+        const int i = 32 * c + t;
+        if (t > 0) {   // uniform
+            ckv.x = lane_shift_down(ckv.x);
+            ckv.y = lane_shift_down(ckv.y);
+            ckv.z = lane_shift_down(ckv.z);
+            ckv.w = lane_shift_down(ckv.w);
+        }
         bT = lane_shift_up(bT);
         ar = lane_shift_up(ar);
         ag = lane_shift_up(ag);
@@ -218,8 +295,7 @@ __global__ __launch_bounds__(256) void k_backward(
         const bool pix_ok = p >= 0 && p < 256 && (p & 15) < wlim && (p >> 4) < hlim;
         if (!(valid && pix_ok)) continue;
         if (sl == 0) {
-            const float4 c = ck[p];
-            bT = c.x; ar = c.y; ag = c.z; ab = c.w;
+            bT = ckv.x; ar = ckv.y; ag = ckv.z; ab = ckv.w;
         }
         if (sidx >= L.pLast[p]) continue;
         const float dx = r0.x - (float)(tx0 + (p & 15));   // Q9: mean - pixel
@@ -230,7 +306,7 @@ __global__ __launch_bounds__(256) void k_backward(
         const float Er = L.pER[p], Eg = L.pEG[p], Eb = L.pEB[p];
         float dar = r2.x * bT, dag = r2.y * bT, dab = r2.z * bT;
         ar += dar * a; ag += dag * a; ab += dab * a;
-        const float iom = 1.0f / (1.0f - a);
+        const float iom = __builtin_amdgcn_rcpf(1.0f - a);
         dar -= (L.pR[p] - ar) * iom;
         dag -= (L.pG[p] - ag) * iom;
         dab -= (L.pB[p] - ab) * iom;
@@ -261,6 +337,7 @@ __global__ __launch_bounds__(256) void k_backward(
         dmx -= dc * r1.z; dmy -= dc * r1.w;
         dphx -= dc * dx; dphy -= dc * dy;
         bT = tT;
+    }
     }
     if (valid && dn > 0) {
         Delta2D* d = delta + idx[s + sidx];

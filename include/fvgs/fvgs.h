@@ -109,7 +109,9 @@ typedef struct fvgs_params {
     float lambda_iso;      /* 0.01 isotropy regulariser of the 2D->3D step, GSSlam.cu:1250 */
     int cull_emit;         /* 1: a culled Gaussian keeps the zero screen record and its single
                               tile-0 instance (the reference's first-call behaviour);
-                              0: culled Gaussians emit no instance */
+                              0: culled Gaussians emit no instance. The pose calls
+                              (fvgs_pose_jacobians, fvgs_render_tracking, fvgs_track) never
+                              emit them: their sums and renders do not depend on them */
     int keep_keys;         /* 1: keep the sorted 64-bit keys for fvgs_read(FVGS_OUT_KEYS) */
     size_t mem_budget;     /* 0 = unlimited; otherwise device bytes the context may allocate */
 } fvgs_params;

@@ -429,6 +429,20 @@ def test_pose_pass2_skips_culled_records():
                                   cq, it)
     assert np.isfinite(tiles).all() and np.isfinite(tot).all()
     assert np.abs(tiles[0]).max() > 0
+    # Pass 1 skips the zero records (v <= 0) and pass 2 skips them (Q15): the same lists
+    # without the culled instances give bit-identical sums (the GPU pose calls emit none).
+    keep = pf["depth"][b["idx"]] != 0
+    lens = b["ranges"][:, 1] - b["ranges"][:, 0]
+    tile_of = np.repeat(np.arange(len(lens)), lens)
+    new_lens = np.bincount(tile_of[keep], minlength=len(lens))
+    ends = np.cumsum(new_lens).astype(np.uint32)
+    b2 = dict(b, idx=np.ascontiguousarray(b["idx"][keep]),
+              ranges=np.ascontiguousarray(np.stack([ends - new_lens, ends], 1).astype(np.uint32)))
+    b2["ranges"][new_lens == 0] = 0
+    assert len(b2["idx"]) < len(b["idx"])
+    tiles2, tot2 = O.pose_jacobians(w, h, b2, rec, sc["color"], sc["opacity"], gx, gy, obs, dep,
+                                    cq, it)
+    np.testing.assert_array_equal(tiles2, tiles)
 
 
 # ------------------------------------------------------------------------------------------
