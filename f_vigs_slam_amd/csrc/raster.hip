@@ -214,19 +214,24 @@ __global__ __launch_bounds__(256) void k_forward(
 }
 
 // ---- A15 ------------------------------------------------------------------------------------
-// One workgroup per tile; each 64-lane wave replays two 32-entry buckets (lanes 0-31 and 32-63):
-// lane l owns entry b*32 + (l & 31), and at step i handles pixel i - (l & 31) with the
-// transmittance and accumulated colour received from lane l-1 (DPP wave shift), lanes 0 and 32
-// starting each pixel from the forward checkpoint of their bucket — the reference's warp
-// pipeline widened to wave64. Per-pixel forward results sit in LDS; one atomicAdd per field and
-// (entry, tile).
+// One 512-thread workgroup per tile; wave w replays bucket w (entries 32w .. 32w+31): lane l owns
+// entry 32w + (l & 31), the lower half-wave walks pixels 0-127 and the upper half pixels
+// 128-255. At step i lane l handles pixel i - (l & 31) of its half with the transmittance and
+// accumulated colour received from lane l-1 (DPP wave shift); the first lane of each half starts
+// each pixel from the forward checkpoint of the bucket — the reference's per-bucket warp
+// pipeline, split over two half-waves so that a tile has 8 waves in flight. The halves' partial
+// sums are added across the wave before one atomicAdd per field and (entry, tile). Per-pixel
+// forward results sit in LDS.
 struct BwdLds {
     Rec recs[256];
-    float pR[256], pG[256], pB[256], pD[256], pER[256], pEG[256], pEB[256], pObs[256];
-    uint32_t pLast[256];
+    float4 pix_a[256];      // rendered colour (r, g, b), median depth
+    float4 pix_b[256];      // colour error (r, g, b), observed depth
+    uint32_t pix_last[256]; // n_contrib; 0 for pixels outside the image
 };
 
-__global__ __launch_bounds__(256) void k_backward(
+__device__ __forceinline__ float add_halves(float v) { return v + __shfl_xor(v, 32); }
+
+__global__ __launch_bounds__(512) void k_backward(
     View v, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ idx,
     const Rec* __restrict__ rec, const float* __restrict__ obs_depth, BlendParams bp,
     const float* __restrict__ o_rgb, const float* __restrict__ o_depth,
@@ -240,106 +245,116 @@ __global__ __launch_bounds__(256) void k_backward(
     const int tx0 = (tile % v.ntx) * kTile, ty0 = (tile / v.ntx) * kTile;
     const uint32_t s = ranges[2 * tile], n = ranges[2 * tile + 1] - s;
     const uint32_t nsplat = min(n, 256u);
-    if (tid < (int)nsplat) L.recs[tid] = rec[idx[s + tid]];
-    {
-        const int x = tx0 + (tid & 15), y = ty0 + (tid >> 4);
+    if (tid < 256) {
+        if (tid < (int)nsplat) L.recs[tid] = rec[idx[s + tid]];
+    } else {
+        const int t = tid - 256;
+        const int x = tx0 + (t & 15), y = ty0 + (t >> 4);
         if (x < v.W && y < v.H) {
             const size_t p = (size_t)y * v.W + x;
-            L.pR[tid] = o_rgb[3 * p]; L.pG[tid] = o_rgb[3 * p + 1]; L.pB[tid] = o_rgb[3 * p + 2];
-            L.pER[tid] = o_err[3 * p]; L.pEG[tid] = o_err[3 * p + 1]; L.pEB[tid] = o_err[3 * p + 2];
-            L.pD[tid] = o_depth[p];
-            L.pObs[tid] = obs_depth[p];
-            L.pLast[tid] = o_nc[p];
+            L.pix_a[t] = make_float4(o_rgb[3 * p], o_rgb[3 * p + 1], o_rgb[3 * p + 2], o_depth[p]);
+            L.pix_b[t] = make_float4(o_err[3 * p], o_err[3 * p + 1], o_err[3 * p + 2], obs_depth[p]);
+            L.pix_last[t] = o_nc[p];
         } else {
-            L.pLast[tid] = 0;
+            L.pix_a[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+            L.pix_b[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+            L.pix_last[t] = 0;
         }
     }
     __syncthreads();
-    const int w = tid >> 6, lane = tid & 63, sl = lane & 31;
-    const int b = 2 * w + (lane >> 5);
-    if ((uint32_t)(2 * w * kBucket) >= maxc) return;   // both buckets of this wave are idle
+    const int b = tid >> 6, lane = tid & 63, sl = lane & 31, half = lane >> 5;
+    if ((uint32_t)(b * kBucket) >= maxc || (uint32_t)(b * kBucket) >= nsplat) return;   // uniform
     const uint32_t sidx = (uint32_t)(b * kBucket + sl);
-    const bool valid = (uint32_t)(b * kBucket) < maxc && sidx < nsplat;
+    const bool valid = sidx < nsplat;
     float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
     if (valid) { r0 = L.recs[sidx].r0; r1 = L.recs[sidx].r1; r2 = L.recs[sidx].r2; }
-    // Checkpoints of this half's bucket: lane sl holds pixel 32c + sl of chunk c and the values
-    // move one lane down per step, so the half's first lane holds pixel i at step i. The next
-    // chunk is loaded 32 steps ahead.
-    const float4* ck = ckpt + ((size_t)tile * kMaxBuckets + b) * 256;
-    // Loads are unconditional and in bounds (b < 8), so that no wait is forced at the load;
-    // values of idle halves are never used.
+    // Checkpoints of this half's pixels: lane sl holds pixel 128 half + 32c + sl of chunk c and
+    // the values move one lane down per step, so the half's first lane holds pixel i of the half
+    // at step i. The next chunk is loaded 32 steps ahead. Loads are unconditional and in bounds;
+    // values of pixels outside the image are never used.
+    const float4* ck = ckpt + ((size_t)tile * kMaxBuckets + b) * 256 + 128 * half;
     float4 ck_next = ck[sl];
     float4 ckv = ck_next;
     float dmx = 0.f, dmy = 0.f, dsx = 0.f, dsy = 0.f, dsz = 0.f, dcr = 0.f, dcg = 0.f, dcb = 0.f,
           ddep = 0.f, dal = 0.f, dphx = 0.f, dphy = 0.f;
     uint32_t dn = 0;
     float bT = 0.f, ar = 0.f, ag = 0.f, ab = 0.f;
-    const int wlim = v.W - tx0, hlim = v.H - ty0;
-    for (int c = 0; c < 9; ++c) {   // 9 chunks of 32 steps cover steps 0 .. 256 + 30
-    ckv = ck_next;
-    ck_next = ck[min(32 * c + 32, 224) + sl];
-    const int tend = c < 8 ? 32 : 31;
-    for (int t = 0; t < tend; ++t) {
-        const int i = 32 * c + t;
-        if (t > 0) {   // uniform
-            ckv.x = lane_shift_down(ckv.x);
-            ckv.y = lane_shift_down(ckv.y);
-            ckv.z = lane_shift_down(ckv.z);
-            ckv.w = lane_shift_down(ckv.w);
+    const bool first = sl == 0;
+    const float mx0 = r0.x - (float)tx0, my0 = r0.y - (float)ty0;   // Q9: mean - pixel
+    for (int c = 0; c < 5; ++c) {   // 5 chunks of 32 steps cover steps 0 .. 128 + 30
+        ckv = ck_next;
+        ck_next = ck[min(32 * c + 32, 96) + sl];
+        const int tend = c < 4 ? 32 : 31;
+        for (int t = 0; t < tend; ++t) {
+            const int i = 32 * c + t;
+            if (t > 0) {   // uniform
+                ckv.x = lane_shift_down(ckv.x);
+                ckv.y = lane_shift_down(ckv.y);
+                ckv.z = lane_shift_down(ckv.z);
+                ckv.w = lane_shift_down(ckv.w);
+            }
+            // State of the pixel from the left lane; each half's first lane takes the checkpoint.
+            bT = lane_shift_up(bT);
+            ar = lane_shift_up(ar);
+            ag = lane_shift_up(ag);
+            ab = lane_shift_up(ab);
+            bT = first ? ckv.x : bT;
+            ar = first ? ckv.y : ar;
+            ag = first ? ckv.z : ag;
+            ab = first ? ckv.w : ab;
+            const int ph = i - sl;                    // pixel within the half
+            const int pc = (128 * half + ph) & 255;   // pixel within the tile
+            const uint32_t plast = L.pix_last[pc];
+            const float dx = mx0 - (float)(pc & 15);
+            const float dy = my0 - (float)(pc >> 4);
+            const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;
+            const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));   // Q8: no q <= 0 test
+            if (!(valid && (unsigned)ph < 128u && sidx < plast) || a < kAlphaMin) continue;
+            const float4 pa = L.pix_a[pc], pb = L.pix_b[pc];
+            const float Er = pb.x, Eg = pb.y, Eb = pb.z;
+            float dar = r2.x * bT, dag = r2.y * bT, dab = r2.z * bT;
+            ar += dar * a; ag += dag * a; ab += dab * a;
+            const float iom = __builtin_amdgcn_rcpf(1.0f - a);
+            dar -= (pa.x - ar) * iom;
+            dag -= (pa.y - ag) * iom;
+            dab -= (pa.z - ab) * iom;
+            dar *= -Er; dag *= -Eg; dab *= -Eb;
+            const float aG = a * (dar + dag + dab);
+            ++dn;
+            const float at = a * bT;
+            dcr -= at * Er; dcg -= at * Eg; dcb -= at * Eb;
+            dal -= aG;
+            dmx -= aG * (r0.z * dx + r0.w * dy);
+            dmy -= aG * (r0.w * dx + r1.x * dy);
+            dsx -= 0.5f * aG * dx * dx;
+            dsy -= 0.5f * aG * dx * dy;
+            dsz -= 0.5f * aG * dy * dy;
+            const float tT = bT * (1.0f - a);
+            const float Dp = pa.w;
+            if (bT > 0.5f && tT <= 0.5f) {
+                const float obs = pb.w;
+                const float de = (obs > 0.1f) ? (Dp - obs) : 0.0f;
+                const float wde = bp.w_depth * de;
+                ddep -= wde;
+                dmx -= wde * r1.z; dmy -= wde * r1.w;
+                dphx -= wde * dx; dphy -= wde * dy;
+            }
+            const float di = r1.y + dx * r1.z + dy * r1.w;
+            const float dc = bp.w_dist * a * bT * (di - Dp);
+            ddep -= dc;
+            dmx -= dc * r1.z; dmy -= dc * r1.w;
+            dphx -= dc * dx; dphy -= dc * dy;
+            bT = tT;
         }
-        bT = lane_shift_up(bT);
-        ar = lane_shift_up(ar);
-        ag = lane_shift_up(ag);
-        ab = lane_shift_up(ab);
-        const int p = i - sl;
-        const bool pix_ok = p >= 0 && p < 256 && (p & 15) < wlim && (p >> 4) < hlim;
-        if (!(valid && pix_ok)) continue;
-        if (sl == 0) {
-            bT = ckv.x; ar = ckv.y; ag = ckv.z; ab = ckv.w;
-        }
-        if (sidx >= L.pLast[p]) continue;
-        const float dx = r0.x - (float)(tx0 + (p & 15));   // Q9: mean - pixel
-        const float dy = r0.y - (float)(ty0 + (p >> 4));
-        const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;
-        const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));   // Q8: no q <= 0 test
-        if (a < kAlphaMin) continue;
-        const float Er = L.pER[p], Eg = L.pEG[p], Eb = L.pEB[p];
-        float dar = r2.x * bT, dag = r2.y * bT, dab = r2.z * bT;
-        ar += dar * a; ag += dag * a; ab += dab * a;
-        const float iom = __builtin_amdgcn_rcpf(1.0f - a);
-        dar -= (L.pR[p] - ar) * iom;
-        dag -= (L.pG[p] - ag) * iom;
-        dab -= (L.pB[p] - ab) * iom;
-        dar *= -Er; dag *= -Eg; dab *= -Eb;
-        const float aG = a * (dar + dag + dab);
-        ++dn;
-        const float at = a * bT;
-        dcr -= at * Er; dcg -= at * Eg; dcb -= at * Eb;
-        dal -= aG;
-        dmx -= aG * (r0.z * dx + r0.w * dy);
-        dmy -= aG * (r0.w * dx + r1.x * dy);
-        dsx -= 0.5f * aG * dx * dx;
-        dsy -= 0.5f * aG * dx * dy;
-        dsz -= 0.5f * aG * dy * dy;
-        const float tT = bT * (1.0f - a);
-        const float Dp = L.pD[p];
-        if (bT > 0.5f && tT <= 0.5f) {
-            const float obs = L.pObs[p];
-            const float de = (obs > 0.1f) ? (Dp - obs) : 0.0f;
-            const float wde = bp.w_depth * de;
-            ddep -= wde;
-            dmx -= wde * r1.z; dmy -= wde * r1.w;
-            dphx -= wde * dx; dphy -= wde * dy;
-        }
-        const float di = r1.y + dx * r1.z + dy * r1.w;
-        const float dc = bp.w_dist * a * bT * (di - Dp);
-        ddep -= dc;
-        dmx -= dc * r1.z; dmy -= dc * r1.w;
-        dphx -= dc * dx; dphy -= dc * dy;
-        bT = tT;
     }
-    }
-    if (valid && dn > 0) {
+    // Lower half + upper half of each entry (fixed order).
+    ddep = add_halves(ddep); dphx = add_halves(dphx); dphy = add_halves(dphy);
+    dmx = add_halves(dmx); dmy = add_halves(dmy);
+    dsx = add_halves(dsx); dsy = add_halves(dsy); dsz = add_halves(dsz);
+    dcr = add_halves(dcr); dcg = add_halves(dcg); dcb = add_halves(dcb);
+    dal = add_halves(dal);
+    dn += (uint32_t)__shfl_xor((int)dn, 32);
+    if (half == 0 && valid && dn > 0) {
         Delta2D* d = delta + idx[s + sidx];
         atomicAdd(&d->depth, ddep);
         atomicAdd(&d->phx, dphx);
@@ -602,7 +617,7 @@ hipError_t launch_backward(const View& v, const uint32_t* ranges, const uint32_t
                            const float* rgb, const float* depth, const uint32_t* n_contrib,
                            const float* color_err, const uint32_t* max_contrib,
                            const float4* ckpt, Delta2D* delta, hipStream_t s) {
-    hipLaunchKernelGGL(k_backward, dim3(v.ntx * v.nty), dim3(256), 0, s, v, ranges, idx, rec,
+    hipLaunchKernelGGL(k_backward, dim3(v.ntx * v.nty), dim3(512), 0, s, v, ranges, idx, rec,
                        obs_depth, bp, rgb, depth, n_contrib, color_err, max_contrib, ckpt, delta);
     return hipGetLastError();
 }
