@@ -9,14 +9,15 @@
 //   k_project    one thread per Gaussian, 1024 Gaussians per block: screen record (64 B),
 //                tile rectangle, and a per-block tile histogram in LDS written as one row of
 //                the [blocks x tiles] count matrix (no global atomics).
-//   k_colscan1/2 column prefix of the count matrix in chunks of 64 rows, then the tile
-//                totals' exclusive scan: tile ranges and the instance count M.
+//   k_colscan1/2 column prefix of the count matrix in chunks of kScanRows rows, then the
+//                tile totals' exclusive scan: tile ranges and the instance count M.
 //   k_emit       per block, LDS cursors at the block's base offset in every tile; each instance
 //                writes (depth bits << 32 | gaussian index) into its tile's segment.
-//   k_tile_sort  one workgroup per tile: LSD radix sort of the unique key
-//                ((depth bits - tile min) << index bits | index), 8-bit digits, stable
-//                wave-level ranks from ballots. Segments up to NT*E entries stay in LDS;
-//                longer ones stream through a global ping-pong buffer.
+//   k_tile_sort  one workgroup per tile: LSD radix sort by (depth bits, index), 8-bit
+//                digits, stable wave-level ranks from ballots. Segments up to NT*E entries
+//                stay in LDS and sort on the depth bits, then order equal depths by index;
+//                longer ones sort the unique key ((depth bits - tile min) << index bits |
+//                index) through a global ping-pong buffer.
 // The resulting order equals the reference's stable sort of (tile << 32 | depth bits) with
 // emission in Gaussian-index order: ties on the depth bits resolve by ascending index.
 //
@@ -300,8 +301,10 @@ __global__ __launch_bounds__(kBinThreads) void k_emit(
 // ------------------------------------------------------------------------------------------
 // Per-tile LSD radix sort.
 //
-// One workgroup per tile sorts the unique 64-bit key ((depth bits - tile min) << idx_bits | idx)
-// on its significant bits, 8 bits per pass. Ranks are stable: within a wave, item by item, the
+// One workgroup per tile sorts by (depth bits, index), 8 bits per pass. Lists that fit in LDS
+// are sorted on the depth bits only and the (rare) runs of equal depth are then ordered by
+// index; longer lists, and lists with many equal depths, sort the unique 64-bit key
+// ((depth bits - tile min) << idx_bits | idx) on its significant bits. Ranks are stable: within a wave, item by item, the
 // lanes holding the same digit are found with 8 ballots ("match"), ranked by lane order and
 // offset by a per-wave digit counter in LDS; per-digit offsets across waves come from a prefix
 // over the per-wave counters. Elements are laid out w*64*ipt + e*64 + lane with ipt =
@@ -486,20 +489,69 @@ __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ r
         return ((uint64_t)((uint32_t)(p >> 32) - mn) << idx_bits) | (p & 0xffffffffull);
     };
     if (resident) {
-        for (uint32_t j = tid; j < n; j += NT) keysA[j] = xform(keysA[j]);
+        // Depth passes only: key = (depth bits - min) << 32 | idx, sorted on the depth bits.
+        // LSD passes keep equal depths in emission order; the runs of equal depth (rare among
+        // visible Gaussians) are then put in index order by odd-even transposition. Tiles with
+        // more than 64 tied neighbours (e.g. many culled zero records) take the full sort of
+        // the unique key instead.
+        for (uint32_t j = tid; j < n; j += NT) {
+            const uint64_t p = keysA[j];
+            keysA[j] = ((uint64_t)((uint32_t)(p >> 32) - mn) << 32) | (p & 0xffffffffull);
+        }
+        if (tid == 0) misc[63] = 0;
         __syncthreads();
         const int ipt = (int)((n + NT - 1) / NT);
+        const int dpasses = ((span ? 32 - __clz(span) : 0) + 7) / 8;
         uint64_t* src = keysA;
         uint64_t* dst = keysB;
-        for (int p = 0; p < passes; ++p) {
-            radix_pass<NT, E>(src, (int)n, ipt, 8 * p, whist, dtot, dbase, misc, dst, nullptr, n);
+        for (int p = 0; p < dpasses; ++p) {
+            radix_pass<NT, E>(src, (int)n, ipt, 32 + 8 * p, whist, dtot, dbase, misc, dst, nullptr,
+                              n);
             uint64_t* t = src; src = dst; dst = t;
+        }
+        uint32_t my_ties = 0;
+        for (uint32_t j = tid; j + 1 < n; j += NT) my_ties += (src[j] >> 32) == (src[j + 1] >> 32);
+        if (my_ties) atomicAdd(&misc[63], my_ties);
+        __syncthreads();
+        const uint32_t ties = misc[63];
+        __syncthreads();
+        if (ties > 64u) {
+            for (uint32_t j = tid; j < n; j += NT) keysA[j] = xform(pairs[s + j]);
+            __syncthreads();
+            src = keysA;
+            dst = keysB;
+            for (int p = 0; p < passes; ++p) {
+                radix_pass<NT, E>(src, (int)n, ipt, 8 * p, whist, dtot, dbase, misc, dst, nullptr,
+                                  n);
+                uint64_t* t = src; src = dst; dst = t;
+            }
+            for (uint32_t j = tid; j < n; j += NT) {
+                const uint64_t key = src[j];
+                sorted_idx[s + j] = (uint32_t)(key & idx_mask);
+                if (keys_out)
+                    keys_out[s + j] = ((uint64_t)tile << 32) | (uint32_t)((key >> idx_bits) + mn);
+            }
+            return;
+        }
+        if (ties > 0u) {
+            int quiet = 0;
+            for (int round = 0; quiet < 2; ++round) {
+                bool swapped = false;
+                for (uint32_t j = 2 * tid + (round & 1); j + 1 < n; j += 2 * NT) {
+                    const uint64_t a = src[j], b = src[j + 1];
+                    if ((a >> 32) == (b >> 32) && (uint32_t)a > (uint32_t)b) {
+                        src[j] = b;
+                        src[j + 1] = a;
+                        swapped = true;
+                    }
+                }
+                quiet = __syncthreads_or(swapped) ? 0 : quiet + 1;
+            }
         }
         for (uint32_t j = tid; j < n; j += NT) {
             const uint64_t key = src[j];
-            sorted_idx[s + j] = (uint32_t)(key & idx_mask);
-            if (keys_out)
-                keys_out[s + j] = ((uint64_t)tile << 32) | (uint32_t)((key >> idx_bits) + mn);
+            sorted_idx[s + j] = (uint32_t)key;
+            if (keys_out) keys_out[s + j] = ((uint64_t)tile << 32) | (uint32_t)((key >> 32) + mn);
         }
         return;
     }

@@ -11,10 +11,10 @@ constexpr int kTilePix = 256;      // pixels per tile (reference BLOCK_SIZE)
 constexpr int kBucket = 32;        // backward bucket length (GSCudaKernels.cu:2087)
 constexpr int kMaxBuckets = 8;     // only the first 256 entries get gradients (Q10)
 constexpr int kMaxTiles = 8192;    // LDS tile histogram of the binning kernels (32 KB)
-constexpr int kBinThreads = 256;   // binning block
+constexpr int kBinThreads = 1024;  // binning block
 constexpr int kBinItems = 4;       // Gaussians per thread per binning block
 constexpr int kBinChunk = kBinThreads * kBinItems;
-constexpr int kScanRows = 64;      // binning blocks per column-scan chunk
+constexpr int kScanRows = 16;      // binning blocks per column-scan chunk
 
 // View description passed by value to kernels.
 struct View {

@@ -64,6 +64,24 @@ def test_binning_bitexact_perturbed_with_culls(ctx):
     assert b["keys"][0] == 0
 
 
+@pytest.mark.parametrize("dup_frac", [0.05, 0.4])
+def test_binning_bitexact_with_equal_depths(ctx, dup_frac):
+    """Duplicated Gaussians give equal depth keys inside a tile: few ties take the depth-only
+    sort plus index ordering of equal runs, many ties (> 64 in a tile) the full-key sort."""
+    scene, intr, pose = workload(50000, seed=8)
+    rng = np.random.default_rng(8)
+    dst = rng.choice(50000, int(dup_frac * 50000), replace=False)
+    src = rng.choice(50000, len(dst), replace=True)
+    for k in scene:
+        scene[k][dst] = scene[k][src]
+    _, b, _ = _check_binning(ctx, scene, pose, intr, W, H)
+    lens = b["ranges"][:, 1] - b["ranges"][:, 0]
+    k = b["keys"]
+    ties = np.count_nonzero(k[1:] == k[:-1])
+    print(f"[binning] {ties} equal adjacent keys over {len(lens)} tiles", flush=True)
+    assert ties > 0
+
+
 def test_binning_long_tile_lists(ctx):
     # Large Gaussians on a 128x128 view: lists past 2048 (512-thread LDS-resident sort) and past
     # 8192 (streamed passes through HBM).
