@@ -631,12 +631,12 @@ hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uin
     int idx_bits = 1;
     while (idx_bits < 32 && ((uint64_t)1 << idx_bits) < (uint64_t)n_gauss) ++idx_bits;
     // Short lists: 256 threads, 2048 keys resident (~36 KB LDS, several workgroups per CU).
-    // Longer: 512 threads, 8192 keys resident (~137 KB), streaming beyond that.
+    // Longer: 1024 threads, 8192 keys resident (~146 KB), streaming beyond that.
     using Small = SortCfg<256, 8>;
-    using Large = SortCfg<512, 16>;
+    using Large = SortCfg<1024, 8>;
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)k_tile_sort<512, 16>,
+        (void)hipFuncSetAttribute((const void*)k_tile_sort<1024, 8>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)Large::kLds);
         attr_set = true;
     }
@@ -646,7 +646,7 @@ hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uin
                        idx_bits, pairs, tmp, sorted_idx, keys_out, 0u, small_cap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_tile_sort<512, 16>), dim3(ntiles), dim3(512), Large::kLds, s, ranges,
+    hipLaunchKernelGGL((k_tile_sort<1024, 8>), dim3(ntiles), dim3(1024), Large::kLds, s, ranges,
                        idx_bits, pairs, tmp, sorted_idx, keys_out, small_cap, 0xffffffffu);
     return hipGetLastError();
 }

@@ -231,7 +231,7 @@ struct BwdLds {
 
 __device__ __forceinline__ float add_halves(float v) { return v + __shfl_xor(v, 32); }
 
-__global__ __launch_bounds__(512) void k_backward(
+__global__ __launch_bounds__(512, 8) void k_backward(
     View v, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ idx,
     const Rec* __restrict__ rec, const float* __restrict__ obs_depth, BlendParams bp,
     const float* __restrict__ o_rgb, const float* __restrict__ o_depth,
