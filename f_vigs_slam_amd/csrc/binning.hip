@@ -158,9 +158,10 @@ __device__ __forceinline__ uint32_t tile_depth_bits(float u, float v, float z, f
 __global__ __launch_bounds__(kBinThreads) void k_project(
     const float* __restrict__ pos, const float* __restrict__ scale,
     const float* __restrict__ quat, const float* __restrict__ color,
-    const float* __restrict__ opac, int n, View v, int cull_emit, Rec* __restrict__ rec,
-    uint2* __restrict__ rect, uint32_t* __restrict__ blk_counts) {
+    const float* __restrict__ opac, int n, View vh, const View* __restrict__ vd, int cull_emit,
+    Rec* __restrict__ rec, uint2* __restrict__ rect, uint32_t* __restrict__ blk_counts) {
     extern __shared__ uint32_t hist[];
+    const View v = vd ? *vd : vh;   // device-resident view (pose updated on the device) or argument
     const int T = v.ntx * v.nty;
     for (int t = threadIdx.x; t < T; t += kBinThreads) hist[t] = 0;
     __syncthreads();
@@ -275,8 +276,9 @@ __global__ __launch_bounds__(1024) void k_colscan2(uint32_t* __restrict__ chunk_
 __global__ __launch_bounds__(kBinThreads) void k_emit(
     const Rec* __restrict__ rec, const uint2* __restrict__ rect,
     const uint32_t* __restrict__ blk_counts, const uint32_t* __restrict__ chunk_pre, int n,
-    View v, uint64_t* __restrict__ pairs, uint64_t cap) {
+    View vh, const View* __restrict__ vd, uint64_t* __restrict__ pairs, uint64_t cap) {
     extern __shared__ uint32_t cur[];
+    const View v = vd ? *vd : vh;
     const int T = v.ntx * v.nty;
     const uint32_t* row = blk_counts + (size_t)blockIdx.x * T;
     const uint32_t* crow = chunk_pre + (size_t)(blockIdx.x / kScanRows) * T;
@@ -593,12 +595,12 @@ __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ r
 
 hipError_t launch_project(const float* pos, const float* scale, const float* quat,
                           const float* color, const float* opac, int n, const View& v,
-                          int cull_emit, Rec* rec, uint2* rect, uint32_t* blk_counts,
-                          hipStream_t s) {
+                          const View* vd, int cull_emit, Rec* rec, uint2* rect,
+                          uint32_t* blk_counts, hipStream_t s) {
     int nblk = (n + kBinChunk - 1) / kBinChunk;
     size_t lds = (size_t)v.ntx * v.nty * 4;
     hipLaunchKernelGGL(k_project, dim3(nblk), dim3(kBinThreads), lds, s, pos, scale, quat, color,
-                       opac, n, v, cull_emit, rec, rect, blk_counts);
+                       opac, n, v, vd, cull_emit, rec, rect, blk_counts);
     return hipGetLastError();
 }
 
@@ -616,12 +618,12 @@ hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* chunk_pre, int nblk, i
 }
 
 hipError_t launch_emit(const Rec* rec, const uint2* rect, const uint32_t* blk_counts,
-                       const uint32_t* chunk_pre, int n, const View& v, uint64_t* pairs,
-                       uint64_t cap, hipStream_t s) {
+                       const uint32_t* chunk_pre, int n, const View& v, const View* vd,
+                       uint64_t* pairs, uint64_t cap, hipStream_t s) {
     int nblk = (n + kBinChunk - 1) / kBinChunk;
     size_t lds = (size_t)v.ntx * v.nty * 4;
     hipLaunchKernelGGL(k_emit, dim3(nblk), dim3(kBinThreads), lds, s, rec, rect, blk_counts,
-                       chunk_pre, n, v, pairs, cap);
+                       chunk_pre, n, v, vd, pairs, cap);
     return hipGetLastError();
 }
 

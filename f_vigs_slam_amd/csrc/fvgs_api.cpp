@@ -6,6 +6,7 @@
 // needed (the instance count M to size the sort buffers, the 27 FP64 sums of a pose call).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -13,9 +14,11 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/fvgs/fvgs.h"
 #include "fvgs_internal.h"
+#include "gn_step.h"
 
 using namespace fvgs;
 
@@ -62,6 +65,11 @@ struct fvgs_ctx {
     DevBuf pose_tiles, pose_sum, t_rgb, t_depth;
     uint32_t* h_m = nullptr;      // pinned
     double* h_sum = nullptr;      // pinned, 27
+    // device-resident tracking loop: [View | pose7 | fail | status[iters] | M[iters]]
+    DevBuf track;
+    uint8_t* h_track = nullptr;   // pinned mirror of `track`
+    size_t h_track_bytes = 0;
+    std::vector<hipEvent_t> track_ev;
 };
 
 namespace {
@@ -219,7 +227,7 @@ int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int
     hipStream_t s = c->stream;
     c->view = v;
     mark(c, 0);
-    HIPCHK(c, launch_project(c->pos, c->scale, c->quat, c->color, c->opac, n, v,
+    HIPCHK(c, launch_project(c->pos, c->scale, c->quat, c->color, c->opac, n, v, nullptr,
                              cull_emit, as<Rec>(c->rec), as<uint2>(c->rect),
                              as<uint32_t>(c->blk_counts), s));
     HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->chunk_pre), nblk, T,
@@ -234,7 +242,8 @@ int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int
     }
     uint64_t* keys = c->prm.keep_keys ? as<uint64_t>(c->keys) : nullptr;
     HIPCHK(c, launch_emit(as<Rec>(c->rec), as<uint2>(c->rect), as<uint32_t>(c->blk_counts),
-                          as<uint32_t>(c->chunk_pre), n, v, as<uint64_t>(c->pairs), c->inst_cap,
+                          as<uint32_t>(c->chunk_pre), n, v, nullptr, as<uint64_t>(c->pairs),
+                          c->inst_cap,
                           s));
     HIPCHK(c, launch_tile_sort(as<uint32_t>(c->ranges), T, n, as<uint64_t>(c->pairs),
                                as<uint64_t>(c->tmp), as<uint32_t>(c->sorted_idx), keys,
@@ -335,7 +344,7 @@ int run_pose(fvgs_ctx* c, bool want_render) {
         trgb = as<float>(c->t_rgb);
         tdep = as<float>(c->t_depth);
     }
-    HIPCHK(c, launch_pose(c->view, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
+    HIPCHK(c, launch_pose(c->view, nullptr, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
                           as<Rec>(c->rec), c->obs_rgb, c->obs_depth, c->gx, c->gy,
                           blend_params(c->prm), as<double>(c->pose_tiles), trgb, tdep, c->stream));
     HIPCHK(c, launch_pose_reduce(as<double>(c->pose_tiles), T, as<double>(c->pose_sum), c->stream));
@@ -464,6 +473,9 @@ int fvgs_destroy(fvgs_ctx* c) {
         for (DevBuf* b : bufs) release(c, *b);
         for (auto& e : c->ev)
             if (e) (void)hipEventDestroy(e);
+        release(c, c->track);
+        for (hipEvent_t e : c->track_ev) (void)hipEventDestroy(e);
+        if (c->h_track) (void)hipHostFree(c->h_track);
         if (c->h_m) (void)hipHostFree(c->h_m);
         if (c->h_sum) (void)hipHostFree(c->h_sum);
         if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -640,7 +652,7 @@ int fvgs_render_tracking(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsi
     TRY(ensure(c, c->pose_tiles, 27 * 8ull * T));
     TRY(ensure(c, c->t_rgb, 12 * P));
     TRY(ensure(c, c->t_depth, 4 * P));
-    HIPCHK(c, launch_pose(c->view, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
+    HIPCHK(c, launch_pose(c->view, nullptr, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
                           as<Rec>(c->rec), nullptr, nullptr, nullptr, nullptr,
                           blend_params(c->prm), as<double>(c->pose_tiles), as<float>(c->t_rgb),
                           as<float>(c->t_depth), c->stream));
@@ -666,77 +678,108 @@ int fvgs_compute_gradients(fvgs_ctx* c) {
 int fvgs_gn_step(const double jtj21[21], const double jtr6[6], double pose7[7],
                  double delta6[6]) {
     if (!jtj21 || !jtr6 || !pose7) return FVGS_E_ARG;
-    double A[6][6];
-    int id = 0;
-    for (int r = 0; r < 6; r++)
-        for (int cc = r; cc < 6; cc++) {
-            A[r][cc] = jtj21[id];
-            A[cc][r] = jtj21[id];
-            id++;
-        }
-    double L[6][6] = {{0}};
-    for (int j = 0; j < 6; j++) {
-        double sacc = A[j][j];
-        for (int k = 0; k < j; k++) sacc -= L[j][k] * L[j][k];
-        if (!(sacc > 0.0) || !std::isfinite(sacc)) return FVGS_E_NOTPD;
-        L[j][j] = std::sqrt(sacc);
-        for (int i = j + 1; i < 6; i++) {
-            double t = A[i][j];
-            for (int k = 0; k < j; k++) t -= L[i][k] * L[j][k];
-            L[i][j] = t / L[j][j];
-        }
-    }
-    double y[6], d[6];
-    for (int i = 0; i < 6; i++) {
-        double t = jtr6[i];
-        for (int k = 0; k < i; k++) t -= L[i][k] * y[k];
-        y[i] = t / L[i][i];
-    }
-    for (int i = 5; i >= 0; i--) {
-        double t = y[i];
-        for (int k = i + 1; k < 6; k++) t -= L[k][i] * d[k];
-        d[i] = t / L[i][i];
-    }
-    if (delta6)
-        for (int i = 0; i < 6; i++) delta6[i] = d[i];
-    // PoseLocalParameterization::Plus: p += dp; q = normalize(q * [1, dtheta / 2]).
-    pose7[0] += d[0];
-    pose7[1] += d[1];
-    pose7[2] += d[2];
-    const double qx = pose7[3], qy = pose7[4], qz = pose7[5], qw = pose7[6];
-    const double bx = 0.5 * d[3], by = 0.5 * d[4], bz = 0.5 * d[5];
-    const double rw = qw - qx * bx - qy * by - qz * bz;
-    const double rx = qw * bx + qx + qy * bz - qz * by;
-    const double ry = qw * by - qx * bz + qy + qz * bx;
-    const double rz = qw * bz + qx * by - qy * bx + qz;
-    const double nrm = std::sqrt(rw * rw + rx * rx + ry * ry + rz * rz);
-    pose7[3] = rx / nrm;
-    pose7[4] = ry / nrm;
-    pose7[5] = rz / nrm;
-    pose7[6] = rw / nrm;
-    return FVGS_OK;
+    return gn_solve_plus(jtj21, jtr6, pose7, delta6) ? FVGS_OK : FVGS_E_NOTPD;
 }
 
 int fvgs_track(fvgs_ctx* c, const fvgs_intrinsics* intr, int iters, double pose7[7],
                double* step_ms) {
     if (!c || !intr || !pose7 || iters < 0) return FVGS_E_ARG;
-    for (int it = 0; it < iters; it++) {
-        auto t0 = std::chrono::steady_clock::now();
-        fvgs_pose p;
-        for (int k = 0; k < 3; k++) p.position[k] = (float)pose7[k];
-        for (int k = 0; k < 4; k++) p.orientation[k] = (float)pose7[3 + k];
-        double jtj[21], jtr[6];
-        int r = fvgs_pose_jacobians(c, &p, intr, jtj, jtr);
-        if (r != FVGS_OK) return r;
-        r = fvgs_gn_step(jtj, jtr, pose7, nullptr);
-        if (r != FVGS_OK) {
-            std::lock_guard<std::mutex> lk(c->mu);
-            return fail(c, r, "Gauss-Newton step failed at iteration %d (JtJ not SPD)", it);
-        }
-        if (step_ms)
-            step_ms[it] = std::chrono::duration<double, std::milli>(
-                              std::chrono::steady_clock::now() - t0).count();
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->prm.device);
+    if (iters == 0) return FVGS_OK;
+    if (!c->obs_rgb) return fail(c, FVGS_E_STATE, "no frame set");
+    for (int k = 0; k < 7; k++)
+        if (!std::isfinite(pose7[k])) return fail(c, FVGS_E_ARG, "non-finite pose");
+    // A synchronous prepare at the start pose sets the view and sizes the buffers for its M;
+    // the loop then runs on the device with no host round trip: projection, binning, Jacobians,
+    // FP64 reduction and the Gauss-Newton step (which writes the next iteration's pose).
+    fvgs_pose p0;
+    for (int k = 0; k < 3; k++) p0.position[k] = (float)pose7[k];
+    for (int k = 0; k < 4; k++) p0.orientation[k] = (float)pose7[3 + k];
+    TRY(prepare(c, &p0, intr, c->fw, c->fh, false, 0));
+    TRY(check_frame(c));
+    if (!c->gx || !c->gy) return fail(c, FVGS_E_STATE, "frame has no image gradients");
+    if ((uint64_t)c->inst_cap < (uint64_t)c->m + c->m / 2)
+        TRY(ensure_instances(c, (uint64_t)c->m + c->m / 2));
+    const View v0 = c->view;
+    const int T = v0.ntx * v0.nty;
+    const int n = (int)c->n;
+    const int nblk = (n + kBinChunk - 1) / kBinChunk;
+    const size_t off_pose = 64, off_fail = off_pose + 7 * 8, off_status = off_fail + 8;
+    const size_t off_m = off_status + 4 * (size_t)iters, bytes = off_m + 4 * (size_t)iters;
+    static_assert(sizeof(View) <= 64, "view header");
+    TRY(ensure(c, c->track, bytes));
+    TRY(ensure(c, c->pose_tiles, 27 * 8ull * T));
+    TRY(ensure(c, c->pose_sum, 27 * 8));
+    if (c->h_track_bytes < bytes) {
+        if (c->h_track) (void)hipHostFree(c->h_track);
+        c->h_track = nullptr;
+        c->h_track_bytes = 0;
+        HIPCHK(c, hipHostMalloc((void**)&c->h_track, bytes, hipHostMallocDefault));
+        c->h_track_bytes = bytes;
     }
+    while ((int)c->track_ev.size() < iters + 1) {
+        hipEvent_t e;
+        HIPCHK(c, hipEventCreate(&e));
+        c->track_ev.push_back(e);
+    }
+    uint8_t* d = as<uint8_t>(c->track);
+    View* vd = reinterpret_cast<View*>(d);
+    double* dpose = reinterpret_cast<double*>(d + off_pose);
+    int* dfail = reinterpret_cast<int*>(d + off_fail);
+    int* dstatus = reinterpret_cast<int*>(d + off_status);
+    uint32_t* dm = reinterpret_cast<uint32_t*>(d + off_m);
+    const BlendParams bp = blend_params(c->prm);
+    hipStream_t s = c->stream;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        std::memcpy(c->h_track, &v0, sizeof(View));
+        std::memcpy(c->h_track + off_pose, pose7, 7 * 8);
+        std::memset(c->h_track + off_fail, 0, 8);
+        HIPCHK(c, hipMemcpyAsync(d, c->h_track, off_status, hipMemcpyHostToDevice, s));
+        for (int it = 0; it < iters; it++) {
+            HIPCHK(c, hipEventRecord(c->track_ev[it], s));
+            HIPCHK(c, launch_project(c->pos, c->scale, c->quat, c->color, c->opac, n, v0, vd, 0,
+                                     as<Rec>(c->rec), as<uint2>(c->rect),
+                                     as<uint32_t>(c->blk_counts), s));
+            HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->chunk_pre), nblk,
+                                     T, as<uint32_t>(c->ranges), dm + it, c->inst_cap, s));
+            HIPCHK(c, launch_emit(as<Rec>(c->rec), as<uint2>(c->rect), as<uint32_t>(c->blk_counts),
+                                  as<uint32_t>(c->chunk_pre), n, v0, vd, as<uint64_t>(c->pairs),
+                                  c->inst_cap, s));
+            HIPCHK(c, launch_tile_sort(as<uint32_t>(c->ranges), T, n, as<uint64_t>(c->pairs),
+                                       as<uint64_t>(c->tmp), as<uint32_t>(c->sorted_idx), nullptr,
+                                       c->lds_limit, s));
+            HIPCHK(c, launch_pose(v0, vd, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
+                                  as<Rec>(c->rec), c->obs_rgb, c->obs_depth, c->gx, c->gy, bp,
+                                  as<double>(c->pose_tiles), nullptr, nullptr, s));
+            HIPCHK(c, launch_pose_reduce(as<double>(c->pose_tiles), T, as<double>(c->pose_sum), s));
+            HIPCHK(c, launch_gn_step(as<double>(c->pose_sum), dpose, vd, dfail, dstatus + it, s));
+        }
+        HIPCHK(c, hipEventRecord(c->track_ev[iters], s));
+        HIPCHK(c, hipMemcpyAsync(c->h_track + off_pose, d + off_pose, bytes - off_pose,
+                                 hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        const uint32_t* hm = reinterpret_cast<const uint32_t*>(c->h_track + off_m);
+        uint32_t mmax = 0;
+        for (int it = 0; it < iters; it++) mmax = std::max(mmax, hm[it]);
+        c->m = hm[iters - 1];
+        if (mmax <= c->inst_cap) break;
+        if (attempt == 1) return fail(c, FVGS_E_STATE, "instance buffers overflowed twice");
+        TRY(ensure_instances(c, (uint64_t)mmax + mmax / 4));   // grow and re-run the solve
+    }
+    c->prepared = c->forwarded = c->backwarded = false;   // buffers hold the last iteration's view
+    std::memcpy(pose7, c->h_track + off_pose, 7 * 8);
+    if (step_ms)
+        for (int it = 0; it < iters; it++) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, c->track_ev[it], c->track_ev[it + 1]);
+            step_ms[it] = ms;
+        }
+    const int* hs = reinterpret_cast<const int*>(c->h_track + off_status);
+    for (int it = 0; it < iters; it++)
+        if (hs[it] == -5)
+            return fail(c, FVGS_E_NOTPD, "Gauss-Newton step failed at iteration %d (JtJ not SPD)",
+                        it);
     return FVGS_OK;
 }
 

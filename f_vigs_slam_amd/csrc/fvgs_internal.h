@@ -52,13 +52,13 @@ struct BlendParams {
 // ---- binning (binning.hip) ----
 hipError_t launch_project(const float* pos, const float* scale, const float* quat,
                           const float* color, const float* opac, int n, const View& v,
-                          int cull_emit, Rec* rec, uint2* rect, uint32_t* blk_counts,
-                          hipStream_t s);
+                          const View* vd, int cull_emit, Rec* rec, uint2* rect,
+                          uint32_t* blk_counts, hipStream_t s);
 hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* chunk_pre, int nblk, int ntiles,
                           uint32_t* ranges, uint32_t* d_m, uint32_t cap, hipStream_t s);
 hipError_t launch_emit(const Rec* rec, const uint2* rect, const uint32_t* blk_counts,
-                       const uint32_t* chunk_pre, int n, const View& v, uint64_t* pairs,
-                       uint64_t cap, hipStream_t s);
+                       const uint32_t* chunk_pre, int n, const View& v, const View* vd,
+                       uint64_t* pairs, uint64_t cap, hipStream_t s);
 hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uint64_t* pairs,
                             uint64_t* tmp, uint32_t* sorted_idx, uint64_t* keys_out,
                             size_t lds_limit, hipStream_t s);
@@ -80,10 +80,14 @@ hipError_t launch_delta3d(const float* pos, const float* scale, const float* qua
                           hipStream_t s);
 
 // ---- tracking (track.hip) ----
-hipError_t launch_pose(const View& v, const uint32_t* ranges, const uint32_t* idx,
-                       const Rec* rec, const float* obs_rgb, const float* obs_depth,
-                       const float* gx, const float* gy, const BlendParams& bp,
-                       double* tiles27, float* out_rgb, float* out_depth, hipStream_t s);
+hipError_t launch_pose(const View& v, const View* vd, const uint32_t* ranges,
+                       const uint32_t* idx, const Rec* rec, const float* obs_rgb,
+                       const float* obs_depth, const float* gx, const float* gy,
+                       const BlendParams& bp, double* tiles27, float* out_rgb, float* out_depth,
+                       hipStream_t s);
+// sum27 = (Jtr[6], JtJ upper[21]); updates pose7 (FP64) and the pose of *vd.
+hipError_t launch_gn_step(const double* sum27, double* pose7, View* vd, int* fail, int* status,
+                          hipStream_t s);
 hipError_t launch_pose_reduce(const double* tiles27, int ntiles, double* out27, hipStream_t s);
 hipError_t launch_sobel(const float* img, int W, int H, float* gx, float* gy, hipStream_t s);
 

@@ -11,6 +11,7 @@
 // tile order. The reference's divergent barriers inside `if (inside)` (Q12) are replaced by a
 // block-uniform batch loop in which every pixel sees the whole list.
 #include "fvgs_internal.h"
+#include "gn_step.h"
 
 namespace fvgs {
 
@@ -64,7 +65,8 @@ __device__ __forceinline__ void accumulate_pose(double acc[27], const float H[9]
         acc[r] += (double)(Jp[r * 3] * b3[0] + Jp[r * 3 + 1] * b3[1] + Jp[r * 3 + 2] * b3[2]);
 }
 
-__global__ __launch_bounds__(256) void k_pose(View v, const uint32_t* __restrict__ ranges,
+__global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ vd,
+                                              const uint32_t* __restrict__ ranges,
                                               const uint32_t* __restrict__ idx,
                                               const Rec* __restrict__ rec,
                                               const float* __restrict__ obs_rgb,
@@ -74,6 +76,7 @@ __global__ __launch_bounds__(256) void k_pose(View v, const uint32_t* __restrict
                                               double* __restrict__ tiles27,
                                               float* __restrict__ out_rgb,
                                               float* __restrict__ out_depth) {
+    const View v = vd ? *vd : vh;   // device-resident view (pose updated on the device) or argument
     __shared__ Rec recs[256];
     __shared__ double red[4][27];
     const int tile = blockIdx.x, tid = threadIdx.x;
@@ -299,6 +302,29 @@ __global__ __launch_bounds__(256) void k_sobel(const float* __restrict__ img, in
     }
 }
 
+// One Gauss-Newton step of the device-resident tracking loop: the 27 FP64 sums (Jtr, then the
+// JtJ upper triangle) -> Cholesky solve + Plus on the FP64 pose, and the float pose of the view
+// that the next iteration's kernels read. After a failed step (JtJ not positive definite) the
+// pose stays as it was and later steps are skipped (*fail is sticky); *status records 0 (step
+// taken), -5 (not positive definite) or 1 (skipped).
+__global__ __launch_bounds__(64) void k_gn_step(const double* __restrict__ sum27,
+                                                double* __restrict__ pose7, View* __restrict__ vd,
+                                                int* __restrict__ fail, int* __restrict__ status) {
+    if (threadIdx.x != 0) return;
+    if (*fail) { *status = 1; return; }
+    double p[7];
+    for (int k = 0; k < 7; ++k) p[k] = pose7[k];
+    if (!gn_solve_plus(sum27 + 6, sum27, p, nullptr)) {
+        *fail = 1;
+        *status = -5;
+        return;
+    }
+    for (int k = 0; k < 7; ++k) pose7[k] = p[k];
+    vd->px = (float)p[0]; vd->py = (float)p[1]; vd->pz = (float)p[2];
+    vd->qx = (float)p[3]; vd->qy = (float)p[4]; vd->qz = (float)p[5]; vd->qw = (float)p[6];
+    *status = 0;
+}
+
 }  // namespace
 
 hipError_t launch_sobel(const float* img, int W, int H, float* gx, float* gy, hipStream_t s) {
@@ -307,12 +333,19 @@ hipError_t launch_sobel(const float* img, int W, int H, float* gx, float* gy, hi
     return hipGetLastError();
 }
 
-hipError_t launch_pose(const View& v, const uint32_t* ranges, const uint32_t* idx,
-                       const Rec* rec, const float* obs_rgb, const float* obs_depth,
-                       const float* gx, const float* gy, const BlendParams& bp, double* tiles27,
-                       float* out_rgb, float* out_depth, hipStream_t s) {
-    hipLaunchKernelGGL(k_pose, dim3(v.ntx * v.nty), dim3(256), 0, s, v, ranges, idx, rec, obs_rgb,
-                       obs_depth, gx, gy, bp, tiles27, out_rgb, out_depth);
+hipError_t launch_pose(const View& v, const View* vd, const uint32_t* ranges,
+                       const uint32_t* idx, const Rec* rec, const float* obs_rgb,
+                       const float* obs_depth, const float* gx, const float* gy,
+                       const BlendParams& bp, double* tiles27, float* out_rgb, float* out_depth,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_pose, dim3(v.ntx * v.nty), dim3(256), 0, s, v, vd, ranges, idx, rec,
+                       obs_rgb, obs_depth, gx, gy, bp, tiles27, out_rgb, out_depth);
+    return hipGetLastError();
+}
+
+hipError_t launch_gn_step(const double* sum27, double* pose7, View* vd, int* fail, int* status,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_gn_step, dim3(1), dim3(64), 0, s, sum27, pose7, vd, fail, status);
     return hipGetLastError();
 }
 

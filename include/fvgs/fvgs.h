@@ -221,7 +221,12 @@ int fvgs_compute_gradients(fvgs_ctx* ctx);
 int fvgs_gn_step(const double jtj21[21], const double jtr6[6], double pose7[7], double delta6[6]);
 
 /* iters Gauss-Newton iterations of fvgs_pose_jacobians + fvgs_gn_step from pose7 (updated in
- * place). step_ms (may be NULL) receives the wall time of each iteration in milliseconds. */
+ * place), run as one device-resident loop: after a synchronous binning at the start pose, every
+ * iteration's projection, binning, Jacobians, FP64 reduction and Gauss-Newton step are enqueued
+ * back to back and the step writes the next iteration's pose in device memory; the host waits
+ * once at the end. Results equal the host-driven loop's bit for bit. step_ms (may be NULL)
+ * receives the device time of each iteration in milliseconds (HIP events). On a failed step the
+ * pose is that of the last successful one and FVGS_E_NOTPD is returned. */
 int fvgs_track(fvgs_ctx* ctx, const fvgs_intrinsics* intr, int iters, double pose7[7],
                double* step_ms);
 

@@ -296,6 +296,22 @@ def test_tracking_gauss_newton_vs_oracle(ctx, track_case):
     assert rot <= 1e-6 and tr <= 1e-6
 
 
+def test_device_resident_track_equals_host_driven_loop(ctx, track_case):
+    """fvgs_track runs projection, binning, Jacobians, reduction and the FP64 step on the device;
+    the host-driven loop (fvgs_pose_jacobians + fvgs_gn_step per iteration) gives the same bits."""
+    c = track_case
+    ctx.set_scene(c["scene"])
+    ctx.set_frame(c["obs_rgb"], c["obs_depth"], c["gx"], c["gy"])
+    pose_d, ms = ctx.track(c["intr"], 6, c["start"])
+    pose_h = c["start"].copy()
+    for _ in range(6):
+        jtj, jtr = ctx.pose_jacobians(pose_h, c["intr"])
+        pose_h, _ = gn_step(jtj, jtr, pose_h)
+    print(f"[track] device loop {np.mean(ms):.3f} ms/step; max |diff| "
+          f"{np.abs(pose_d - pose_h).max():.3g}", flush=True)
+    np.testing.assert_array_equal(pose_d, pose_h)
+
+
 def test_gn_step_matches_oracle():
     rng = np.random.default_rng(5)
     A = rng.standard_normal((6, 6))
