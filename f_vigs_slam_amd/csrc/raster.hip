@@ -1,4 +1,4 @@
-// Tile blending for gfx950: render-only forward, fused mapping forward + backward, 2D->3D.
+// Tile blending for gfx950: render-only forward, mapping forward and backward, 2D->3D.
 //
 // Replaces forwardPassTileKernel (GSCudaKernels.cu:710-859), optimizeGaussiansForwardPass
 // (2092-2263), optimizeGaussiansPerGaussianPass (2265-2502) and computeDeltaGaussians3D_kernel
@@ -10,12 +10,12 @@
 // pipeline.
 // Reference defects Q8, Q9, Q10 and Q14 are transcribed (see the oracle header).
 #include "fvgs_internal.h"
+#include "blend_common.h"
 
 namespace fvgs {
 
 namespace {
 
-constexpr float kAlphaMin = 1.0f / 255.0f;
 
 // Lane l receives lane l-1's value across the whole wave64 (DPP wave_shr:1, one VALU op);
 // lane 0 keeps its own value, as __shfl_up(v, 1) does.
@@ -83,39 +83,6 @@ __global__ __launch_bounds__(256) void k_render(View v, const uint32_t* __restri
 }
 
 // ---- A14 ------------------------------------------------------------------------------------
-// Wave-level culling for the blend walks. A wave owns an 8x8 pixel quadrant of the tile; an entry
-// can change a pixel's state only where alpha >= 1/255, i.e. q <= 2 ln(255 opacity), and (in the
-// forward) q > 0. Interval arithmetic bounds q = c0 dx^2 + 2 c1 dx dy + c2 dy^2 over the quadrant's
-// dx, dy ranges; an entry whose bounds exclude every pixel of a quadrant (with a relative slack
-// far above FP32 rounding of either evaluation) is skipped by that wave. Skipped entries leave
-// every pixel of the wave untouched, so results are those of the full walk. NaN bounds keep the
-// entry. Returns bit w set for each quadrant w whose wave must evaluate the entry.
-__device__ __forceinline__ uint32_t quadrant_mask(const Rec& r, int tx0, int ty0, bool need_pos_q) {
-    const float u = r.r0.x, vv = r.r0.y, c0 = r.r0.z, c1x2 = 2.0f * r.r0.w, c2 = r.r1.x;
-    const float qcut = 2.0f * __logf(255.0f * r.r2.w) + 1e-3f;
-    uint32_t mask = 0;
-#pragma unroll 1
-    for (int w = 0; w < 4; ++w) {
-        const float ax0 = (float)(tx0 + 8 * (w & 1)) - u, ax1 = (float)(tx0 + 8 * (w & 1) + 7) - u;
-        const float ay0 = (float)(ty0 + 8 * (w >> 1)) - vv, ay1 = (float)(ty0 + 8 * (w >> 1) + 7) - vv;
-        const float xa = ax0 * ax0, xb = ax1 * ax1, ya = ay0 * ay0, yb = ay1 * ay1;
-        const float xxlo = (ax0 <= 0.0f && ax1 >= 0.0f) ? 0.0f : fminf(xa, xb), xxhi = fmaxf(xa, xb);
-        const float yylo = (ay0 <= 0.0f && ay1 >= 0.0f) ? 0.0f : fminf(ya, yb), yyhi = fmaxf(ya, yb);
-        const float p0 = ax0 * ay0, p1 = ax0 * ay1, p2 = ax1 * ay0, p3 = ax1 * ay1;
-        const float xylo = fminf(fminf(p0, p1), fminf(p2, p3)), xyhi = fmaxf(fmaxf(p0, p1), fmaxf(p2, p3));
-        const float t0lo = c0 >= 0.0f ? c0 * xxlo : c0 * xxhi, t0hi = c0 >= 0.0f ? c0 * xxhi : c0 * xxlo;
-        const float t1lo = c1x2 >= 0.0f ? c1x2 * xylo : c1x2 * xyhi,
-                    t1hi = c1x2 >= 0.0f ? c1x2 * xyhi : c1x2 * xylo;
-        const float t2lo = c2 >= 0.0f ? c2 * yylo : c2 * yyhi, t2hi = c2 >= 0.0f ? c2 * yyhi : c2 * yylo;
-        const float qlo = t0lo + t1lo + t2lo, qhi = t0hi + t1hi + t2hi;
-        const float slack = 1e-5f * (fabsf(t0lo) + fabsf(t0hi) + fabsf(t1lo) + fabsf(t1hi) +
-                                     fabsf(t2lo) + fabsf(t2hi)) + 1e-30f;
-        const bool skip = (need_pos_q && qhi + slack <= 0.0f) || (qlo - slack > qcut);
-        if (!skip) mask |= 1u << w;
-    }
-    return mask;
-}
-
 // One workgroup per tile, one wave per 8x8 quadrant, one thread per pixel. Each batch of 256
 // records is staged in LDS (16 KB) with its quadrant masks; each wave walks the compacted list of
 // the entries its quadrant needs. The (T, colour) checkpoints at list indices 0, 32, ..., 224 go

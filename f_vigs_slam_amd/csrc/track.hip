@@ -12,12 +12,12 @@
 // block-uniform batch loop in which every pixel sees the whole list.
 #include "fvgs_internal.h"
 #include "gn_step.h"
+#include "blend_common.h"
 
 namespace fvgs {
 
 namespace {
 
-constexpr float kAlphaMin = 1.0f / 255.0f;
 
 __device__ __forceinline__ void qmat_t(float x, float y, float z, float w, float R[3][3]) {
     R[0][0] = 1.0f - 2.0f * (y * y + z * z);
@@ -31,40 +31,81 @@ __device__ __forceinline__ void qmat_t(float x, float y, float z, float w, float
     R[2][2] = 1.0f - 2.0f * (x * x + y * y);
 }
 
-// acc[0..5] += Jpose * b ; acc[6..26] += upper(Jpose * H * Jpose^T) with
-// Jpose = [R ; d * [ray]x] (6x3) (GSCudaKernels.cu:1116-1165, 1293-1336). Each term is formed in
-// FP32 as in the reference; the running sums are FP64: JtJ sums terms spanning ~10 orders of
-// magnitude (pass-2 terms carry 1/(1 - alpha)), and FP32 running sums lose positive
-// definiteness.
-__device__ __forceinline__ void accumulate_pose(double acc[27], const float H[9], const float b3[3],
-                                                const float R[3][3], float d, const float ray[3]) {
-    float Jp[18];
+// Per-pixel pose sums. The reference adds, per pixel term, Jtr += Jp b and JtJ += Jp H Jp^T
+// with Jp = [R ; d [ray]x] (6x3) (GSCudaKernels.cu:1116-1165, 1293-1336). R is the same for the
+// whole view, so the blocks factor as
+//   JtJ = [[R S_H R^T, R S_HK], [., S_KHK]],  Jtr = [R S_b ; S_Kb]
+// with S_H = sum H, S_HK = sum d H K^T, S_KHK = sum d^2 K H K^T, S_b = sum b, S_Kb = sum d K b and
+// K = [ray]x (ray z = 1). Each term is formed in FP32 from the reference's H and b; the 27 running
+// sums are FP64 (JtJ adds terms spanning ~10 orders of magnitude, FP32 sums lose positive
+// definiteness); R is applied once per tile, in FP64.
+// raw layout: S_H (00 01 02 11 12 22) | S_HK (3x3 row-major) | S_KHK (00 01 02 11 12 22) | S_b | S_Kb
+__device__ __forceinline__ void accumulate_raw(double acc[27], const float H[9], const float b3[3],
+                                               float d, float rx, float ry) {
+    acc[0] += (double)H[0]; acc[1] += (double)H[1]; acc[2] += (double)H[2];
+    acc[3] += (double)H[4]; acc[4] += (double)H[5]; acc[5] += (double)H[8];
+    // M = H K^T: K rows (0, -1, ry), (1, 0, -rx), (-ry, rx, 0)
+    float M[9];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+    for (int i = 0; i < 3; ++i) {
+        M[i * 3 + 0] = ry * H[i * 3 + 2] - H[i * 3 + 1];
+        M[i * 3 + 1] = H[i * 3 + 0] - rx * H[i * 3 + 2];
+        M[i * 3 + 2] = rx * H[i * 3 + 1] - ry * H[i * 3 + 0];
+    }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) Jp[r * 3 + k] = R[r][k];
-    // d * [ray]x = d * [[0, -rz, ry], [rz, 0, -rx], [-ry, rx, 0]]
-    Jp[9] = 0.f;           Jp[10] = -d * ray[2];  Jp[11] = d * ray[1];
-    Jp[12] = d * ray[2];   Jp[13] = 0.f;          Jp[14] = -d * ray[0];
-    Jp[15] = -d * ray[1];  Jp[16] = d * ray[0];   Jp[17] = 0.f;
-    float tmp[18];
-#pragma unroll
-    for (int r = 0; r < 6; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            tmp[r * 3 + c] = Jp[r * 3] * H[c] + Jp[r * 3 + 1] * H[3 + c] + Jp[r * 3 + 2] * H[6 + c];
-    int id = 6;
-#pragma unroll
-    for (int r = 0; r < 6; ++r)
-#pragma unroll
-        for (int c = r; c < 6; ++c)
-            acc[id++] += (double)(tmp[r * 3] * Jp[c * 3] + tmp[r * 3 + 1] * Jp[c * 3 + 1] +
-                                  tmp[r * 3 + 2] * Jp[c * 3 + 2]);
-#pragma unroll
-    for (int r = 0; r < 6; ++r)
-        acc[r] += (double)(Jp[r * 3] * b3[0] + Jp[r * 3 + 1] * b3[1] + Jp[r * 3 + 2] * b3[2]);
+    for (int k = 0; k < 9; ++k) acc[6 + k] += (double)(d * M[k]);
+    // K M (symmetric): row 0 = -M[1][:] + ry M[2][:], row 1 = M[0][:] - rx M[2][:],
+    // row 2 = -ry M[0][:] + rx M[1][:]
+    const float d2 = d * d;
+    const float k00 = ry * M[6] - M[3], k01 = ry * M[7] - M[4], k02 = ry * M[8] - M[5];
+    const float k11 = M[1] - rx * M[7], k12 = M[2] - rx * M[8];
+    const float k22 = rx * M[5] - ry * M[2];
+    acc[15] += (double)(d2 * k00); acc[16] += (double)(d2 * k01); acc[17] += (double)(d2 * k02);
+    acc[18] += (double)(d2 * k11); acc[19] += (double)(d2 * k12); acc[20] += (double)(d2 * k22);
+    acc[21] += (double)b3[0]; acc[22] += (double)b3[1]; acc[23] += (double)b3[2];
+    acc[24] += (double)(d * (ry * b3[2] - b3[1]));
+    acc[25] += (double)(d * (b3[0] - rx * b3[2]));
+    acc[26] += (double)(d * (rx * b3[1] - ry * b3[0]));
 }
 
+// Raw sums -> (Jtr[6], JtJ upper[21]) with the view's R (FP64).
+__device__ __forceinline__ void raw_to_pose(const double* raw, const float Rf[3][3], double* out) {
+    double R[3][3], SH[3][3], SHK[3][3], SKHK[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) R[i][j] = (double)Rf[i][j];
+    SH[0][0] = raw[0]; SH[0][1] = SH[1][0] = raw[1]; SH[0][2] = SH[2][0] = raw[2];
+    SH[1][1] = raw[3]; SH[1][2] = SH[2][1] = raw[4]; SH[2][2] = raw[5];
+    for (int k = 0; k < 9; ++k) SHK[k / 3][k % 3] = raw[6 + k];
+    SKHK[0][0] = raw[15]; SKHK[0][1] = SKHK[1][0] = raw[16]; SKHK[0][2] = SKHK[2][0] = raw[17];
+    SKHK[1][1] = raw[18]; SKHK[1][2] = SKHK[2][1] = raw[19]; SKHK[2][2] = raw[20];
+    double J[6][6], RS[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            RS[i][j] = R[i][0] * SH[0][j] + R[i][1] * SH[1][j] + R[i][2] * SH[2][j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            J[i][j] = RS[i][0] * R[j][0] + RS[i][1] * R[j][1] + RS[i][2] * R[j][2];
+            J[i][3 + j] = R[i][0] * SHK[0][j] + R[i][1] * SHK[1][j] + R[i][2] * SHK[2][j];
+            J[3 + i][3 + j] = SKHK[i][j];
+        }
+    for (int i = 0; i < 3; ++i) {
+        out[i] = R[i][0] * raw[21] + R[i][1] * raw[22] + R[i][2] * raw[23];
+        out[3 + i] = raw[24 + i];
+    }
+    int id = 6;
+    for (int r = 0; r < 6; ++r)
+        for (int c = r; c < 6; ++c) out[id++] = J[r][c];
+}
+
+// One workgroup per 16x16 tile, one wave per 8x8 quadrant, one thread per pixel.
+//   pass 1 re-renders the pixel (GSCudaKernels.cu:978-1036);
+//   the pixel term (1044-1168) uses the image gradients at the pixel;
+//   pass 2 walks the list again for the per-Gaussian terms (1174-1348).
+// Both walks use the per-wave compacted entry lists of the blend kernels (pass 1: q > 0 and
+// alpha >= 1/255; pass 2: alpha >= 1/255 only, Q8). Every pixel sees the whole list (Q7/Q12).
+// The tile's partial is a wave butterfly of the raw FP64 sums plus a fixed-order sum of the four
+// waves, transformed with R and written once per tile (no atomics, deterministic).
+// Render-only mode (obs_depth == nullptr): pass 1 for every in-bounds pixel, no Jacobians.
 __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ vd,
                                               const uint32_t* __restrict__ ranges,
                                               const uint32_t* __restrict__ idx,
@@ -78,51 +119,69 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
                                               float* __restrict__ out_depth) {
     const View v = vd ? *vd : vh;   // device-resident view (pose updated on the device) or argument
     __shared__ Rec recs[256];
+    __shared__ uint8_t qmask[256];
+    __shared__ uint8_t wlist[4][256];
     __shared__ double red[4][27];
-    const int tile = blockIdx.x, tid = threadIdx.x;
-    const int x = (tile % v.ntx) * kTile + (tid & 15), y = (tile / v.ntx) * kTile + (tid >> 4);
+    const int tile = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int tx0 = (tile % v.ntx) * kTile, ty0 = (tile / v.ntx) * kTile;
+    const int x = tx0 + 8 * (w & 1) + (lane & 7), y = ty0 + 8 * (w >> 1) + (lane >> 3);
     const bool in_bounds = x < v.W && y < v.H;
     const size_t p = in_bounds ? (size_t)y * v.W + x : 0;
-    // Render-only mode (obs_depth == nullptr): every in-bounds pixel renders; no Jacobians.
     const bool render_only = obs_depth == nullptr;
     const float img_depth = in_bounds ? (render_only ? 1.0f : obs_depth[p]) : 0.0f;
     bool inside = in_bounds && img_depth > 0.1f;
     const uint32_t s = ranges[2 * tile], n = ranges[2 * tile + 1] - s;
     const float px = (float)x, py = (float)y;
 
-    double acc[27];
+    // Stages batch `base` in LDS and builds this wave's list of entries whose quadrant bit
+    // (pass 1: low nibble, pass 2: high nibble) is set. Returns the list length (uniform).
+    auto stage = [&](uint32_t base, int pass) -> int {
+        const int cnt = (int)min(256u, n - base);
+        if (tid < cnt) {
+            const Rec r = rec[idx[s + base + tid]];
+            recs[tid] = r;
+            qmask[tid] = (uint8_t)quadrant_mask(r, tx0, ty0, pass == 1);
+        }
+        __syncthreads();
+        int wc = 0;
 #pragma unroll
-    for (int k = 0; k < 27; ++k) acc[k] = 0.0;
+        for (int r4 = 0; r4 < 4; ++r4) {
+            const int e = r4 * 64 + lane;
+            const bool bit = e < cnt && ((qmask[e] >> w) & 1u);
+            const uint64_t bal = __ballot(bit);
+            if (bit) wlist[w][wc + __popcll(bal & ((1ull << lane) - 1ull))] = (uint8_t)e;
+            wc += __popcll(bal);
+        }
+        __syncthreads();
+        return wc;
+    };
 
-    // ---------------- pass 1: render (GSCudaKernels.cu:978-1036) ----------------
+    // ---------------- pass 1: render ----------------
     float C0 = 0.f, C1 = 0.f, C2 = 0.f, D = 0.f, final_T = 1.0f;
     uint32_t n_contrib = 0;
-    if (n > 0) {
+    {
         float T = 1.f;
-        uint32_t contributor = 0, last = 0;
+        uint32_t last = 0;
         bool done = !inside;
         for (uint32_t base = 0; base < n; base += 256) {
             if (__syncthreads_count(!done) == 0) break;
-            if (base + tid < n) recs[tid] = rec[idx[s + base + tid]];
-            __syncthreads();
-            const int cnt = (int)min(256u, n - base);
-            if (!done) {
-                for (int i = 0; i < cnt; ++i) {
-                    ++contributor;
-                    const float4 r0 = recs[i].r0, r1 = recs[i].r1;
-                    const float dx = r0.x - px, dy = r0.y - py;
-                    const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;  // Q14
-                    const float4 r2 = recs[i].r2;
-                    const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));
-                    if (a < kAlphaMin || q <= 0.0f) continue;
-                    const float tT = T * (1.0f - a);
-                    if (tT < 0.0001f) { done = true; break; }
-                    C0 += r2.x * a * T; C1 += r2.y * a * T; C2 += r2.z * a * T;
-                    const float d = r1.y + dx * r1.z + dy * r1.w;
-                    if (T > 0.5f && tT < 0.5f) D = d;
-                    T = tT;
-                    last = contributor;
-                }
+            const int wc = stage(base, 1);
+            if (done) continue;
+            for (int k = 0; k < wc; ++k) {
+                const int i = wlist[w][k];
+                const float4 r0 = recs[i].r0, r1 = recs[i].r1;
+                const float dx = r0.x - px, dy = r0.y - py;
+                const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;  // Q14
+                const float4 r2 = recs[i].r2;
+                const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));
+                if (a < kAlphaMin || q <= 0.0f) continue;
+                const float tT = T * (1.0f - a);
+                if (tT < 0.0001f) { done = true; break; }
+                C0 += r2.x * a * T; C1 += r2.y * a * T; C2 += r2.z * a * T;
+                const float d = r1.y + dx * r1.z + dy * r1.w;
+                if (T > 0.5f && tT < 0.5f) D = d;
+                T = tT;
+                last = base + (uint32_t)i + 1;
             }
         }
         if (inside) {
@@ -138,15 +197,16 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
     if (render_only) return;   // uniform over the block
     inside = inside && (final_T < bp.alpha_thresh);
 
-    float R[3][3];
-    qmat_t(v.qx, v.qy, v.qz, v.qw, R);
-    float e0 = 0.f, e1 = 0.f, e2 = 0.f, de = 0.f, wc = 0.f;
+    double acc[27];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) acc[k] = 0.0;
+    float e0 = 0.f, e1 = 0.f, e2 = 0.f, de = 0.f, wc_col = 0.f;
     if (inside) {
-        // ---------------- pixel term (GSCudaKernels.cu:1044-1168) ----------------
+        // ---------------- pixel term ----------------
         e0 = C0 - obs_rgb[3 * p]; e1 = C1 - obs_rgb[3 * p + 1]; e2 = C2 - obs_rgb[3 * p + 2];
         de = (img_depth > 0.1f) ? (D - img_depth) : 0.0f;
         const float cl = sqrtf(e0 * e0 + e1 * e1 + e2 * e2);
-        wc = (cl < bp.color_thresh) ? 1.0f : bp.color_thresh / cl;
+        wc_col = (cl < bp.color_thresh) ? 1.0f : bp.color_thresh / cl;
         const float ray[3] = {(px - v.cx) / v.fx, (py - v.cy) / v.fy, 1.0f};
         const float gX[3] = {gxi[3 * p], gxi[3 * p + 1], gxi[3 * p + 2]};
         const float gY[3] = {gyi[3 * p], gyi[3 * p + 1], gyi[3 * p + 2]};
@@ -163,10 +223,11 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                H[r * 3 + c] = wc * (A[r * 3] * A[c * 3] + A[r * 3 + 1] * A[c * 3 + 1] +
-                                     A[r * 3 + 2] * A[c * 3 + 2]);
+                H[r * 3 + c] = wc_col * (A[r * 3] * A[c * 3] + A[r * 3 + 1] * A[c * 3 + 1] +
+                                         A[r * 3 + 2] * A[c * 3 + 2]);
 #pragma unroll
-        for (int r = 0; r < 3; ++r) b3[r] = -wc * (A[r * 3] * e0 + A[r * 3 + 1] * e1 + A[r * 3 + 2] * e2);
+        for (int r = 0; r < 3; ++r)
+            b3[r] = -wc_col * (A[r * 3] * e0 + A[r * 3 + 1] * e1 + A[r * 3 + 2] * e2);
         const float ld = fabsf(de);
         float wd = (ld < bp.depth_thresh) ? 1.0f : bp.depth_thresh / ld;
         wd /= img_depth;
@@ -177,77 +238,79 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
             for (int c = 0; c < 3; ++c) H[r * 3 + c] += wd * ray[r] * ray[c];
             b3[r] += wd * de * ray[r];
         }
-        accumulate_pose(acc, H, b3, R, img_depth, ray);
+        accumulate_raw(acc, H, b3, img_depth, ray[0], ray[1]);
     }
 
-    // ---------------- pass 2: per-Gaussian terms (GSCudaKernels.cu:1174-1348) ----------------
-    float prod = 1.f, T2 = 1.f, a0 = 0.f, a1 = 0.f, a2 = 0.f;
-    bool active = inside && n_contrib > 0;
-    for (uint32_t base = 0; base < n; base += 256) {
-        const bool want = active && base < n_contrib;
-        if (__syncthreads_count(want) == 0) break;
-        if (base + tid < n) recs[tid] = rec[idx[s + base + tid]];
-        __syncthreads();
-        if (!want) continue;
-        const int cnt = (int)min(256u, n - base);
-        for (int i = 0; i < cnt && base + i < n_contrib; ++i) {
-            const float4 r0 = recs[i].r0, r1 = recs[i].r1, r2 = recs[i].r2;
-            const float dx = r0.x - px, dy = r0.y - py;
-            const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;
-            const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));   // Q8: no q <= 0 test
-            // Q15: a culled Gaussian (zero record, depth 0) would divide by zero depth below.
-            if (a < kAlphaMin || r1.y == 0.0f) continue;
-            const float d = r1.y + dx * r1.z + dy * r1.w;
-            float da0 = r2.x * prod, da1 = r2.y * prod, da2 = r2.z * prod;
-            a0 += a * da0; a1 += a * da1; a2 += a * da2;
-            const float iom = 1.0f / (1.0f - a);
-            da0 -= (C0 - a0) * iom;
-            da1 -= (C1 - a1) * iom;
-            da2 -= (C2 - a2) * iom;
-            const float dm0 = r0.z * dx + r0.w * dy, dm1 = r0.w * dx + r1.x * dy;
-            const float rg[3] = {(r0.x - v.cx) / v.fx, (r0.y - v.cy) / v.fy, 1.0f};
-            const float id_ = 1.0f / d;
-            const float Jt0 = v.fx * id_, Jt3 = v.fy * id_, Jt4 = -v.fx * rg[0] * id_,
-                        Jt5 = -v.fy * rg[1] * id_;
-            const float jd[3] = {Jt0 * dm0, Jt3 * dm1, Jt4 * dm0 + Jt5 * dm1};
-            float Jc[9];
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                Jc[r * 3] = jd[r] * a * da0;
-                Jc[r * 3 + 1] = jd[r] * a * da1;
-                Jc[r * 3 + 2] = jd[r] * a * da2;
-            }
-            float H[9], b3[3];
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    H[r * 3 + c] = wc * (Jc[r * 3] * Jc[c * 3] + Jc[r * 3 + 1] * Jc[c * 3 + 1] +
-                                         Jc[r * 3 + 2] * Jc[c * 3 + 2]);
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-                b3[r] = -wc * (Jc[r * 3] * e0 + Jc[r * 3 + 1] * e1 + Jc[r * 3 + 2] * e2);
-            prod *= (1.0f - a);
-            if (T2 > 0.5f && prod <= 0.5f && img_depth > 0.5f) {
-                T2 = prod;
-                const float ld = fabsf(de);
-                float wd = (ld < bp.depth_thresh) ? 1.0f : bp.depth_thresh / ld;
-                wd /= img_depth;
-                wd *= bp.depth_weight;
+    // ---------------- pass 2: per-Gaussian terms ----------------
+    {
+        float prod = 1.f, T2 = 1.f, a0 = 0.f, a1 = 0.f, a2 = 0.f;
+        bool active = inside && n_contrib > 0;
+        for (uint32_t base = 0; base < n; base += 256) {
+            const bool want = active && base < n_contrib;
+            if (__syncthreads_count(want) == 0) break;
+            const int wc = stage(base, 2);
+            if (!want) continue;
+            for (int k = 0; k < wc; ++k) {
+                const int i = wlist[w][k];
+                if (base + (uint32_t)i >= n_contrib) break;
+                const float4 r0 = recs[i].r0, r1 = recs[i].r1, r2 = recs[i].r2;
+                const float dx = r0.x - px, dy = r0.y - py;
+                const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;
+                const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));   // Q8: no q <= 0 test
+                // Q15: a culled Gaussian (zero record, depth 0) would divide by zero depth below.
+                if (a < kAlphaMin || r1.y == 0.0f) continue;
+                const float d = r1.y + dx * r1.z + dy * r1.w;
+                float da0 = r2.x * prod, da1 = r2.y * prod, da2 = r2.z * prod;
+                a0 += a * da0; a1 += a * da1; a2 += a * da2;
+                const float iom = 1.0f / (1.0f - a);
+                da0 -= (C0 - a0) * iom;
+                da1 -= (C1 - a1) * iom;
+                da2 -= (C2 - a2) * iom;
+                const float dm0 = r0.z * dx + r0.w * dy, dm1 = r0.w * dx + r1.x * dy;
+                const float rgx = (r0.x - v.cx) / v.fx, rgy = (r0.y - v.cy) / v.fy;
+                const float id_ = 1.0f / d;
+                const float Jt0 = v.fx * id_, Jt3 = v.fy * id_, Jt4 = -v.fx * rgx * id_,
+                            Jt5 = -v.fy * rgy * id_;
+                const float jd[3] = {Jt0 * dm0, Jt3 * dm1, Jt4 * dm0 + Jt5 * dm1};
+                float Jc[9];
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) H[r * 3 + c] += wd * rg[r] * rg[c];
-                    b3[r] += wd * de * rg[r];
+                    Jc[r * 3] = jd[r] * a * da0;
+                    Jc[r * 3 + 1] = jd[r] * a * da1;
+                    Jc[r * 3 + 2] = jd[r] * a * da2;
                 }
+                float H[9], b3[3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        H[r * 3 + c] = wc_col * (Jc[r * 3] * Jc[c * 3] + Jc[r * 3 + 1] * Jc[c * 3 + 1] +
+                                                 Jc[r * 3 + 2] * Jc[c * 3 + 2]);
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+                    b3[r] = -wc_col * (Jc[r * 3] * e0 + Jc[r * 3 + 1] * e1 + Jc[r * 3 + 2] * e2);
+                prod *= (1.0f - a);
+                if (T2 > 0.5f && prod <= 0.5f && img_depth > 0.5f) {
+                    T2 = prod;
+                    const float ld = fabsf(de);
+                    float wd = (ld < bp.depth_thresh) ? 1.0f : bp.depth_thresh / ld;
+                    wd /= img_depth;
+                    wd *= bp.depth_weight;
+                    const float rg[3] = {rgx, rgy, 1.0f};
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) H[r * 3 + c] += wd * rg[r] * rg[c];
+                        b3[r] += wd * de * rg[r];
+                    }
+                }
+                accumulate_raw(acc, H, b3, d, rgx, rgy);
+                if (prod < 0.001f) { active = false; break; }
             }
-            accumulate_pose(acc, H, b3, R, d, rg);
-            if (prod < 0.001f) { active = false; break; }
         }
     }
 
     // ---------------- tile reduction (fixed order) ----------------
-    const int w = tid >> 6, lane = tid & 63;
 #pragma unroll
     for (int k = 0; k < 27; ++k) {
         double vk = acc[k];
@@ -256,7 +319,18 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
         if (lane == 0) red[w][k] = vk;
     }
     __syncthreads();
-    if (tid < 27) tiles27[(size_t)tile * 27 + tid] = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+    if (tid < 27) {
+        const double t = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+        red[0][tid] = t;   // each thread reads and writes only its own column
+    }
+    __syncthreads();
+    if (tid == 0) {
+        float R[3][3];
+        qmat_t(v.qx, v.qy, v.qz, v.qw, R);
+        double out[27];
+        raw_to_pose(red[0], R, out);
+        for (int k = 0; k < 27; ++k) tiles27[(size_t)tile * 27 + k] = out[k];
+    }
 }
 
 // FP64 sum of the per-tile partials: block k sums component k over tiles in a fixed order.
