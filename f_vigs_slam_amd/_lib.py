@@ -35,6 +35,11 @@ EXPORTS = [
     "fvgs_render", "fvgs_forward", "fvgs_backward", "fvgs_gradients_3d", "fvgs_mapping_step",
     "fvgs_pose_jacobians", "fvgs_render_tracking", "fvgs_compute_gradients", "fvgs_gn_step", "fvgs_track", "fvgs_read", "fvgs_device_ptr",
     "fvgs_sync", "fvgs_stats", "fvgs_set_timing", "fvgs_last_timing",
+    # fvgs_server.h
+    "fvgs_server_default_params", "fvgs_server_create", "fvgs_server_destroy",
+    "fvgs_server_last_error", "fvgs_server_open_session", "fvgs_server_close_session",
+    "fvgs_server_add_keyframe", "fvgs_server_submit_track", "fvgs_server_submit_mapping",
+    "fvgs_server_wait", "fvgs_server_read_gradients", "fvgs_server_memory",
 ]
 
 
@@ -53,6 +58,18 @@ class Pose(ctypes.Structure):
 
 class Intrinsics(ctypes.Structure):
     _fields_ = [("f", ctypes.c_float * 2), ("c", ctypes.c_float * 2)]
+
+
+class ServerParams(ctypes.Structure):
+    _fields_ = [("lanes", ctypes.c_int), ("max_sessions", ctypes.c_int),
+                ("hbm_budget", ctypes.c_size_t), ("params", Params)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("kind", ctypes.c_int), ("session", ctypes.c_int),
+                ("iters", ctypes.c_int), ("pose7", ctypes.c_double * 7),
+                ("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double),
+                ("device_ms", ctypes.c_double)]
 
 
 DELTA2D_DTYPE = np.dtype([("meanImg", "<f4", 2), ("invSigmaImg", "<f4", 3), ("color", "<f4", 3),
@@ -101,6 +118,20 @@ def load():
         "fvgs_stats": ([vp, P(ctypes.c_uint64), P(ctypes.c_uint64), P(u32), P(u32)], i),
         "fvgs_set_timing": ([vp, i], i),
         "fvgs_last_timing": ([vp, P(f)], i),
+        "fvgs_server_default_params": ([P(ServerParams)], i),
+        "fvgs_server_create": ([P(ServerParams), P(vp)], i),
+        "fvgs_server_destroy": ([vp], i),
+        "fvgs_server_last_error": ([vp], ctypes.c_char_p),
+        "fvgs_server_open_session": ([vp, u32, vp, vp, vp, vp, vp, i, i, i, P(i)], i),
+        "fvgs_server_close_session": ([vp, i], i),
+        "fvgs_server_add_keyframe": ([vp, i, P(Pose), i, i, vp, vp, i, P(i)], i),
+        "fvgs_server_submit_track": ([vp, i, i, i, vp, vp, i, P(Intrinsics), i, vp,
+                                      P(ctypes.c_uint64)], i),
+        "fvgs_server_submit_mapping": ([vp, i, P(Intrinsics), i, vp, P(ctypes.c_uint64)], i),
+        "fvgs_server_wait": ([vp, ctypes.c_uint64, P(RequestResult)], i),
+        "fvgs_server_read_gradients": ([vp, i, vp, ctypes.c_size_t], i),
+        "fvgs_server_memory": ([vp, P(ctypes.c_size_t), P(ctypes.c_size_t),
+                                P(ctypes.c_size_t)], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -79,6 +79,8 @@ hipError_t launch_delta3d(const float* pos, const float* scale, const float* qua
                           const View& v, float lambda_iso, const Delta2D* d2, Delta3D* d3,
                           hipStream_t s);
 
+hipError_t launch_accum_delta3d(const Delta3D* src, Delta3D* dst, int n, hipStream_t s);
+
 // ---- tracking (track.hip) ----
 hipError_t launch_pose(const View& v, const View* vd, const uint32_t* ranges,
                        const uint32_t* idx, const Rec* rec, const float* obs_rgb,

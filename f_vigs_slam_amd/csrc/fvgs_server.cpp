@@ -1,0 +1,550 @@
+// fvgs_server (include/fvgs/fvgs_server.h): sessions with device-resident maps, admission by a
+// device-memory budget, and continuous batching of their requests over concurrent lanes.
+//
+// A lane is a worker thread that owns a fvgs_ctx (its own HIP stream and workspace) plus a
+// stream of its own for server-side kernels. Requests wait in one FIFO queue; a free lane takes
+// the oldest request it may run (a session's mapping requests run one at a time, since they add
+// into the session's gradient accumulator). Lanes run concurrently, so requests of different
+// sessions overlap on the device. Results do not depend on the mix: every kernel of a tracking
+// request is deterministic and reads only the request's own session and lane buffers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/fvgs/fvgs.h"
+#include "../../include/fvgs/fvgs_server.h"
+#include "fvgs_internal.h"
+
+using namespace fvgs;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Keyframe {
+    fvgs_pose pose;
+    int w = 0, h = 0;
+    DevMem rgb, depth;
+};
+
+struct Session {
+    int id = 0;
+    uint32_t n = 0;
+    int frame_w = 0, frame_h = 0;
+    DevMem pos, scale, quat, color, opac, grad;
+    std::vector<Keyframe> keyframes;
+    size_t bytes = 0;
+    int in_flight = 0;        // queued + running requests
+    bool mapping_busy = false;
+    bool closing = false;
+};
+
+struct Request {
+    uint64_t ticket = 0;
+    int kind = 0;
+    int session = 0;
+    // tracking
+    int w = 0, h = 0, mem = FVGS_MEM_HOST, iters = 0;
+    const float* rgb = nullptr;
+    const float* depth = nullptr;
+    double pose7[7] = {0, 0, 0, 0, 0, 0, 1};
+    // mapping
+    std::vector<int> keyframes;
+    fvgs_intrinsics intr{};
+    // result
+    bool done = false;
+    fvgs_request_result res{};
+    Clock::time_point t_submit, t_start, t_end;
+};
+
+// Workspace a lane needs for a session of n Gaussians and p pixels (bytes, with the 25 % growth
+// slack of the context's buffers): screen records + rectangles + 2D/3D gradients, up to 8 tile
+// instances per Gaussian (pairs, sort buffer, indices), per-pixel outputs, frame copies and
+// gradient images, per-tile checkpoints.
+size_t lane_workspace(uint64_t n, uint64_t p) {
+    const uint64_t tiles = (p + 255) / 256 + 64;
+    const uint64_t m = 8 * n;
+    const uint64_t b = (64 + 8 + 52 + 56) * n + (8 + 8 + 4) * m + (36 + 32 + 16 + 40) * p +
+                       (8 * 256 * 16 + 256) * tiles + (1u << 20);
+    return (size_t)(b + b / 4);
+}
+
+}  // namespace
+
+struct fvgs_server {
+    fvgs_server_params prm{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> queue;
+    std::map<uint64_t, std::unique_ptr<Request>> requests;
+    std::map<int, std::unique_ptr<Session>> sessions;
+    uint64_t next_ticket = 1;
+    int next_session = 1;
+    size_t session_bytes = 0;
+    size_t lane_reserve = 0;   // per lane
+    uint64_t max_n = 0, max_p = 0;
+    bool stop = false;
+    std::string err;
+    struct Lane {
+        fvgs_ctx* ctx = nullptr;
+        hipStream_t stream = nullptr;
+        std::thread th;
+    };
+    std::vector<Lane> lanes;
+};
+
+namespace {
+
+int sfail(fvgs_server* s, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (s) s->err = buf;
+    return code;
+}
+
+int dev_alloc(DevMem& m, size_t bytes) {
+    m.p = nullptr;
+    m.bytes = 0;
+    if (bytes == 0) return FVGS_OK;
+    if (hipMalloc(&m.p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        m.p = nullptr;
+        return FVGS_E_OOM;
+    }
+    m.bytes = bytes;
+    return FVGS_OK;
+}
+
+void dev_free(DevMem& m) {
+    if (m.p) (void)hipFree(m.p);
+    m.p = nullptr;
+    m.bytes = 0;
+}
+
+void free_session(Session& s) {
+    DevMem* all[] = {&s.pos, &s.scale, &s.quat, &s.color, &s.opac, &s.grad};
+    for (DevMem* m : all) dev_free(*m);
+    for (Keyframe& k : s.keyframes) {
+        dev_free(k.rgb);
+        dev_free(k.depth);
+    }
+    s.keyframes.clear();
+}
+
+int copy_in(DevMem& dst, const void* src, size_t bytes, int mem) {
+    if (int r = dev_alloc(dst, bytes)) return r;
+    const hipMemcpyKind kind = mem == FVGS_MEM_DEVICE ? hipMemcpyDeviceToDevice
+                                                      : hipMemcpyHostToDevice;
+    if (hipMemcpy(dst.p, src, bytes, kind) != hipSuccess) {
+        (void)hipGetLastError();
+        dev_free(dst);
+        return FVGS_E_HIP;
+    }
+    return FVGS_OK;
+}
+
+// Runs one request on a lane (no server lock held).
+void run_request(fvgs_server* srv, fvgs_server::Lane& lane, Session& ses, Request& rq) {
+    fvgs_ctx* c = lane.ctx;
+    int r = fvgs_set_gaussians(c, ses.n, static_cast<const float*>(ses.pos.p),
+                               static_cast<const float*>(ses.scale.p),
+                               static_cast<const float*>(ses.quat.p),
+                               static_cast<const float*>(ses.color.p),
+                               static_cast<const float*>(ses.opac.p), FVGS_MEM_DEVICE);
+    if (rq.kind == FVGS_REQ_TRACK) {
+        if (r == FVGS_OK) r = fvgs_set_frame(c, rq.w, rq.h, rq.rgb, rq.depth, nullptr, nullptr, rq.mem);
+        if (r == FVGS_OK) r = fvgs_compute_gradients(c);
+        std::vector<double> ms((size_t)std::max(1, rq.iters), 0.0);
+        double pose[7];
+        std::memcpy(pose, rq.pose7, sizeof pose);
+        if (r == FVGS_OK) r = fvgs_track(c, &rq.intr, rq.iters, pose, ms.data());
+        std::memcpy(rq.res.pose7, pose, sizeof pose);
+        double dev = 0.0;
+        for (int i = 0; i < rq.iters; i++) dev += ms[(size_t)i];
+        rq.res.device_ms = dev;
+        rq.res.iters = rq.iters;
+    } else {
+        for (size_t k = 0; k < rq.keyframes.size() && r == FVGS_OK; k++) {
+            const Keyframe& kf = ses.keyframes[(size_t)rq.keyframes[k]];
+            r = fvgs_set_frame(c, kf.w, kf.h, static_cast<const float*>(kf.rgb.p),
+                               static_cast<const float*>(kf.depth.p), nullptr, nullptr,
+                               FVGS_MEM_DEVICE);
+            if (r == FVGS_OK) r = fvgs_mapping_step(c, &kf.pose, &rq.intr);
+            void* d3 = nullptr;
+            if (r == FVGS_OK) r = fvgs_device_ptr(c, FVGS_OUT_DELTA3D, &d3, nullptr);
+            if (r == FVGS_OK) {
+                if (launch_accum_delta3d(static_cast<const Delta3D*>(d3),
+                                         static_cast<Delta3D*>(ses.grad.p), (int)ses.n,
+                                         lane.stream) != hipSuccess ||
+                    hipStreamSynchronize(lane.stream) != hipSuccess) {
+                    (void)hipGetLastError();
+                    r = FVGS_E_HIP;
+                }
+            }
+        }
+    }
+    rq.res.status = r;
+    if (r != FVGS_OK) {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->err = fvgs_last_error(c);
+    }
+}
+
+// Next runnable request (server lock held): oldest first; a mapping request waits while its
+// session has one running.
+Request* take_request(fvgs_server* srv) {
+    for (auto it = srv->queue.begin(); it != srv->queue.end(); ++it) {
+        Request* rq = *it;
+        Session& s = *srv->sessions.at(rq->session);
+        if (rq->kind == FVGS_REQ_MAPPING && s.mapping_busy) continue;
+        if (rq->kind == FVGS_REQ_MAPPING) s.mapping_busy = true;
+        srv->queue.erase(it);
+        return rq;
+    }
+    return nullptr;
+}
+
+void lane_main(fvgs_server* srv, int li) {
+    fvgs_server::Lane& lane = srv->lanes[(size_t)li];
+    (void)hipSetDevice(srv->prm.params.device);
+    std::unique_lock<std::mutex> lk(srv->mu);
+    for (;;) {
+        Request* rq = nullptr;
+        srv->cv_work.wait(lk, [&] { return srv->stop || (rq = take_request(srv)) != nullptr; });
+        if (!rq) return;   // stop and nothing runnable
+        Session& ses = *srv->sessions.at(rq->session);
+        rq->t_start = Clock::now();
+        lk.unlock();
+        run_request(srv, lane, ses, *rq);
+        lk.lock();
+        rq->t_end = Clock::now();
+        rq->res.queue_ms = ms_between(rq->t_submit, rq->t_start);
+        rq->res.run_ms = ms_between(rq->t_start, rq->t_end);
+        if (rq->kind == FVGS_REQ_MAPPING) ses.mapping_busy = false;
+        ses.in_flight--;
+        rq->done = true;
+        srv->cv_done.notify_all();
+        srv->cv_work.notify_all();   // a waiting mapping request may now run
+    }
+}
+
+// Server lock held. Bytes that admitting (n, p) would add: its own memory plus the growth of
+// every lane's reserve when it is the largest session so far.
+size_t admission_bytes(const fvgs_server* srv, uint64_t n, uint64_t p, size_t own,
+                       size_t* new_reserve) {
+    const size_t res = lane_workspace(std::max<uint64_t>(srv->max_n, n),
+                                      std::max<uint64_t>(srv->max_p, p));
+    *new_reserve = std::max(res, srv->lane_reserve);
+    return srv->session_bytes + own + *new_reserve * srv->lanes.size();
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvgs_server_default_params(fvgs_server_params* p) {
+    if (!p) return FVGS_E_ARG;
+    std::memset(p, 0, sizeof *p);
+    p->lanes = 4;
+    p->max_sessions = 64;
+    p->hbm_budget = 0;
+    return fvgs_default_params(&p->params);
+}
+
+int fvgs_server_create(const fvgs_server_params* p, fvgs_server** out) {
+    if (!out) return FVGS_E_ARG;
+    *out = nullptr;
+    fvgs_server_params prm;
+    if (p) prm = *p;
+    else fvgs_server_default_params(&prm);
+    if (prm.lanes < 1 || prm.lanes > 16 || prm.max_sessions < 1) return FVGS_E_ARG;
+    if (prm.hbm_budget == 0) prm.hbm_budget = (size_t)200 << 30;
+    fvgs_server* srv = new (std::nothrow) fvgs_server();
+    if (!srv) return FVGS_E_OOM;
+    srv->prm = prm;
+    srv->lanes.resize((size_t)prm.lanes);
+    for (auto& l : srv->lanes) {
+        int r = fvgs_create(&prm.params, &l.ctx);
+        if (r == FVGS_OK && hipStreamCreate(&l.stream) != hipSuccess) r = FVGS_E_HIP;
+        if (r != FVGS_OK) {
+            for (auto& m : srv->lanes) {
+                if (m.ctx) fvgs_destroy(m.ctx);
+                if (m.stream) (void)hipStreamDestroy(m.stream);
+            }
+            delete srv;
+            return r;
+        }
+    }
+    for (int i = 0; i < prm.lanes; i++)
+        srv->lanes[(size_t)i].th = std::thread(lane_main, srv, i);
+    *out = srv;
+    return FVGS_OK;
+}
+
+int fvgs_server_destroy(fvgs_server* srv) {
+    if (!srv) return FVGS_E_ARG;
+    {
+        std::unique_lock<std::mutex> lk(srv->mu);
+        srv->cv_done.wait(lk, [&] {
+            for (auto& kv : srv->sessions)
+                if (kv.second->in_flight) return false;
+            return true;
+        });
+        srv->stop = true;
+    }
+    srv->cv_work.notify_all();
+    for (auto& l : srv->lanes)
+        if (l.th.joinable()) l.th.join();
+    for (auto& l : srv->lanes) {
+        fvgs_destroy(l.ctx);
+        (void)hipStreamDestroy(l.stream);
+    }
+    for (auto& kv : srv->sessions) free_session(*kv.second);
+    delete srv;
+    return FVGS_OK;
+}
+
+const char* fvgs_server_last_error(const fvgs_server* srv) {
+    return srv ? srv->err.c_str() : "null server";
+}
+
+int fvgs_server_open_session(fvgs_server* srv, uint32_t n, const float* pos3, const float* scale3,
+                             const float* quat4, const float* color3, const float* opacity,
+                             int mem, int frame_w, int frame_h, int* session) {
+    if (!srv || !session) return FVGS_E_ARG;
+    *session = 0;
+    if (n == 0 || !pos3 || !scale3 || !quat4 || !color3 || !opacity)
+        return sfail(srv, FVGS_E_ARG, "empty or null map");
+    if (mem != FVGS_MEM_HOST && mem != FVGS_MEM_DEVICE) return sfail(srv, FVGS_E_ARG, "bad mem");
+    if (frame_w <= 0 || frame_h <= 0) return sfail(srv, FVGS_E_ARG, "bad frame size");
+    const uint64_t p = (uint64_t)frame_w * frame_h;
+    const size_t own = (size_t)n * (56 + 56);   // map + gradient accumulator
+    size_t reserve = 0;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        if ((int)srv->sessions.size() >= srv->prm.max_sessions)
+            return sfail(srv, FVGS_E_STATE, "%d sessions open", (int)srv->sessions.size());
+        const size_t need = admission_bytes(srv, n, p, own, &reserve);
+        if (need > srv->prm.hbm_budget)
+            return sfail(srv, FVGS_E_OOM,
+                         "rejected: %zu bytes needed with the session, budget %zu", need,
+                         srv->prm.hbm_budget);
+        // Account now so that concurrent opens cannot both pass.
+        srv->session_bytes += own;
+        srv->lane_reserve = reserve;
+        srv->max_n = std::max<uint64_t>(srv->max_n, n);
+        srv->max_p = std::max<uint64_t>(srv->max_p, p);
+    }
+    (void)hipSetDevice(srv->prm.params.device);
+    auto s = std::make_unique<Session>();
+    s->n = n;
+    s->frame_w = frame_w;
+    s->frame_h = frame_h;
+    int r = copy_in(s->pos, pos3, 12ull * n, mem);
+    if (r == FVGS_OK) r = copy_in(s->scale, scale3, 12ull * n, mem);
+    if (r == FVGS_OK) r = copy_in(s->quat, quat4, 16ull * n, mem);
+    if (r == FVGS_OK) r = copy_in(s->color, color3, 12ull * n, mem);
+    if (r == FVGS_OK) r = copy_in(s->opac, opacity, 4ull * n, mem);
+    if (r == FVGS_OK) r = dev_alloc(s->grad, sizeof(Delta3D) * (size_t)n);
+    if (r == FVGS_OK && hipMemset(s->grad.p, 0, sizeof(Delta3D) * (size_t)n) != hipSuccess)
+        r = FVGS_E_HIP;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (r != FVGS_OK) {
+        free_session(*s);
+        srv->session_bytes -= own;
+        return sfail(srv, r, "session allocation failed");
+    }
+    s->bytes = own;
+    s->id = srv->next_session++;
+    *session = s->id;
+    srv->sessions[s->id] = std::move(s);
+    return FVGS_OK;
+}
+
+int fvgs_server_close_session(fvgs_server* srv, int session) {
+    if (!srv) return FVGS_E_ARG;
+    std::unique_ptr<Session> s;
+    {
+        std::unique_lock<std::mutex> lk(srv->mu);
+        auto it = srv->sessions.find(session);
+        if (it == srv->sessions.end()) return sfail(srv, FVGS_E_ARG, "no session %d", session);
+        it->second->closing = true;
+        srv->cv_done.wait(lk, [&] { return it->second->in_flight == 0; });
+        s = std::move(it->second);
+        srv->sessions.erase(it);
+        srv->session_bytes -= s->bytes;
+    }
+    free_session(*s);
+    return FVGS_OK;
+}
+
+int fvgs_server_add_keyframe(fvgs_server* srv, int session, const fvgs_pose* pose, int width,
+                             int height, const float* rgb3, const float* depth, int mem,
+                             int* keyframe) {
+    if (!srv || !pose || !rgb3 || !depth || !keyframe) return FVGS_E_ARG;
+    if (width <= 0 || height <= 0) return sfail(srv, FVGS_E_ARG, "bad keyframe size");
+    const size_t P = (size_t)width * height, own = 16 * P;
+    Session* s = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        auto it = srv->sessions.find(session);
+        if (it == srv->sessions.end() || it->second->closing)
+            return sfail(srv, FVGS_E_ARG, "no session %d", session);
+        s = it->second.get();
+        if (s->mapping_busy || s->in_flight)
+            return sfail(srv, FVGS_E_STATE, "session %d has requests in flight", session);
+        size_t reserve = 0;
+        if (admission_bytes(srv, s->n, P, own, &reserve) > srv->prm.hbm_budget)
+            return sfail(srv, FVGS_E_OOM, "rejected: keyframe exceeds the budget");
+        srv->session_bytes += own;
+        srv->lane_reserve = reserve;
+        srv->max_p = std::max<uint64_t>(srv->max_p, P);
+        s->bytes += own;
+    }
+    (void)hipSetDevice(srv->prm.params.device);
+    Keyframe k;
+    k.pose = *pose;
+    k.w = width;
+    k.h = height;
+    int r = copy_in(k.rgb, rgb3, 12 * P, mem);
+    if (r == FVGS_OK) r = copy_in(k.depth, depth, 4 * P, mem);
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (r != FVGS_OK) {
+        dev_free(k.rgb);
+        dev_free(k.depth);
+        srv->session_bytes -= own;
+        s->bytes -= own;
+        return sfail(srv, r, "keyframe allocation failed");
+    }
+    *keyframe = (int)s->keyframes.size();
+    s->keyframes.push_back(k);
+    return FVGS_OK;
+}
+
+int fvgs_server_submit_track(fvgs_server* srv, int session, int width, int height,
+                             const float* rgb3, const float* depth, int mem,
+                             const fvgs_intrinsics* intr, int iters, const double pose7[7],
+                             uint64_t* ticket) {
+    if (!srv || !rgb3 || !depth || !intr || !pose7 || !ticket || iters < 0) return FVGS_E_ARG;
+    if (mem != FVGS_MEM_HOST && mem != FVGS_MEM_DEVICE) return sfail(srv, FVGS_E_ARG, "bad mem");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->sessions.find(session);
+    if (it == srv->sessions.end() || it->second->closing)
+        return sfail(srv, FVGS_E_ARG, "no session %d", session);
+    Session& s = *it->second;
+    if ((uint64_t)width * height > (uint64_t)s.frame_w * s.frame_h)
+        return sfail(srv, FVGS_E_ARG, "frame larger than the session's admitted frame size");
+    auto rq = std::make_unique<Request>();
+    rq->ticket = srv->next_ticket++;
+    rq->kind = FVGS_REQ_TRACK;
+    rq->session = session;
+    rq->w = width;
+    rq->h = height;
+    rq->rgb = rgb3;
+    rq->depth = depth;
+    rq->mem = mem;
+    rq->intr = *intr;
+    rq->iters = iters;
+    std::memcpy(rq->pose7, pose7, sizeof rq->pose7);
+    rq->res.kind = FVGS_REQ_TRACK;
+    rq->res.session = session;
+    rq->t_submit = Clock::now();
+    s.in_flight++;
+    *ticket = rq->ticket;
+    srv->queue.push_back(rq.get());
+    srv->requests[rq->ticket] = std::move(rq);
+    srv->cv_work.notify_one();
+    return FVGS_OK;
+}
+
+int fvgs_server_submit_mapping(fvgs_server* srv, int session, const fvgs_intrinsics* intr,
+                               int n_keyframes, const int* keyframes, uint64_t* ticket) {
+    if (!srv || !intr || !keyframes || n_keyframes <= 0 || !ticket) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->sessions.find(session);
+    if (it == srv->sessions.end() || it->second->closing)
+        return sfail(srv, FVGS_E_ARG, "no session %d", session);
+    Session& s = *it->second;
+    for (int k = 0; k < n_keyframes; k++)
+        if (keyframes[k] < 0 || keyframes[k] >= (int)s.keyframes.size())
+            return sfail(srv, FVGS_E_ARG, "no keyframe %d in session %d", keyframes[k], session);
+    auto rq = std::make_unique<Request>();
+    rq->ticket = srv->next_ticket++;
+    rq->kind = FVGS_REQ_MAPPING;
+    rq->session = session;
+    rq->intr = *intr;
+    rq->keyframes.assign(keyframes, keyframes + n_keyframes);
+    rq->res.kind = FVGS_REQ_MAPPING;
+    rq->res.session = session;
+    rq->t_submit = Clock::now();
+    s.in_flight++;
+    *ticket = rq->ticket;
+    srv->queue.push_back(rq.get());
+    srv->requests[rq->ticket] = std::move(rq);
+    srv->cv_work.notify_one();
+    return FVGS_OK;
+}
+
+int fvgs_server_wait(fvgs_server* srv, uint64_t ticket, fvgs_request_result* out) {
+    if (!srv) return FVGS_E_ARG;
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->requests.find(ticket);
+    if (it == srv->requests.end())
+        return sfail(srv, FVGS_E_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+    Request* rq = it->second.get();
+    srv->cv_done.wait(lk, [&] { return rq->done; });
+    if (out) *out = rq->res;
+    const int st = rq->res.status;
+    srv->requests.erase(it);
+    return st;
+}
+
+int fvgs_server_read_gradients(fvgs_server* srv, int session, fvgs_delta3d* host, size_t bytes) {
+    if (!srv || !host) return FVGS_E_ARG;
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->sessions.find(session);
+    if (it == srv->sessions.end()) return sfail(srv, FVGS_E_ARG, "no session %d", session);
+    Session& s = *it->second;
+    srv->cv_done.wait(lk, [&] { return !s.mapping_busy; });
+    const size_t n = std::min(bytes, s.grad.bytes);
+    if (hipMemcpy(host, s.grad.p, n, hipMemcpyDeviceToHost) != hipSuccess) {
+        (void)hipGetLastError();
+        return sfail(srv, FVGS_E_HIP, "gradient read failed");
+    }
+    return FVGS_OK;
+}
+
+int fvgs_server_memory(fvgs_server* srv, size_t* sessions, size_t* lanes, size_t* budget) {
+    if (!srv) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (sessions) *sessions = srv->session_bytes;
+    if (lanes) *lanes = srv->lane_reserve * srv->lanes.size();
+    if (budget) *budget = srv->prm.hbm_budget;
+    return FVGS_OK;
+}
+
+}  // extern "C"

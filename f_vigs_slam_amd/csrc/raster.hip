@@ -561,6 +561,23 @@ __global__ __launch_bounds__(256) void k_delta3d(const float* __restrict__ pos,
     out[i] = o;
 }
 
+// dst += src per field (the n counts too): the 3D gradients of one mapping view added into an
+// accumulator over a window of views.
+__global__ __launch_bounds__(256) void k_accum_delta3d(const Delta3D* __restrict__ src,
+                                                       Delta3D* __restrict__ dst, int n) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= n) return;
+    const Delta3D a = src[g];
+    Delta3D b = dst[g];
+    b.px += a.px; b.py += a.py; b.pz += a.pz;
+    b.sx += a.sx; b.sy += a.sy; b.sz += a.sz;
+    b.ox += a.ox; b.oy += a.oy; b.oz += a.oz;
+    b.cr += a.cr; b.cg += a.cg; b.cb += a.cb;
+    b.alpha += a.alpha;
+    b.n += a.n;
+    dst[g] = b;
+}
+
 }  // namespace
 
 hipError_t launch_render(const View& v, const uint32_t* ranges, const uint32_t* idx,
@@ -594,6 +611,12 @@ hipError_t launch_delta3d(const float* pos, const float* scale, const float* qua
                           hipStream_t s) {
     hipLaunchKernelGGL(k_delta3d, dim3((n + 255) / 256), dim3(256), 0, s, pos, scale, quat, n, v,
                        lambda_iso, d2, d3);
+    return hipGetLastError();
+}
+
+hipError_t launch_accum_delta3d(const Delta3D* src, Delta3D* dst, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_accum_delta3d, dim3((n + 255) / 256), dim3(256), 0, s, src, dst, n);
     return hipGetLastError();
 }
 

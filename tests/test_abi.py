@@ -15,10 +15,11 @@ from f_vigs_slam_amd import context as C
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "fvgs", "fvgs.h")
+HEADERS = [HEADER, os.path.join(ROOT, "include", "fvgs", "fvgs_server.h")]
 
 
 def declared_functions():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(fvgs_\w+)\s*\(", text, re.M)))
 
 

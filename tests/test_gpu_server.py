@@ -1,0 +1,150 @@
+"""GPU tests of the session server (fvgs_server.h) through the C ABI.
+
+  * a request's result is the same alone and in a mix of concurrent requests of other sessions
+    (tracking: bit for bit, and equal to a plain context's fvgs_track);
+  * a mapping request's accumulated 3D gradients equal the sum of plain mapping steps over its
+    keyframes (relative 1e-5: float atomics of the backward add in a different order);
+  * admission by the memory budget rejects a session that does not fit and admits it again
+    once memory is freed.
+"""
+import numpy as np
+import pytest
+
+from f_vigs_slam_amd import Context, FvgsError, Server
+from f_vigs_slam_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+IDENT = np.array([0, 0, 0, 0, 0, 0, 1.0])
+
+
+def _session_case(n, w, h, seed):
+    scene = S.make_scene(n, seed=seed, width=w, height=h)
+    intr = S.intrinsics(w, h)
+    ctx = Context()
+    ctx.set_scene(scene)
+    ctx.render_tracking(IDENT, intr, w, h)
+    rgb, depth = ctx.read("TRACK_RGB"), ctx.read("TRACK_DEPTH")
+    ctx.close()
+    starts = [S.perturbed_pose(seed * 10 + k, rot_deg=2.0, trans_m=0.03) for k in range(2)]
+    return dict(scene=scene, intr=intr, w=w, h=h, rgb=rgb.reshape(h, w, 3),
+                depth=depth.reshape(h, w), starts=starts)
+
+
+@pytest.fixture(scope="module")
+def cases():
+    print("[server] building 3 session cases", flush=True)
+    return [_session_case(20000, 320, 240, 1), _session_case(60000, 640, 480, 2),
+            _session_case(120000, 848, 480, 3)]
+
+
+def _plain_track(c, start, iters):
+    ctx = Context()
+    ctx.set_scene(c["scene"])
+    ctx.set_frame(c["rgb"], c["depth"])
+    ctx.compute_gradients()
+    pose, _ = ctx.track(c["intr"], iters, start)
+    ctx.close()
+    return pose
+
+
+def test_server_tracking_same_alone_and_in_mix(cases):
+    iters = 8
+    plain = [[_plain_track(c, st, iters) for st in c["starts"]] for c in cases]
+    # alone: one lane, each request waited for before the next is submitted
+    alone = Server(lanes=1)
+    sids = [alone.open_session(c["scene"], c["w"], c["h"]) for c in cases]
+    res_alone = []
+    for sid, c in zip(sids, cases):
+        row = []
+        for st in c["starts"]:
+            r = alone.wait(alone.submit_track(sid, c["rgb"], c["depth"], c["intr"], iters, st))
+            assert r["status"] == 0
+            row.append(r["pose7"])
+        res_alone.append(row)
+    alone.close()
+    # mix: four lanes, every request submitted at once, interleaved across sessions
+    mix = Server(lanes=4)
+    sids = [mix.open_session(c["scene"], c["w"], c["h"]) for c in cases]
+    tickets = {}
+    for k in range(2):
+        for j, (sid, c) in enumerate(zip(sids, cases)):
+            for rep in range(2):   # the same request twice, concurrently
+                tickets[(j, k, rep)] = mix.submit_track(sid, c["rgb"], c["depth"], c["intr"],
+                                                        iters, c["starts"][k])
+    out = {key: mix.wait(t) for key, t in tickets.items()}
+    mix.close()
+    for (j, k, rep), r in out.items():
+        assert r["status"] == 0
+        np.testing.assert_array_equal(r["pose7"], res_alone[j][k])
+        np.testing.assert_array_equal(r["pose7"], plain[j][k])
+    q = [r["queue_ms"] for r in out.values()]
+    print(f"[server] {len(out)} mixed requests equal alone and plain; queue ms max {max(q):.2f}",
+          flush=True)
+
+
+def test_server_mapping_accumulates_keyframe_gradients(cases):
+    c = cases[1]
+    poses = [IDENT, S.perturbed_pose(77, rot_deg=3.0, trans_m=0.05)]
+    ctx = Context()
+    ctx.set_scene(c["scene"])
+    kf_imgs, want = [], None
+    for p in poses:
+        ctx.render_tracking(p, c["intr"], c["w"], c["h"])
+        rgb = ctx.read("TRACK_RGB").reshape(c["h"], c["w"], 3) * 0.9
+        depth = ctx.read("TRACK_DEPTH").reshape(c["h"], c["w"]) + 0.01
+        kf_imgs.append((rgb.astype(np.float32), depth.astype(np.float32)))
+        ctx.set_frame(*kf_imgs[-1])
+        ctx.mapping_step(p, c["intr"])
+        g = ctx.read("DELTA3D")
+        if want is None:
+            want = {f: g[f].astype(np.float64) for f in g.dtype.names}
+        else:
+            for f in g.dtype.names:
+                want[f] = want[f] + g[f]
+    ctx.close()
+    srv = Server(lanes=2)
+    sid = srv.open_session(c["scene"], c["w"], c["h"])
+    kfs = [srv.add_keyframe(sid, p, rgb, depth) for p, (rgb, depth) in zip(poses, kf_imgs)]
+    r = srv.wait(srv.submit_mapping(sid, c["intr"], kfs))
+    assert r["status"] == 0
+    got = srv.gradients(sid, len(c["scene"]["pos"]))
+    srv.close()
+    np.testing.assert_array_equal(got["n"], want["n"])
+    for f in ("position", "scale", "orientation", "color", "alpha"):
+        ref = want[f]
+        err = np.abs(got[f] - ref).max() / max(np.abs(ref).max(), 1e-30)
+        print(f"[server] mapping {f} rel err {err:.2e}", flush=True)
+        assert err <= 1e-5
+
+
+def test_server_admission_by_memory_budget(cases):
+    c = cases[0]
+    srv = Server(lanes=2)
+    mem = srv.memory()
+    sid = srv.open_session(c["scene"], c["w"], c["h"])
+    one = srv.memory()
+    need = one["sessions"] + one["lanes"]
+    budget = need + one["sessions"] // 2
+    srv.close()
+    # a budget that holds one such session (and its lane workspaces) but not two
+    srv = Server(lanes=2, hbm_budget=budget)
+    a = srv.open_session(c["scene"], c["w"], c["h"])
+    with pytest.raises(FvgsError) as e:
+        srv.open_session(c["scene"], c["w"], c["h"])
+    assert e.value.code == -3
+    # requests of the admitted session still run
+    r = srv.wait(srv.submit_track(a, c["rgb"], c["depth"], c["intr"], 2, c["starts"][0]))
+    assert r["status"] == 0
+    srv.close_session(a)
+    b = srv.open_session(c["scene"], c["w"], c["h"])
+    assert b != a
+    with pytest.raises(FvgsError) as e:
+        srv.submit_mapping(b, c["intr"], [0])   # no keyframe 0
+    assert e.value.code == -1
+    with pytest.raises(FvgsError) as e:
+        srv.submit_track(12345, c["rgb"], c["depth"], c["intr"], 2, c["starts"][0])
+    assert e.value.code == -1
+    srv.close()
+    print(f"[server] budget {budget / 2**20:.1f} MiB admits one session; lanes reserve "
+          f"{one['lanes'] / 2**20:.1f} MiB; empty server {mem}", flush=True)
