@@ -42,6 +42,11 @@ def parse():
     ap.add_argument("--track-solves", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tracking", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-sessions", type=int, default=16)
+    ap.add_argument("--served-lanes", type=int, default=4)
+    ap.add_argument("--served-requests", type=int, default=6,
+                    help="tracking requests per session in the closed-loop runs")
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--breakdown", action="store_true",
                     help="after the timed steps, run the forward-only blend K times (profiling)")
@@ -134,6 +139,192 @@ def tracking(args, log_fn):
                 final_trans_err_m=float(max(e[1] for e in errs)))
 
 
+def served(args, log_fn):
+    """Served vs plain on a mixed multi-session workload (closed loop and open loop).
+
+    16 sessions: N log-uniform in [1e5, 2e6], frames drawn from 640x480, 848x480, 1280x720; each
+    session's target is its pass-1 render at the identity pose. A tracking request is a 20-step
+    Gauss-Newton solve from a 2 deg / 3 cm perturbed start against a host frame; a mapping
+    request is a 4-keyframe window of mapping steps. Requests mix as the 30 Hz tracking / 2 Hz
+    mapping rates do (15 tracking : 1 mapping).
+      plain:  one context per session (maps resident), requests run one at a time;
+      served: fvgs_server with L lanes, requests overlap on the device.
+    Closed loop: every request is available at t = 0 (throughput). Open loop: Poisson arrivals
+    at 70 % of the plain loop's measured capacity (latency under load).
+    """
+    import threading
+    from f_vigs_slam_amd import Context, FvgsError, Server
+    from f_vigs_slam_amd import scenes as S
+    rng = np.random.default_rng(4)
+    sizes = [(640, 480), (848, 480), (1280, 720)]
+    sess = []
+    t0 = time.perf_counter()
+    for k in range(args.served_sessions):
+        n = int(np.exp(rng.uniform(np.log(1e5), np.log(2e6))))
+        w, h = sizes[int(rng.integers(0, 3))]
+        scene = S.make_scene(n, seed=100 + k, width=w, height=h)
+        intr = S.intrinsics(w, h)
+        ctx = Context()
+        ctx.set_scene(scene)
+        ctx.render_tracking(np.array([0, 0, 0, 0, 0, 0, 1.0]), intr, w, h)
+        rgb = ctx.read("TRACK_RGB").reshape(h, w, 3).copy()
+        depth = ctx.read("TRACK_DEPTH").reshape(h, w).copy()
+        kfs = []
+        for j in range(4):
+            p = S.perturbed_pose(1000 + 10 * k + j, rot_deg=1.0, trans_m=0.02)
+            ctx.render_tracking(p, intr, w, h)
+            kfs.append((p, (ctx.read("TRACK_RGB").reshape(h, w, 3) * 0.95).astype(np.float32),
+                        (ctx.read("TRACK_DEPTH").reshape(h, w) + 0.01).astype(np.float32)))
+        ctx.close()
+        sess.append(dict(n=n, w=w, h=h, scene=scene, intr=intr, rgb=rgb, depth=depth, kfs=kfs))
+    log_fn(f"served: {len(sess)} sessions, {sum(x['n'] for x in sess)} Gaussians, built in "
+           f"{time.perf_counter() - t0:.1f} s")
+    iters = args.track_iters
+    reqs = []   # (kind, session, start pose or None)
+    for r in range(args.served_requests):
+        for k in range(len(sess)):
+            reqs.append(("track", k, S.perturbed_pose(5000 + 100 * r + k, rot_deg=2.0,
+                                                      trans_m=0.03)))
+            if (r * len(sess) + k) % 15 == 14:
+                reqs.append(("map", k, None))
+    order = rng.permutation(len(reqs))
+    reqs = [reqs[i] for i in order]
+    n_track = sum(1 for q in reqs if q[0] == "track")
+
+    # ---- plain: one resident context per session, one request at a time ----
+    ctxs = []
+    for x in sess:
+        c = Context()
+        c.set_scene(x["scene"])
+        ctxs.append(c)
+
+    def plain_one(q):
+        kind, k, start = q
+        x, c = sess[k], ctxs[k]
+        if kind == "track":
+            c.set_frame(x["rgb"], x["depth"])
+            c.compute_gradients()
+            try:
+                pose, ms = c.track(x["intr"], iters, start)
+            except FvgsError as e:   # a failed Gauss-Newton step: same outcome when served
+                return ("failed", e.code, k), None
+            return pose, ms
+        for p, rgb, depth in x["kfs"]:
+            c.set_frame(rgb, depth)
+            c.mapping_step(p, x["intr"])
+        return None, None
+
+    for q in reqs[:4]:
+        plain_one(q)   # warm-up
+    t0 = time.perf_counter()
+    plain_res, plain_lat = [], []
+    for q in reqs:
+        a = time.perf_counter()
+        plain_res.append(plain_one(q))
+        plain_lat.append((time.perf_counter() - a) * 1e3)
+    plain_t = time.perf_counter() - t0
+    plain_steps = [float(np.mean(r[1])) for r in plain_res if r[1] is not None]
+
+    # ---- served: all requests submitted at once ----
+    srv = Server(lanes=args.served_lanes)
+    sids = [srv.open_session(x["scene"], x["w"], x["h"]) for x in sess]
+    kf_ids = [[srv.add_keyframe(sid, p, rgb, depth) for p, rgb, depth in x["kfs"]]
+              for sid, x in zip(sids, sess)]
+
+    def submit(q):
+        kind, k, start = q
+        x = sess[k]
+        if kind == "track":
+            return srv.submit_track(sids[k], x["rgb"], x["depth"], x["intr"], iters, start)
+        return srv.submit_mapping(sids[k], x["intr"], kf_ids[k])
+
+    for q in reqs[:4]:
+        srv.wait(submit(q))   # warm-up
+    t0 = time.perf_counter()
+    tickets = [submit(q) for q in reqs]
+    served_res = [srv.wait(t) for t in tickets]
+    served_t = time.perf_counter() - t0
+    served_steps = [r["device_ms"] / max(r["iters"], 1) for r in served_res if r["kind"] == 1]
+    same = True
+    failed = []
+    for sr, pr, q in zip(served_res, plain_res, reqs):
+        if q[0] != "track":
+            continue
+        if isinstance(pr[0], tuple):   # failed in the plain loop: must fail the same way
+            same &= sr["status"] == pr[0][1]
+            failed.append(dict(session=q[1], n=sess[q[1]]["n"], status=pr[0][1]))
+        else:
+            same &= sr["status"] == 0 and np.array_equal(sr["pose7"], pr[0])
+    bad = [r["status"] for r in served_res if r["status"] != 0]
+
+    # ---- open loop at 70 % of the plain capacity ----
+    rate = 0.7 * len(reqs) / plain_t
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+
+    def open_plain():
+        lat = []
+        t0 = time.perf_counter()
+        for q, at in zip(reqs, arrivals):
+            now = time.perf_counter() - t0
+            if now < at:
+                time.sleep(at - now)
+            plain_one(q)
+            lat.append((time.perf_counter() - t0 - at) * 1e3)
+        return lat
+
+    def open_served():
+        tk = [None] * len(reqs)
+        t0 = time.perf_counter()
+
+        def feeder():
+            for i, (q, at) in enumerate(zip(reqs, arrivals)):
+                now = time.perf_counter() - t0
+                if now < at:
+                    time.sleep(at - now)
+                tk[i] = submit(q)
+
+        th = threading.Thread(target=feeder)
+        th.start()
+        lat = []
+        for i in range(len(reqs)):
+            while tk[i] is None:
+                time.sleep(0.0002)
+            r = srv.wait(tk[i])
+            lat.append(r["queue_ms"] + r["run_ms"])
+        th.join()
+        return lat
+
+    lat_plain = open_plain()
+    lat_served = open_served()
+    srv.close()
+    for c in ctxs:
+        c.close()
+    pct = lambda a, q: round(float(np.percentile(a, q)), 3)
+    out = dict(
+        workload=f"{len(sess)} sessions, N log-uniform 1e5-2e6 (total "
+                 f"{sum(x['n'] for x in sess)}), frames 640x480/848x480/1280x720; "
+                 f"{n_track} tracking solves x {iters} steps + {len(reqs) - n_track} "
+                 f"4-keyframe mapping windows",
+        lanes=args.served_lanes,
+        closed_loop=dict(
+            plain_s=round(plain_t, 4), served_s=round(served_t, 4),
+            plain_track_steps_per_s=round(n_track * iters / plain_t, 1),
+            served_track_steps_per_s=round(n_track * iters / served_t, 1),
+            throughput_ratio=round(plain_t / served_t, 3),
+            step_ms_plain_p50=pct(plain_steps, 50), step_ms_plain_p99=pct(plain_steps, 99),
+            step_ms_served_p50=pct(served_steps, 50), step_ms_served_p99=pct(served_steps, 99),
+            served_equals_plain=bool(same), errors=bad, failed_solves=failed),
+        open_loop=dict(
+            rate_req_per_s=round(rate, 2),
+            plain_latency_ms_p50=pct(lat_plain, 50), plain_latency_ms_p99=pct(lat_plain, 99),
+            served_latency_ms_p50=pct(lat_served, 50),
+            served_latency_ms_p99=pct(lat_served, 99)))
+    log_fn(f"served: closed loop plain {plain_t:.2f} s served {served_t:.2f} s "
+           f"(x{plain_t / served_t:.2f}); open loop p99 plain {pct(lat_plain, 99)} ms served "
+           f"{pct(lat_served, 99)} ms; same results {same}")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -223,6 +414,11 @@ def main():
             result["tracking"] = tracking(args, log)
         except Exception as e:  # report, do not hide the main number
             result["tracking"] = {"error": str(e)}
+    if not args.no_served:
+        try:
+            result["served"] = served(args, log)
+        except Exception as e:  # report, do not hide the main number
+            result["served"] = {"error": repr(e)}
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         result["cpu_baseline"] = cpu_baseline(scene, intr, pose, obs_rgb, obs_depth, w, h,
