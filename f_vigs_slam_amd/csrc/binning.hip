@@ -632,24 +632,32 @@ hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uin
                             size_t lds_limit, hipStream_t s) {
     int idx_bits = 1;
     while (idx_bits < 32 && ((uint64_t)1 << idx_bits) < (uint64_t)n_gauss) ++idx_bits;
-    // Short lists: 256 threads, 2048 keys resident (~36 KB LDS, several workgroups per CU).
-    // Longer: 1024 threads, 8192 keys resident (~146 KB), streaming beyond that.
+    // Lists up to 2048 keys: 256 threads (~36 KB LDS, several workgroups per CU); up to 4096:
+    // 512 threads (~76 KB, two per CU); longer: 1024 threads, 8192 keys resident (~146 KB),
+    // streaming beyond that. Each launch covers every tile and skips the other lengths.
     using Small = SortCfg<256, 8>;
+    using Medium = SortCfg<512, 8>;
     using Large = SortCfg<1024, 8>;
-    static bool attr_set = false;
-    if (!attr_set) {
+    static const bool attrs = [] {
+        (void)hipFuncSetAttribute((const void*)k_tile_sort<512, 8>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)Medium::kLds);
         (void)hipFuncSetAttribute((const void*)k_tile_sort<1024, 8>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)Large::kLds);
-        attr_set = true;
-    }
+        return true;
+    }();
+    (void)attrs;
     if (Large::kLds > lds_limit) return hipErrorInvalidConfiguration;
-    const uint32_t small_cap = (uint32_t)Small::CHUNK;
+    const uint32_t small_cap = (uint32_t)Small::CHUNK, medium_cap = (uint32_t)Medium::CHUNK;
     hipLaunchKernelGGL((k_tile_sort<256, 8>), dim3(ntiles), dim3(256), Small::kLds, s, ranges,
                        idx_bits, pairs, tmp, sorted_idx, keys_out, 0u, small_cap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_tile_sort<512, 8>), dim3(ntiles), dim3(512), Medium::kLds, s, ranges,
+                       idx_bits, pairs, tmp, sorted_idx, keys_out, small_cap, medium_cap);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_tile_sort<1024, 8>), dim3(ntiles), dim3(1024), Large::kLds, s, ranges,
-                       idx_bits, pairs, tmp, sorted_idx, keys_out, small_cap, 0xffffffffu);
+                       idx_bits, pairs, tmp, sorted_idx, keys_out, medium_cap, 0xffffffffu);
     return hipGetLastError();
 }
 

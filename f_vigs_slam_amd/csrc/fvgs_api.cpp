@@ -62,7 +62,7 @@ struct fvgs_ctx {
     DevBuf r_rgb, r_depth, r_alpha;
     DevBuf o_rgb, o_depth, o_T, o_nc, o_err, o_maxc;
     DevBuf d2, d3, ckpt;
-    DevBuf pose_tiles, pose_sum, t_rgb, t_depth;
+    DevBuf pose_tiles, pose_sum, t_rgb, t_depth, pose_pix;
     uint32_t* h_m = nullptr;      // pinned
     double* h_sum = nullptr;      // pinned, 27
     // device-resident tracking loop: [View | pose7 | fail | status[iters] | M[iters]]
@@ -329,6 +329,14 @@ int run_delta3d(fvgs_ctx* c) {
     return FVGS_OK;
 }
 
+// Per-pixel pass-1 results handed from the render launch to the Jacobian launch (24 B/pixel).
+int pose_pixel_state(fvgs_ctx* c, size_t P, float4** pc, uint2** ps) {
+    TRY(ensure(c, c->pose_pix, 24 * P));
+    *pc = as<float4>(c->pose_pix);
+    *ps = reinterpret_cast<uint2*>(as<uint8_t>(c->pose_pix) + 16 * P);
+    return FVGS_OK;
+}
+
 // Pose Jacobians at the prepared view; leaves 27 FP64 sums in c->h_sum.
 int run_pose(fvgs_ctx* c, bool want_render) {
     TRY(check_frame(c));
@@ -344,9 +352,13 @@ int run_pose(fvgs_ctx* c, bool want_render) {
         trgb = as<float>(c->t_rgb);
         tdep = as<float>(c->t_depth);
     }
+    float4* pc = nullptr;
+    uint2* ps = nullptr;
+    TRY(pose_pixel_state(c, P, &pc, &ps));
     HIPCHK(c, launch_pose(c->view, nullptr, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
                           as<Rec>(c->rec), c->obs_rgb, c->obs_depth, c->gx, c->gy,
-                          blend_params(c->prm), as<double>(c->pose_tiles), trgb, tdep, c->stream));
+                          blend_params(c->prm), as<double>(c->pose_tiles), trgb, tdep, pc, ps,
+                          c->stream));
     HIPCHK(c, launch_pose_reduce(as<double>(c->pose_tiles), T, as<double>(c->pose_sum), c->stream));
     HIPCHK(c, hipMemcpyAsync(c->h_sum, c->pose_sum.p, 27 * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -469,7 +481,7 @@ int fvgs_destroy(fvgs_ctx* c) {
                           &c->blk_counts, &c->chunk_pre, &c->ranges, &c->d_m, &c->pairs,
                           &c->tmp, &c->sorted_idx, &c->keys, &c->r_rgb, &c->r_depth, &c->r_alpha,
                           &c->o_rgb, &c->o_depth, &c->o_T, &c->o_nc, &c->o_err, &c->o_maxc,
-                          &c->d2, &c->d3, &c->ckpt, &c->pose_tiles, &c->pose_sum, &c->t_rgb, &c->t_depth};
+                          &c->d2, &c->d3, &c->ckpt, &c->pose_tiles, &c->pose_sum, &c->t_rgb, &c->t_depth, &c->pose_pix};
         for (DevBuf* b : bufs) release(c, *b);
         for (auto& e : c->ev)
             if (e) (void)hipEventDestroy(e);
@@ -655,7 +667,7 @@ int fvgs_render_tracking(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsi
     HIPCHK(c, launch_pose(c->view, nullptr, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
                           as<Rec>(c->rec), nullptr, nullptr, nullptr, nullptr,
                           blend_params(c->prm), as<double>(c->pose_tiles), as<float>(c->t_rgb),
-                          as<float>(c->t_depth), c->stream));
+                          as<float>(c->t_depth), nullptr, nullptr, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return FVGS_OK;
 }
@@ -711,6 +723,9 @@ int fvgs_track(fvgs_ctx* c, const fvgs_intrinsics* intr, int iters, double pose7
     TRY(ensure(c, c->track, bytes));
     TRY(ensure(c, c->pose_tiles, 27 * 8ull * T));
     TRY(ensure(c, c->pose_sum, 27 * 8));
+    float4* pc = nullptr;
+    uint2* ps = nullptr;
+    TRY(pose_pixel_state(c, (size_t)v0.W * v0.H, &pc, &ps));
     if (c->h_track_bytes < bytes) {
         if (c->h_track) (void)hipHostFree(c->h_track);
         c->h_track = nullptr;
@@ -751,7 +766,7 @@ int fvgs_track(fvgs_ctx* c, const fvgs_intrinsics* intr, int iters, double pose7
                                        c->lds_limit, s));
             HIPCHK(c, launch_pose(v0, vd, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
                                   as<Rec>(c->rec), c->obs_rgb, c->obs_depth, c->gx, c->gy, bp,
-                                  as<double>(c->pose_tiles), nullptr, nullptr, s));
+                                  as<double>(c->pose_tiles), nullptr, nullptr, pc, ps, s));
             HIPCHK(c, launch_pose_reduce(as<double>(c->pose_tiles), T, as<double>(c->pose_sum), s));
             HIPCHK(c, launch_gn_step(as<double>(c->pose_sum), dpose, vd, dfail, dstatus + it, s));
         }

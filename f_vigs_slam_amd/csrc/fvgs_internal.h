@@ -86,7 +86,7 @@ hipError_t launch_pose(const View& v, const View* vd, const uint32_t* ranges,
                        const uint32_t* idx, const Rec* rec, const float* obs_rgb,
                        const float* obs_depth, const float* gx, const float* gy,
                        const BlendParams& bp, double* tiles27, float* out_rgb, float* out_depth,
-                       hipStream_t s);
+                       float4* pix_c, uint2* pix_s, hipStream_t s);
 // sum27 = (Jtr[6], JtJ upper[21]); updates pose7 (FP64) and the pose of *vd.
 hipError_t launch_gn_step(const double* sum27, double* pose7, View* vd, int* fail, int* status,
                           hipStream_t s);

@@ -106,6 +106,7 @@ __device__ __forceinline__ void raw_to_pose(const double* raw, const float Rf[3]
 // The tile's partial is a wave butterfly of the raw FP64 sums plus a fixed-order sum of the four
 // waves, transformed with R and written once per tile (no atomics, deterministic).
 // Render-only mode (obs_depth == nullptr): pass 1 for every in-bounds pixel, no Jacobians.
+template <int kPass>
 __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ vd,
                                               const uint32_t* __restrict__ ranges,
                                               const uint32_t* __restrict__ idx,
@@ -116,7 +117,9 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
                                               const float* __restrict__ gyi, BlendParams bp,
                                               double* __restrict__ tiles27,
                                               float* __restrict__ out_rgb,
-                                              float* __restrict__ out_depth) {
+                                              float* __restrict__ out_depth,
+                                              float4* __restrict__ pix_c,
+                                              uint2* __restrict__ pix_s) {
     const View v = vd ? *vd : vh;   // device-resident view (pose updated on the device) or argument
     __shared__ Rec recs[256];
     __shared__ uint8_t qmask[256];
@@ -133,8 +136,9 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
     const uint32_t s = ranges[2 * tile], n = ranges[2 * tile + 1] - s;
     const float px = (float)x, py = (float)y;
 
-    // Stages batch `base` in LDS and builds this wave's list of entries whose quadrant bit
-    // (pass 1: low nibble, pass 2: high nibble) is set. Returns the list length (uniform).
+    // Stages batch `base` in LDS and builds this wave's list of the entries its quadrant needs in
+    // this pass (pass 1: q > 0 and alpha >= 1/255; pass 2: alpha >= 1/255). Returns the list
+    // length (uniform).
     auto stage = [&](uint32_t base, int pass) -> int {
         const int cnt = (int)min(256u, n - base);
         if (tid < cnt) {
@@ -159,7 +163,15 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
     // ---------------- pass 1: render ----------------
     float C0 = 0.f, C1 = 0.f, C2 = 0.f, D = 0.f, final_T = 1.0f;
     uint32_t n_contrib = 0;
-    {
+    if (kPass == 2) {   // pass 1 ran as its own launch: read its per-pixel results
+        if (inside) {
+            const float4 c = pix_c[p];
+            const uint2 st = pix_s[p];
+            C0 = c.x; C1 = c.y; C2 = c.z; D = c.w;
+            n_contrib = st.x;
+            final_T = __uint_as_float(st.y);
+        }
+    } else {
         float T = 1.f;
         uint32_t last = 0;
         bool done = !inside;
@@ -190,11 +202,17 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
             C0 += T * bp.bg[0]; C1 += T * bp.bg[1]; C2 += T * bp.bg[2];
         }
     }
-    if (out_rgb && in_bounds) {
-        out_rgb[3 * p] = C0; out_rgb[3 * p + 1] = C1; out_rgb[3 * p + 2] = C2;
-        out_depth[p] = D;
+    if (kPass == 1) {
+        if (out_rgb && in_bounds) {
+            out_rgb[3 * p] = C0; out_rgb[3 * p + 1] = C1; out_rgb[3 * p + 2] = C2;
+            out_depth[p] = D;
+        }
+        if (!render_only && inside) {
+            pix_c[p] = make_float4(C0, C1, C2, D);
+            pix_s[p] = make_uint2(n_contrib, __float_as_uint(final_T));
+        }
+        return;
     }
-    if (render_only) return;   // uniform over the block
     inside = inside && (final_T < bp.alpha_thresh);
 
     double acc[27];
@@ -411,9 +429,15 @@ hipError_t launch_pose(const View& v, const View* vd, const uint32_t* ranges,
                        const uint32_t* idx, const Rec* rec, const float* obs_rgb,
                        const float* obs_depth, const float* gx, const float* gy,
                        const BlendParams& bp, double* tiles27, float* out_rgb, float* out_depth,
-                       hipStream_t s) {
-    hipLaunchKernelGGL(k_pose, dim3(v.ntx * v.nty), dim3(256), 0, s, v, vd, ranges, idx, rec,
-                       obs_rgb, obs_depth, gx, gy, bp, tiles27, out_rgb, out_depth);
+                       float4* pix_c, uint2* pix_s, hipStream_t s) {
+    // Pass 1 (few registers, 8 waves per SIMD) and the Jacobian passes (FP64 sums) as two
+    // launches, joined by 24 B of per-pixel state.
+    hipLaunchKernelGGL(k_pose<1>, dim3(v.ntx * v.nty), dim3(256), 0, s, v, vd, ranges, idx, rec,
+                       obs_rgb, obs_depth, gx, gy, bp, tiles27, out_rgb, out_depth, pix_c, pix_s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || obs_depth == nullptr) return e;
+    hipLaunchKernelGGL(k_pose<2>, dim3(v.ntx * v.nty), dim3(256), 0, s, v, vd, ranges, idx, rec,
+                       obs_rgb, obs_depth, gx, gy, bp, tiles27, out_rgb, out_depth, pix_c, pix_s);
     return hipGetLastError();
 }
 
