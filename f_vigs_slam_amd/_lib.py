@@ -22,11 +22,13 @@ FVGS_E_NOTPD = -5
 FVGS_E_STATE = -6
 MEM_HOST = 0
 MEM_DEVICE = 1
+PRECISION_F32 = 0
+PRECISION_F16 = 1
 
 OUT = dict(POS2D=1, COV2D=2, CONIC=3, DEPTH_G=4, PHAT=5, KEYS=6, INDICES=7, RANGES=8,
            RENDER_RGB=10, RENDER_DEPTH=11, RENDER_ALPHA=12, RGB=20, DEPTH=21, FINAL_T=22,
            N_CONTRIB=23, COLOR_ERROR=24, MAX_CONTRIB=25, DELTA2D=30, DELTA3D=31,
-           POSE_TILES=40, TRACK_RGB=41, TRACK_DEPTH=42)
+           POSE_TILES=40, TRACK_RGB=41, TRACK_DEPTH=42, MAP=50)
 
 # Every function the header declares (checked by the CPU test suite against the library).
 EXPORTS = [
@@ -49,7 +51,8 @@ class Params(ctypes.Structure):
                 ("depth_thresh", ctypes.c_float), ("depth_weight", ctypes.c_float),
                 ("w_depth", ctypes.c_float), ("w_dist", ctypes.c_float),
                 ("lambda_iso", ctypes.c_float), ("cull_emit", ctypes.c_int),
-                ("keep_keys", ctypes.c_int), ("mem_budget", ctypes.c_size_t)]
+                ("keep_keys", ctypes.c_int), ("mem_budget", ctypes.c_size_t),
+                ("precision", ctypes.c_int)]
 
 
 class Pose(ctypes.Structure):

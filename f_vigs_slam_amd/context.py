@@ -226,7 +226,8 @@ class Context:
                       N_CONTRIB=((h, w), np.uint32), COLOR_ERROR=((h, w, 3), np.float32),
                       MAX_CONTRIB=((T,), np.uint32), DELTA2D=((n,), L.DELTA2D_DTYPE),
                       DELTA3D=((n,), L.DELTA3D_DTYPE), POSE_TILES=((T, 27), np.float64),
-                      TRACK_RGB=((h, w, 3), np.float32), TRACK_DEPTH=((h, w), np.float32))
+                      TRACK_RGB=((h, w, 3), np.float32), TRACK_DEPTH=((h, w), np.float32),
+                      MAP=((n, 14), np.float32))
         shape, dt = shapes[what]
         out = np.zeros(shape, dt)
         self._check(self._lib.fvgs_read(self._h, L.OUT[what], L.ptr(out), out.nbytes))

@@ -156,9 +156,7 @@ __device__ __forceinline__ uint32_t tile_depth_bits(float u, float v, float z, f
 }
 
 __global__ __launch_bounds__(kBinThreads) void k_project(
-    const float* __restrict__ pos, const float* __restrict__ scale,
-    const float* __restrict__ quat, const float* __restrict__ color,
-    const float* __restrict__ opac, int n, View vh, const View* __restrict__ vd, int cull_emit,
+    MapRef map, int n, View vh, const View* __restrict__ vd, int cull_emit,
     Rec* __restrict__ rec, uint2* __restrict__ rect, uint32_t* __restrict__ blk_counts) {
     extern __shared__ uint32_t hist[];
     const View v = vd ? *vd : vh;   // device-resident view (pose updated on the device) or argument
@@ -172,8 +170,10 @@ __global__ __launch_bounds__(kBinThreads) void k_project(
         if (g >= n) break;
         float u = 0.f, vv = 0.f, z = 0.f, cov[3] = {0.f, 0.f, 0.f}, con[3] = {0.f, 0.f, 0.f},
               ph[2] = {0.f, 0.f};
-        bool vis = project_exact(pos + 3 * (size_t)g, scale + 3 * (size_t)g,
-                                 quat + 4 * (size_t)g, v, Rc, u, vv, z, cov, con, ph);
+        float pw[3], sc[3], qg[4], col[3], op;
+        map.geometry((size_t)g, pw, sc, qg);
+        map.appearance((size_t)g, col, op);
+        bool vis = project_exact(pw, sc, qg, v, Rc, u, vv, z, cov, con, ph);
         if (!vis) {
             u = vv = z = 0.f;
             cov[0] = cov[1] = cov[2] = 0.f;
@@ -183,8 +183,7 @@ __global__ __launch_bounds__(kBinThreads) void k_project(
         Rec r;
         r.r0 = make_float4(u, vv, con[0], con[1]);
         r.r1 = make_float4(con[2], z, ph[0], ph[1]);
-        r.r2 = make_float4(color[3 * (size_t)g], color[3 * (size_t)g + 1],
-                           color[3 * (size_t)g + 2], opac[g]);
+        r.r2 = make_float4(col[0], col[1], col[2], op);
         r.r3 = make_float4(cov[0], cov[1], cov[2], 0.f);
         rec[g] = r;
         uint2 rc;
@@ -591,16 +590,87 @@ __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ r
     }
 }
 
+// Precision-form storage of a chunk of kQuantChunk Gaussians (one workgroup): origin = centre
+// of the chunk's position bounding box (FP32), positions as FP16 offsets from it, the other
+// attributes as FP16 (round to nearest even).
+__global__ __launch_bounds__(kQuantChunk) void k_quantize(
+    const float* __restrict__ pos, const float* __restrict__ scale, const float* __restrict__ quat,
+    const float* __restrict__ color, const float* __restrict__ opac, int n,
+    float4* __restrict__ origin, uint16_t* __restrict__ hpos, uint16_t* __restrict__ hscale,
+    uint16_t* __restrict__ hquat, uint16_t* __restrict__ hcolor, uint16_t* __restrict__ hopac) {
+    __shared__ float red[2][3][kQuantChunk / 64];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const size_t g = (size_t)blockIdx.x * kQuantChunk + tid;
+    const bool valid = g < (size_t)n;
+    float mn[3], mx[3];
+    for (int k = 0; k < 3; ++k) {
+        mn[k] = valid ? pos[3 * g + k] : 3.0e38f;
+        mx[k] = valid ? pos[3 * g + k] : -3.0e38f;
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = fminf(mn[k], __shfl_xor(mn[k], o));
+            mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o));
+        }
+    if (lane == 0)
+        for (int k = 0; k < 3; ++k) { red[0][k][w] = mn[k]; red[1][k][w] = mx[k]; }
+    __syncthreads();
+    float c[3];
+    for (int k = 0; k < 3; ++k) {
+        float a = red[0][k][0], b = red[1][k][0];
+        for (int i = 1; i < kQuantChunk / 64; ++i) {
+            a = fminf(a, red[0][k][i]);
+            b = fmaxf(b, red[1][k][i]);
+        }
+        c[k] = (a + b) * 0.5f;
+    }
+    if (tid == 0) origin[blockIdx.x] = make_float4(c[0], c[1], c[2], 0.f);
+    if (!valid) return;
+    auto h = [](float x) { return __half_as_ushort(__float2half_rn(x)); };
+    for (int k = 0; k < 3; ++k) {
+        hpos[3 * g + k] = h(pos[3 * g + k] - c[k]);
+        hscale[3 * g + k] = h(scale[3 * g + k]);
+        hcolor[3 * g + k] = h(color[3 * g + k]);
+    }
+    for (int k = 0; k < 4; ++k) hquat[4 * g + k] = h(quat[4 * g + k]);
+    hopac[g] = h(opac[g]);
+}
+
+__global__ __launch_bounds__(256) void k_dequantize(MapRef map, int n, float* __restrict__ out) {
+    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= (size_t)n) return;
+    float p[3], sc[3], q[4], c[3], o;
+    map.geometry(g, p, sc, q);
+    map.appearance(g, c, o);
+    float* r = out + 14 * g;
+    for (int k = 0; k < 3; ++k) { r[k] = p[k]; r[3 + k] = sc[k]; r[10 + k] = c[k]; }
+    for (int k = 0; k < 4; ++k) r[6 + k] = q[k];
+    r[13] = o;
+}
+
 }  // namespace
 
-hipError_t launch_project(const float* pos, const float* scale, const float* quat,
-                          const float* color, const float* opac, int n, const View& v,
-                          const View* vd, int cull_emit, Rec* rec, uint2* rect,
-                          uint32_t* blk_counts, hipStream_t s) {
+hipError_t launch_project(const MapRef& map, int n, const View& v, const View* vd, int cull_emit,
+                          Rec* rec, uint2* rect, uint32_t* blk_counts, hipStream_t s) {
     int nblk = (n + kBinChunk - 1) / kBinChunk;
     size_t lds = (size_t)v.ntx * v.nty * 4;
-    hipLaunchKernelGGL(k_project, dim3(nblk), dim3(kBinThreads), lds, s, pos, scale, quat, color,
-                       opac, n, v, vd, cull_emit, rec, rect, blk_counts);
+    hipLaunchKernelGGL(k_project, dim3(nblk), dim3(kBinThreads), lds, s, map, n, v, vd,
+                       cull_emit, rec, rect, blk_counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_quantize(const float* pos, const float* scale, const float* quat,
+                           const float* color, const float* opac, int n, float4* origin,
+                           uint16_t* hpos, uint16_t* hscale, uint16_t* hquat, uint16_t* hcolor,
+                           uint16_t* hopac, hipStream_t s) {
+    const int nchunk = (n + kQuantChunk - 1) / kQuantChunk;
+    hipLaunchKernelGGL(k_quantize, dim3(nchunk), dim3(kQuantChunk), 0, s, pos, scale, quat,
+                       color, opac, n, origin, hpos, hscale, hquat, hcolor, hopac);
+    return hipGetLastError();
+}
+
+hipError_t launch_dequantize(const MapRef& map, int n, float* out14, hipStream_t s) {
+    hipLaunchKernelGGL(k_dequantize, dim3((n + 255) / 256), dim3(256), 0, s, map, n, out14);
     return hipGetLastError();
 }
 

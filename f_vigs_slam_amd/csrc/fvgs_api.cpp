@@ -45,6 +45,7 @@ struct fvgs_ctx {
     // map
     uint32_t n = 0;
     DevBuf own_pos, own_scale, own_quat, own_color, own_opac;
+    DevBuf q_origin, q_pos, q_scale, q_quat, q_color, q_opac, map_out;   // precision form
     const float *pos = nullptr, *scale = nullptr, *quat = nullptr, *color = nullptr,
                 *opac = nullptr;
     // frame
@@ -147,7 +148,24 @@ BlendParams blend_params(const fvgs_params& p) {
     b.w_depth = p.w_depth;
     b.w_dist = p.w_dist;
     b.lambda_iso = p.lambda_iso;
+    b.kahan = p.precision == FVGS_PRECISION_F16 ? 1 : 0;
     return b;
+}
+
+MapRef map_ref(fvgs_ctx* c) {
+    MapRef m{};
+    if (c->prm.precision == FVGS_PRECISION_F16) {
+        m.origin = as<float4>(c->q_origin);
+        m.hpos = as<uint16_t>(c->q_pos);
+        m.hscale = as<uint16_t>(c->q_scale);
+        m.hquat = as<uint16_t>(c->q_quat);
+        m.hcolor = as<uint16_t>(c->q_color);
+        m.hopac = as<uint16_t>(c->q_opac);
+    } else {
+        m.pos = c->pos; m.scale = c->scale; m.quat = c->quat; m.color = c->color;
+        m.opac = c->opac;
+    }
+    return m;
 }
 
 bool finite_pose(const fvgs_pose* p) {
@@ -227,8 +245,8 @@ int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int
     hipStream_t s = c->stream;
     c->view = v;
     mark(c, 0);
-    HIPCHK(c, launch_project(c->pos, c->scale, c->quat, c->color, c->opac, n, v, nullptr,
-                             cull_emit, as<Rec>(c->rec), as<uint2>(c->rect),
+    HIPCHK(c, launch_project(map_ref(c), n, v, nullptr, cull_emit, as<Rec>(c->rec),
+                             as<uint2>(c->rect),
                              as<uint32_t>(c->blk_counts), s));
     HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->chunk_pre), nblk, T,
                              as<uint32_t>(c->ranges), as<uint32_t>(c->d_m),
@@ -324,7 +342,7 @@ int run_backward(fvgs_ctx* c) {
 int run_delta3d(fvgs_ctx* c) {
     if (!c->backwarded) return fail(c, FVGS_E_STATE, "call fvgs_backward first");
     TRY(ensure(c, c->d3, sizeof(Delta3D) * (size_t)c->n));
-    HIPCHK(c, launch_delta3d(c->pos, c->scale, c->quat, (int)c->n, c->view, c->prm.lambda_iso,
+    HIPCHK(c, launch_delta3d(map_ref(c), (int)c->n, c->view, c->prm.lambda_iso,
                              as<Delta2D>(c->d2), as<Delta3D>(c->d3), c->stream));
     return FVGS_OK;
 }
@@ -404,6 +422,7 @@ OutDesc describe(fvgs_ctx* c, int what) {
         case FVGS_OUT_POSE_TILES: return {c->pose_tiles.p, 216 * T};
         case FVGS_OUT_TRACK_RGB: return {c->t_rgb.p, 12 * P};
         case FVGS_OUT_TRACK_DEPTH: return {c->t_depth.p, 4 * P};
+        case FVGS_OUT_MAP: return {c->map_out.p, 56 * n};
         default: return {nullptr, 0};
     }
 }
@@ -477,6 +496,8 @@ int fvgs_destroy(fvgs_ctx* c) {
         std::lock_guard<std::mutex> lk(c->mu);
         if (c->stream) (void)hipStreamSynchronize(c->stream);
         DevBuf* bufs[] = {&c->own_pos, &c->own_scale, &c->own_quat, &c->own_color, &c->own_opac,
+                          &c->q_origin, &c->q_pos, &c->q_scale, &c->q_quat, &c->q_color,
+                          &c->q_opac, &c->map_out,
                           &c->own_rgb, &c->own_depth, &c->own_gx, &c->own_gy, &c->rec, &c->rect,
                           &c->blk_counts, &c->chunk_pre, &c->ranges, &c->d_m, &c->pairs,
                           &c->tmp, &c->sorted_idx, &c->keys, &c->r_rgb, &c->r_depth, &c->r_alpha,
@@ -515,6 +536,24 @@ int fvgs_set_gaussians(fvgs_ctx* c, uint32_t n, const float* pos3, const float* 
     TRY(upload(c, c->own_quat, c->quat, quat4, 16ull * n, mem));
     TRY(upload(c, c->own_color, c->color, color3, 12ull * n, mem));
     TRY(upload(c, c->own_opac, c->opac, opacity, 4ull * n, mem));
+    if (c->prm.precision == FVGS_PRECISION_F16) {
+        const size_t nchunk = (n + kQuantChunk - 1) / kQuantChunk;
+        TRY(ensure(c, c->q_origin, 16 * nchunk));
+        TRY(ensure(c, c->q_pos, 6ull * n));
+        TRY(ensure(c, c->q_scale, 6ull * n));
+        TRY(ensure(c, c->q_quat, 8ull * n));
+        TRY(ensure(c, c->q_color, 6ull * n));
+        TRY(ensure(c, c->q_opac, 2ull * n));
+        HIPCHK(c, launch_quantize(c->pos, c->scale, c->quat, c->color, c->opac, (int)n,
+                                  as<float4>(c->q_origin), as<uint16_t>(c->q_pos),
+                                  as<uint16_t>(c->q_scale), as<uint16_t>(c->q_quat),
+                                  as<uint16_t>(c->q_color), as<uint16_t>(c->q_opac), c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        // only the compact form stays resident
+        DevBuf* own[] = {&c->own_pos, &c->own_scale, &c->own_quat, &c->own_color, &c->own_opac};
+        for (DevBuf* b : own) release(c, *b);
+        c->pos = c->scale = c->quat = c->color = c->opac = nullptr;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->n = n;
     return FVGS_OK;
@@ -753,8 +792,7 @@ int fvgs_track(fvgs_ctx* c, const fvgs_intrinsics* intr, int iters, double pose7
         HIPCHK(c, hipMemcpyAsync(d, c->h_track, off_status, hipMemcpyHostToDevice, s));
         for (int it = 0; it < iters; it++) {
             HIPCHK(c, hipEventRecord(c->track_ev[it], s));
-            HIPCHK(c, launch_project(c->pos, c->scale, c->quat, c->color, c->opac, n, v0, vd, 0,
-                                     as<Rec>(c->rec), as<uint2>(c->rect),
+            HIPCHK(c, launch_project(map_ref(c), n, v0, vd, 0, as<Rec>(c->rec), as<uint2>(c->rect),
                                      as<uint32_t>(c->blk_counts), s));
             HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->chunk_pre), nblk,
                                      T, as<uint32_t>(c->ranges), dm + it, c->inst_cap, s));
@@ -798,9 +836,19 @@ int fvgs_track(fvgs_ctx* c, const fvgs_intrinsics* intr, int iters, double pose7
     return FVGS_OK;
 }
 
+// FVGS_OUT_MAP: the map as the kernels read it, written into map_out on demand.
+static int materialise_map(fvgs_ctx* c) {
+    if (c->n == 0) return fail(c, FVGS_E_EMPTY, "no Gaussians");
+    TRY(ensure(c, c->map_out, 56ull * c->n));
+    HIPCHK(c, launch_dequantize(map_ref(c), (int)c->n, as<float>(c->map_out), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FVGS_OK;
+}
+
 int fvgs_read(fvgs_ctx* c, int what, void* host, size_t bytes) {
     if (!c || !host) return FVGS_E_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
+    if (what == FVGS_OUT_MAP) TRY(materialise_map(c));
     OutDesc d = describe(c, what);
     if (!d.p) return fail(c, FVGS_E_STATE, "result %d is not available", what);
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -838,6 +886,7 @@ int fvgs_read(fvgs_ctx* c, int what, void* host, size_t bytes) {
 int fvgs_device_ptr(fvgs_ctx* c, int what, void** ptr, size_t* bytes) {
     if (!c || !ptr) return FVGS_E_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
+    if (what == FVGS_OUT_MAP) TRY(materialise_map(c));
     OutDesc d = describe(c, what);
     if (!d.p) return fail(c, FVGS_E_STATE, "result %d is not available", what);
     *ptr = const_cast<void*>(d.p);

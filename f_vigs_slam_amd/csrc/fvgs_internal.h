@@ -1,5 +1,6 @@
 // Internal interface between the C ABI (fvgs_api.cpp) and the gfx950 kernels (*.hip).
 #pragma once
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -43,17 +44,58 @@ struct Delta3D {  // DeltaGaussian3D layout (56 bytes)
     int32_t n;
 };
 
+// The map as the kernels read it. Plain form: FP32 arrays (pos3, scale3, quat4, color3, opacity).
+// Precision form (hpos != nullptr): FP16 storage, 28 B per Gaussian — positions as FP16 offsets
+// from an FP32 origin per chunk of kQuantChunk consecutive Gaussians (the centre of the chunk's
+// bounding box), scale, quaternion, colour and opacity as FP16 — dequantised to FP32 on load.
+constexpr int kQuantChunk = 256;
+struct MapRef {
+    const float *pos, *scale, *quat, *color, *opac;
+    const uint16_t *hpos, *hscale, *hquat, *hcolor, *hopac;
+    const float4* origin;
+    __device__ __forceinline__ static float h2f(uint16_t u) {
+        return __half2float(__ushort_as_half(u));
+    }
+    __device__ __forceinline__ void geometry(size_t g, float p[3], float s[3], float q[4]) const {
+        if (hpos) {
+            const float4 o = origin[g / kQuantChunk];
+            p[0] = o.x + h2f(hpos[3 * g]); p[1] = o.y + h2f(hpos[3 * g + 1]);
+            p[2] = o.z + h2f(hpos[3 * g + 2]);
+            for (int k = 0; k < 3; ++k) s[k] = h2f(hscale[3 * g + k]);
+            for (int k = 0; k < 4; ++k) q[k] = h2f(hquat[4 * g + k]);
+        } else {
+            for (int k = 0; k < 3; ++k) { p[k] = pos[3 * g + k]; s[k] = scale[3 * g + k]; }
+            for (int k = 0; k < 4; ++k) q[k] = quat[4 * g + k];
+        }
+    }
+    __device__ __forceinline__ void appearance(size_t g, float c[3], float& o) const {
+        if (hpos) {
+            for (int k = 0; k < 3; ++k) c[k] = h2f(hcolor[3 * g + k]);
+            o = h2f(hopac[g]);
+        } else {
+            for (int k = 0; k < 3; ++k) c[k] = color[3 * g + k];
+            o = opac[g];
+        }
+    }
+};
+
 struct BlendParams {
     float bg[3];
     float alpha_thresh, color_thresh, depth_thresh, depth_weight;
     float w_depth, w_dist, lambda_iso;
+    int kahan;   // pose sums: 0 FP64 running sums, 1 Kahan FP32 (precision form)
 };
 
 // ---- binning (binning.hip) ----
-hipError_t launch_project(const float* pos, const float* scale, const float* quat,
-                          const float* color, const float* opac, int n, const View& v,
-                          const View* vd, int cull_emit, Rec* rec, uint2* rect,
-                          uint32_t* blk_counts, hipStream_t s);
+hipError_t launch_project(const MapRef& map, int n, const View& v, const View* vd, int cull_emit,
+                          Rec* rec, uint2* rect, uint32_t* blk_counts, hipStream_t s);
+// FP32 map -> precision-form storage (origins, FP16 offsets and attributes).
+hipError_t launch_quantize(const float* pos, const float* scale, const float* quat,
+                           const float* color, const float* opac, int n, float4* origin,
+                           uint16_t* hpos, uint16_t* hscale, uint16_t* hquat, uint16_t* hcolor,
+                           uint16_t* hopac, hipStream_t s);
+// The map as the kernels read it, FP32 [n][14]: pos3 scale3 quat4 color3 opacity.
+hipError_t launch_dequantize(const MapRef& map, int n, float* out14, hipStream_t s);
 hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* chunk_pre, int nblk, int ntiles,
                           uint32_t* ranges, uint32_t* d_m, uint32_t cap, hipStream_t s);
 hipError_t launch_emit(const Rec* rec, const uint2* rect, const uint32_t* blk_counts,
@@ -75,9 +117,8 @@ hipError_t launch_backward(const View& v, const uint32_t* ranges, const uint32_t
                            const float* rgb, const float* depth, const uint32_t* n_contrib,
                            const float* color_err, const uint32_t* max_contrib,
                            const float4* ckpt, Delta2D* delta, hipStream_t s);
-hipError_t launch_delta3d(const float* pos, const float* scale, const float* quat, int n,
-                          const View& v, float lambda_iso, const Delta2D* d2, Delta3D* d3,
-                          hipStream_t s);
+hipError_t launch_delta3d(const MapRef& map, int n, const View& v, float lambda_iso,
+                          const Delta2D* d2, Delta3D* d3, hipStream_t s);
 
 hipError_t launch_accum_delta3d(const Delta3D* src, Delta3D* dst, int n, hipStream_t s);
 

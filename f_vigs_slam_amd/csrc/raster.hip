@@ -352,10 +352,7 @@ __device__ __forceinline__ void qmat(float x, float y, float z, float w, float R
     R[2][2] = 1.0f - 2.0f * (x * x + y * y);
 }
 
-__global__ __launch_bounds__(256) void k_delta3d(const float* __restrict__ pos,
-                                                 const float* __restrict__ scale,
-                                                 const float* __restrict__ quat, int n, View v,
-                                                 float lambda_iso,
+__global__ __launch_bounds__(256) void k_delta3d(MapRef map, int n, View v, float lambda_iso,
                                                  const Delta2D* __restrict__ d2in,
                                                  Delta3D* __restrict__ out) {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -372,8 +369,9 @@ __global__ __launch_bounds__(256) void k_delta3d(const float* __restrict__ pos,
     o.alpha = d2.alpha * eta;
     float Rq[3][3];
     qmat(v.qx, v.qy, v.qz, v.qw, Rq);
-    const float dw[3] = {pos[3 * (size_t)i] - v.px, pos[3 * (size_t)i + 1] - v.py,
-                         pos[3 * (size_t)i + 2] - v.pz};
+    float pw[3], sw[3], qw[4];
+    map.geometry((size_t)i, pw, sw, qw);
+    const float dw[3] = {pw[0] - v.px, pw[1] - v.py, pw[2] - v.pz};
     float mu[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) mu[a] = Rq[0][a] * dw[0] + Rq[1][a] * dw[1] + Rq[2][a] * dw[2];
@@ -386,9 +384,8 @@ __global__ __launch_bounds__(256) void k_delta3d(const float* __restrict__ pos,
 #pragma unroll
         for (int b = 0; b < 3; b++) Rcw[a][b] = Rq[b][a];
     float Rg[3][3];
-    qmat(quat[4 * (size_t)i], quat[4 * (size_t)i + 1], quat[4 * (size_t)i + 2],
-         quat[4 * (size_t)i + 3], Rg);
-    const float s[3] = {scale[3 * (size_t)i], scale[3 * (size_t)i + 1], scale[3 * (size_t)i + 2]};
+    qmat(qw[0], qw[1], qw[2], qw[3], Rg);
+    const float s[3] = {sw[0], sw[1], sw[2]};
     float T[3][3], RS[3][3], M[3][3], Sp[3][3], S3[3][3];
 #pragma unroll
     for (int a = 0; a < 3; a++)
@@ -606,11 +603,10 @@ hipError_t launch_backward(const View& v, const uint32_t* ranges, const uint32_t
     return hipGetLastError();
 }
 
-hipError_t launch_delta3d(const float* pos, const float* scale, const float* quat, int n,
-                          const View& v, float lambda_iso, const Delta2D* d2, Delta3D* d3,
-                          hipStream_t s) {
-    hipLaunchKernelGGL(k_delta3d, dim3((n + 255) / 256), dim3(256), 0, s, pos, scale, quat, n, v,
-                       lambda_iso, d2, d3);
+hipError_t launch_delta3d(const MapRef& map, int n, const View& v, float lambda_iso,
+                          const Delta2D* d2, Delta3D* d3, hipStream_t s) {
+    hipLaunchKernelGGL(k_delta3d, dim3((n + 255) / 256), dim3(256), 0, s, map, n, v, lambda_iso,
+                       d2, d3);
     return hipGetLastError();
 }
 

@@ -40,10 +40,11 @@ __device__ __forceinline__ void qmat_t(float x, float y, float z, float w, float
 // sums are FP64 (JtJ adds terms spanning ~10 orders of magnitude, FP32 sums lose positive
 // definiteness); R is applied once per tile, in FP64.
 // raw layout: S_H (00 01 02 11 12 22) | S_HK (3x3 row-major) | S_KHK (00 01 02 11 12 22) | S_b | S_Kb
-__device__ __forceinline__ void accumulate_raw(double acc[27], const float H[9], const float b3[3],
+template <typename Add>
+__device__ __forceinline__ void accumulate_raw(Add&& add, const float H[9], const float b3[3],
                                                float d, float rx, float ry) {
-    acc[0] += (double)H[0]; acc[1] += (double)H[1]; acc[2] += (double)H[2];
-    acc[3] += (double)H[4]; acc[4] += (double)H[5]; acc[5] += (double)H[8];
+    add(0, H[0]); add(1, H[1]); add(2, H[2]);
+    add(3, H[4]); add(4, H[5]); add(5, H[8]);
     // M = H K^T: K rows (0, -1, ry), (1, 0, -rx), (-ry, rx, 0)
     float M[9];
 #pragma unroll
@@ -53,19 +54,33 @@ __device__ __forceinline__ void accumulate_raw(double acc[27], const float H[9],
         M[i * 3 + 2] = rx * H[i * 3 + 1] - ry * H[i * 3 + 0];
     }
 #pragma unroll
-    for (int k = 0; k < 9; ++k) acc[6 + k] += (double)(d * M[k]);
+    for (int k = 0; k < 9; ++k) add(6 + k, d * M[k]);
     // K M (symmetric): row 0 = -M[1][:] + ry M[2][:], row 1 = M[0][:] - rx M[2][:],
     // row 2 = -ry M[0][:] + rx M[1][:]
     const float d2 = d * d;
     const float k00 = ry * M[6] - M[3], k01 = ry * M[7] - M[4], k02 = ry * M[8] - M[5];
     const float k11 = M[1] - rx * M[7], k12 = M[2] - rx * M[8];
     const float k22 = rx * M[5] - ry * M[2];
-    acc[15] += (double)(d2 * k00); acc[16] += (double)(d2 * k01); acc[17] += (double)(d2 * k02);
-    acc[18] += (double)(d2 * k11); acc[19] += (double)(d2 * k12); acc[20] += (double)(d2 * k22);
-    acc[21] += (double)b3[0]; acc[22] += (double)b3[1]; acc[23] += (double)b3[2];
-    acc[24] += (double)(d * (ry * b3[2] - b3[1]));
-    acc[25] += (double)(d * (b3[0] - rx * b3[2]));
-    acc[26] += (double)(d * (rx * b3[1] - ry * b3[0]));
+    add(15, d2 * k00); add(16, d2 * k01); add(17, d2 * k02);
+    add(18, d2 * k11); add(19, d2 * k12); add(20, d2 * k22);
+    add(21, b3[0]); add(22, b3[1]); add(23, b3[2]);
+    add(24, d * (ry * b3[2] - b3[1]));
+    add(25, d * (b3[0] - rx * b3[2]));
+    add(26, d * (rx * b3[1] - ry * b3[0]));
+}
+
+// Compensated FP32 arithmetic of the precision form: a value is an unevaluated pair hi + lo.
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void pair_add(float& hi, float& lo, float bhi, float blo) {
+    float s, e;
+    two_sum(hi, bhi, s, e);
+    e += lo + blo;
+    hi = s + e;
+    lo = e - (hi - s);
 }
 
 // Raw sums -> (Jtr[6], JtJ upper[21]) with the view's R (FP64).
@@ -121,6 +136,9 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
                                               float4* __restrict__ pix_c,
                                               uint2* __restrict__ pix_s) {
     const View v = vd ? *vd : vh;   // device-resident view (pose updated on the device) or argument
+    // kPass 1: pass 1 only; 2: Jacobians with FP64 running sums; 3: Jacobians with Kahan-
+    // compensated FP32 running sums and a compensated FP32 block reduction (precision form).
+    constexpr bool kKahan = kPass == 3;
     __shared__ Rec recs[256];
     __shared__ uint8_t qmask[256];
     __shared__ uint8_t wlist[4][256];
@@ -163,7 +181,7 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
     // ---------------- pass 1: render ----------------
     float C0 = 0.f, C1 = 0.f, C2 = 0.f, D = 0.f, final_T = 1.0f;
     uint32_t n_contrib = 0;
-    if (kPass == 2) {   // pass 1 ran as its own launch: read its per-pixel results
+    if (kPass != 1) {   // pass 1 ran as its own launch: read its per-pixel results
         if (inside) {
             const float4 c = pix_c[p];
             const uint2 st = pix_s[p];
@@ -215,9 +233,22 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
     }
     inside = inside && (final_T < bp.alpha_thresh);
 
-    double acc[27];
+    double acc[kKahan ? 1 : 27];
+    float ks[kKahan ? 27 : 1], kc[kKahan ? 27 : 1];   // Kahan sum and compensation
 #pragma unroll
-    for (int k = 0; k < 27; ++k) acc[k] = 0.0;
+    for (int k = 0; k < (kKahan ? 1 : 27); ++k) acc[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < (kKahan ? 27 : 1); ++k) ks[k] = kc[k] = 0.f;
+    auto add = [&](int k, float x) {
+        if constexpr (kKahan) {
+            const float y = x - kc[k];
+            const float t = ks[k] + y;
+            kc[k] = (t - ks[k]) - y;
+            ks[k] = t;
+        } else {
+            acc[k] += (double)x;
+        }
+    };
     float e0 = 0.f, e1 = 0.f, e2 = 0.f, de = 0.f, wc_col = 0.f;
     if (inside) {
         // ---------------- pixel term ----------------
@@ -256,7 +287,7 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
             for (int c = 0; c < 3; ++c) H[r * 3 + c] += wd * ray[r] * ray[c];
             b3[r] += wd * de * ray[r];
         }
-        accumulate_raw(acc, H, b3, img_depth, ray[0], ray[1]);
+        accumulate_raw(add, H, b3, img_depth, ray[0], ray[1]);
     }
 
     // ---------------- pass 2: per-Gaussian terms ----------------
@@ -322,24 +353,50 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
                         b3[r] += wd * de * rg[r];
                     }
                 }
-                accumulate_raw(acc, H, b3, d, rgx, rgy);
+                accumulate_raw(add, H, b3, d, rgx, rgy);
                 if (prod < 0.001f) { active = false; break; }
             }
         }
     }
 
     // ---------------- tile reduction (fixed order) ----------------
+    if constexpr (kKahan) {
+        // pixel (sum, -compensation) pairs, added as pairs; FP64 only from the tile total on
 #pragma unroll
-    for (int k = 0; k < 27; ++k) {
-        double vk = acc[k];
+        for (int k = 0; k < 27; ++k) {
+            float hi = ks[k], lo = -kc[k];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) vk += __shfl_xor(vk, o);
-        if (lane == 0) red[w][k] = vk;
-    }
-    __syncthreads();
-    if (tid < 27) {
-        const double t = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
-        red[0][tid] = t;   // each thread reads and writes only its own column
+            for (int o = 32; o > 0; o >>= 1) {
+                const float bhi = __shfl_xor(hi, o), blo = __shfl_xor(lo, o);
+                pair_add(hi, lo, bhi, blo);
+            }
+            if (lane == 0) {
+                reinterpret_cast<float*>(&red[w][k])[0] = hi;
+                reinterpret_cast<float*>(&red[w][k])[1] = lo;
+            }
+        }
+        __syncthreads();
+        if (tid < 27) {
+            float hi = reinterpret_cast<float*>(&red[0][tid])[0];
+            float lo = reinterpret_cast<float*>(&red[0][tid])[1];
+            for (int ww = 1; ww < 4; ++ww)
+                pair_add(hi, lo, reinterpret_cast<float*>(&red[ww][tid])[0],
+                         reinterpret_cast<float*>(&red[ww][tid])[1]);
+            red[0][tid] = (double)hi + (double)lo;   // only its own column
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 27; ++k) {
+            double vk = acc[k];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) vk += __shfl_xor(vk, o);
+            if (lane == 0) red[w][k] = vk;
+        }
+        __syncthreads();
+        if (tid < 27) {
+            const double t = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+            red[0][tid] = t;   // each thread reads and writes only its own column
+        }
     }
     __syncthreads();
     if (tid == 0) {
@@ -436,8 +493,14 @@ hipError_t launch_pose(const View& v, const View* vd, const uint32_t* ranges,
                        obs_rgb, obs_depth, gx, gy, bp, tiles27, out_rgb, out_depth, pix_c, pix_s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || obs_depth == nullptr) return e;
-    hipLaunchKernelGGL(k_pose<2>, dim3(v.ntx * v.nty), dim3(256), 0, s, v, vd, ranges, idx, rec,
-                       obs_rgb, obs_depth, gx, gy, bp, tiles27, out_rgb, out_depth, pix_c, pix_s);
+    if (bp.kahan)
+        hipLaunchKernelGGL(k_pose<3>, dim3(v.ntx * v.nty), dim3(256), 0, s, v, vd, ranges, idx,
+                           rec, obs_rgb, obs_depth, gx, gy, bp, tiles27, out_rgb, out_depth, pix_c,
+                           pix_s);
+    else
+        hipLaunchKernelGGL(k_pose<2>, dim3(v.ntx * v.nty), dim3(256), 0, s, v, vd, ranges, idx,
+                           rec, obs_rgb, obs_depth, gx, gy, bp, tiles27, out_rgb, out_depth, pix_c,
+                           pix_s);
     return hipGetLastError();
 }
 

@@ -155,3 +155,19 @@ def upstream_grads(width, height, seed=1, sigma=1e-2):
     rng = np.random.default_rng(seed)
     return (rng.standard_normal((height, width, 3)) * sigma).astype(np.float32), \
         (rng.standard_normal((height, width)) * sigma).astype(np.float32)
+
+
+def spatial_order(scene, bits=10):
+    """The map permuted into Morton (Z-curve) order of its positions, as maps grown from
+    keyframes are stored: consecutive Gaussians are close in space, which keeps the per-chunk
+    position offsets of the precision form small. Returns (permuted scene, permutation)."""
+    p = np.asarray(scene["pos"], np.float64)
+    lo, hi = p.min(0), p.max(0)
+    q = np.clip(((p - lo) / np.maximum(hi - lo, 1e-12) * ((1 << bits) - 1)).astype(np.int64), 0,
+                (1 << bits) - 1)
+    code = np.zeros(len(p), np.int64)
+    for b in range(bits):
+        for a in range(3):
+            code |= ((q[:, a] >> b) & 1) << (3 * b + a)
+    perm = np.argsort(code, kind="stable")
+    return {k: np.ascontiguousarray(v[perm]) for k, v in scene.items()}, perm

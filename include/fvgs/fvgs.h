@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define FVGS_ABI_VERSION 2
+#define FVGS_ABI_VERSION 3
 
 /* Status codes. */
 #define FVGS_OK 0
@@ -114,7 +114,14 @@ typedef struct fvgs_params {
                               emit them: their sums and renders do not depend on them */
     int keep_keys;         /* 1: keep the sorted 64-bit keys for fvgs_read(FVGS_OUT_KEYS) */
     size_t mem_budget;     /* 0 = unlimited; otherwise device bytes the context may allocate */
+    int precision;         /* FVGS_PRECISION_F32 (plain form) or FVGS_PRECISION_F16: the map is
+                              stored as FP16 (positions as offsets from an FP32 origin per 256
+                              Gaussians), 28 B per Gaussian, blended in FP32; pose sums are
+                              Kahan-compensated FP32 per pixel and per tile, FP64 across tiles */
 } fvgs_params;
+
+#define FVGS_PRECISION_F32 0
+#define FVGS_PRECISION_F16 1
 
 /* Result / buffer identifiers for fvgs_read and fvgs_device_ptr. */
 enum fvgs_out {
@@ -140,7 +147,9 @@ enum fvgs_out {
     FVGS_OUT_POSE_TILES = 40,  /* double[T][27] per-tile partials of the last pose call, in the
                                   order of fvgs_pose_data (Jtr[6], then the JtJ upper triangle) */
     FVGS_OUT_TRACK_RGB = 41,   /* float3[P]  pass-1 render of the last pose call */
-    FVGS_OUT_TRACK_DEPTH = 42  /* float[P]   pass-1 median depth of the last pose call */
+    FVGS_OUT_TRACK_DEPTH = 42, /* float[P]   pass-1 median depth of the last pose call */
+    FVGS_OUT_MAP = 50          /* float[N][14] the map as the kernels read it (dequantised in the
+                                  precision form): pos3 scale3 quat4 (x,y,z,w) color3 opacity */
 };
 
 int fvgs_default_params(fvgs_params* params);

@@ -70,3 +70,24 @@ def rel_l2(got, ref):
     got = np.asarray(got, np.float64)
     ref = np.asarray(ref, np.float64)
     return float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300))
+
+
+def quantize_reference(scene, chunk=256):
+    """The precision form's storage restated in numpy: per chunk of 256 Gaussians an FP32 origin
+    at the centre of the positions' bounding box, FP16 (round to nearest even) offsets and FP16
+    scale, quaternion, colour and opacity. Returns the dequantised FP32 map as the kernels read
+    it (dict like the scene)."""
+    f32 = np.float32
+    pos = np.asarray(scene["pos"], f32)
+    n = len(pos)
+    out = {}
+    deq = np.empty_like(pos)
+    for c0 in range(0, n, chunk):
+        blk = pos[c0:c0 + chunk]
+        centre = ((blk.min(0) + blk.max(0)) * f32(0.5)).astype(f32)
+        off = (blk - centre).astype(np.float16).astype(f32)
+        deq[c0:c0 + chunk] = centre + off
+    out["pos"] = deq
+    for k in ("scale", "quat", "color", "opacity"):
+        out[k] = np.asarray(scene[k], f32).astype(np.float16).astype(f32)
+    return out

@@ -54,6 +54,7 @@ int main(void) {{
          sizeof(fvgs_intrinsics), sizeof(fvgs_delta2d), sizeof(fvgs_delta3d),
          sizeof(fvgs_pose_data), offsetof(fvgs_params, cull_emit),
          offsetof(fvgs_params, mem_budget), offsetof(fvgs_pose, orientation));
+  printf("%zu\\n", offsetof(fvgs_params, precision));
   return 0;
 }}''')
     exe = tmp_path / "layout"
@@ -62,7 +63,8 @@ int main(void) {{
                                           check=True).stdout.split()]
     want = [ctypes.sizeof(L.Params), ctypes.sizeof(L.Pose), ctypes.sizeof(L.Intrinsics),
             L.DELTA2D_DTYPE.itemsize, L.DELTA3D_DTYPE.itemsize, 108,
-            L.Params.cull_emit.offset, L.Params.mem_budget.offset, L.Pose.orientation.offset]
+            L.Params.cull_emit.offset, L.Params.mem_budget.offset, L.Pose.orientation.offset,
+            L.Params.precision.offset]
     assert got == want
 
 
