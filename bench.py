@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tracking", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--served-sessions", type=int, default=16)
     ap.add_argument("--served-lanes", type=int, default=4)
     ap.add_argument("--served-requests", type=int, default=6,
@@ -325,6 +326,76 @@ def served(args, log_fn):
     return out
 
 
+def precision(args, log_fn):
+    """Precision form (FP16 map storage, FP32 blend, Kahan pose sums) beside the plain form on
+    the same spatially ordered 1M-Gaussian 640x480 view: fwd+bwd step time, a 20-step tracking
+    solve on the 500k tracking view, and the error of image / 3D gradients against the plain
+    FP32 run."""
+    from f_vigs_slam_amd import Context
+    from f_vigs_slam_amd import scenes as S
+    w, h = args.width, args.height
+    scene, _ = S.spatial_order(S.make_scene(args.n, seed=0, width=w, height=h))
+    intr = S.intrinsics(w, h)
+    pose = np.array([0, 0, 0, 0, 0, 0, 1.0])
+    out = {}
+    runs = {}
+    for name, prec in (("plain", 0), ("precision", 1)):
+        ctx = Context(precision=prec)
+        ctx.set_scene(scene)
+        # each form's own render minus the same N(0, 1e-2) noise: equal residuals, so the
+        # gradient difference is the effect of the storage alone
+        obs = mapping_frame(ctx, scene, intr, pose, w, h)
+        ctx.set_frame(*obs)
+        for _ in range(args.warmup):
+            ctx.mapping_step(pose, intr)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.mapping_step(pose, intr)
+        ctx.sync()
+        t = (time.perf_counter() - t0) / args.steps
+        runs[name] = dict(rgb=ctx.read("RGB"), d3=ctx.read("DELTA3D"), t=t)
+        ctx.close()
+    ref, got = runs["plain"], runs["precision"]
+    err_img = np.abs(got["rgb"] - ref["rgb"])
+    grad = {}
+    for f in ("position", "scale", "orientation", "color", "alpha"):
+        a = np.asarray(got["d3"][f], np.float64).reshape(len(scene["pos"]), -1)
+        b = np.asarray(ref["d3"][f], np.float64).reshape(len(scene["pos"]), -1)
+        grad[f] = round(float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)), 6)
+    out["fwd_bwd"] = dict(plain_ms=round(ref["t"] * 1e3, 4), precision_ms=round(got["t"] * 1e3, 4),
+                          precision_mpix_s=round(w * h / got["t"] / 1e6, 2),
+                          map_bytes_per_gaussian={"plain": 56, "precision": 28},
+                          image_abs_err_vs_plain={"median": float(np.median(err_img)),
+                                                  "p99": float(np.quantile(err_img, 0.99))},
+                          grad3d_rel_l2_vs_plain=grad)
+    # 20-step tracking solve (500k, spatially ordered), both forms from the same start
+    tscene, _ = S.spatial_order(S.make_scene(args.track_n, seed=21))
+    gt = np.array([0, 0, 0, 0, 0, 0, 1.0])
+    start = S.perturbed_pose(22, rot_deg=2.0, trans_m=0.03)
+    tr = {}
+    for name, prec in (("plain", 0), ("precision", 1)):
+        ctx = Context(precision=prec)
+        ctx.set_scene(tscene)
+        if name == "plain":
+            ctx.render_tracking(gt, intr, w, h)
+            target = (ctx.read("TRACK_RGB"), ctx.read("TRACK_DEPTH"))
+        ctx.set_frame(*target)
+        ctx.compute_gradients()
+        ctx.track(intr, 3, start)   # warm-up
+        pose_f, ms = ctx.track(intr, args.track_iters, start)
+        tr[name] = (pose_f, ms)
+        ctx.close()
+    rot, trn = S.pose_error(tr["precision"][0], tr["plain"][0])
+    out["tracking"] = dict(plain_step_ms_p50=round(float(np.median(tr["plain"][1])), 4),
+                           precision_step_ms_p50=round(float(np.median(tr["precision"][1])), 4),
+                           pose_diff_vs_plain_rad=rot, pose_diff_vs_plain_m=trn)
+    log_fn(f"precision: fwd+bwd {got['t'] * 1e3:.3f} ms (plain {ref['t'] * 1e3:.3f}); tracking "
+           f"step {out['tracking']['precision_step_ms_p50']} ms (plain "
+           f"{out['tracking']['plain_step_ms_p50']})")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -414,6 +485,11 @@ def main():
             result["tracking"] = tracking(args, log)
         except Exception as e:  # report, do not hide the main number
             result["tracking"] = {"error": str(e)}
+    if not args.no_precision:
+        try:
+            result["precision"] = precision(args, log)
+        except Exception as e:  # report, do not hide the main number
+            result["precision"] = {"error": repr(e)}
     if not args.no_served:
         try:
             result["served"] = served(args, log)
