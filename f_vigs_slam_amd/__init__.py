@@ -1,13 +1,14 @@
 """fvgs — MI355X-native differentiable tile rasterizer and RGB-D pose tracker.
 
 The compute path is libfvgs.so (hand-written HIP for gfx950) behind the C ABI in
-include/fvgs/fvgs.h and include/fvgs/fvgs_server.h; this package holds its Python handles
-(Context, Server) plus the seeded scene generator.
+include/fvgs/fvgs.h, fvgs_server.h and fvgs_window.h; this package holds its Python handles
+(Context, Server, Window) plus the seeded scene generator.
 """
 from . import scenes  # noqa: F401
 from ._lib import LIB_PATH, OUT, load  # noqa: F401
 from .context import Context, FvgsError, default_params, gn_step, make_intr, make_pose  # noqa: F401
 from .server import Server  # noqa: F401
+from .window import Window  # noqa: F401
 
-__all__ = ["Context", "Server", "FvgsError", "default_params", "gn_step", "make_intr", "make_pose",
+__all__ = ["Context", "Server", "Window", "FvgsError", "default_params", "gn_step", "make_intr", "make_pose",
            "scenes", "load", "LIB_PATH", "OUT"]

@@ -42,6 +42,10 @@ EXPORTS = [
     "fvgs_server_last_error", "fvgs_server_open_session", "fvgs_server_close_session",
     "fvgs_server_add_keyframe", "fvgs_server_submit_track", "fvgs_server_submit_mapping",
     "fvgs_server_wait", "fvgs_server_read_gradients", "fvgs_server_memory",
+    # fvgs_window.h
+    "fvgs_window_create", "fvgs_window_destroy", "fvgs_window_last_error",
+    "fvgs_window_set_gaussians", "fvgs_window_set_view", "fvgs_window_run", "fvgs_window_read",
+    "fvgs_window_get_stats",
 ]
 
 
@@ -73,6 +77,13 @@ class RequestResult(ctypes.Structure):
                 ("iters", ctypes.c_int), ("pose7", ctypes.c_double * 7),
                 ("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double),
                 ("device_ms", ctypes.c_double)]
+
+
+class WindowStats(ctypes.Structure):
+    _fields_ = [("concurrent_views", ctypes.c_int), ("wave_rows", ctypes.c_int),
+                ("candidates", ctypes.c_int), ("window_ms", ctypes.c_float),
+                ("best_ms", ctypes.c_float), ("worst_ms", ctypes.c_float),
+                ("instances", ctypes.c_uint64)]
 
 
 DELTA2D_DTYPE = np.dtype([("meanImg", "<f4", 2), ("invSigmaImg", "<f4", 3), ("color", "<f4", 3),
@@ -135,6 +146,14 @@ def load():
         "fvgs_server_read_gradients": ([vp, i, vp, ctypes.c_size_t], i),
         "fvgs_server_memory": ([vp, P(ctypes.c_size_t), P(ctypes.c_size_t),
                                 P(ctypes.c_size_t)], i),
+        "fvgs_window_create": ([P(Params), i, P(vp)], i),
+        "fvgs_window_destroy": ([vp], i),
+        "fvgs_window_last_error": ([vp], ctypes.c_char_p),
+        "fvgs_window_set_gaussians": ([vp, u32, vp, vp, vp, vp, vp, i], i),
+        "fvgs_window_set_view": ([vp, i, i, i, vp, vp, i], i),
+        "fvgs_window_run": ([vp, i, vp, vp], i),
+        "fvgs_window_read": ([vp, vp, ctypes.c_size_t], i),
+        "fvgs_window_get_stats": ([vp, P(WindowStats)], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -15,14 +15,24 @@ constexpr float kAlphaMin = 1.0f / 255.0f;   // GSCudaKernels.cu:2189, 974
 // far above FP32 rounding of either evaluation) is skipped by that wave. Skipped entries leave
 // every pixel of the wave untouched, so results are those of the full walk. NaN bounds keep the
 // entry. Returns bit w set for each quadrant w whose wave must evaluate the entry.
+// Pixel block of wave w (of 4) in a 16x16 tile: QW x (64 / QW) pixels, QW = 8 (2x2 quadrants)
+// or 16 (four row bands).
+template <int QW>
+__device__ __forceinline__ int wave_x0(int w) { return (w % (kTile / QW)) * QW; }
+template <int QW>
+__device__ __forceinline__ int wave_y0(int w) { return (w / (kTile / QW)) * (64 / QW); }
+
+template <int QW = 8>
 __device__ __forceinline__ uint32_t quadrant_mask(const Rec& r, int tx0, int ty0, bool need_pos_q) {
+    constexpr int QH = 64 / QW;
     const float u = r.r0.x, vv = r.r0.y, c0 = r.r0.z, c1x2 = 2.0f * r.r0.w, c2 = r.r1.x;
     const float qcut = 2.0f * __logf(255.0f * r.r2.w) + 1e-3f;
     uint32_t mask = 0;
 #pragma unroll 1
     for (int w = 0; w < 4; ++w) {
-        const float ax0 = (float)(tx0 + 8 * (w & 1)) - u, ax1 = (float)(tx0 + 8 * (w & 1) + 7) - u;
-        const float ay0 = (float)(ty0 + 8 * (w >> 1)) - vv, ay1 = (float)(ty0 + 8 * (w >> 1) + 7) - vv;
+        const int bx = tx0 + wave_x0<QW>(w), by = ty0 + wave_y0<QW>(w);
+        const float ax0 = (float)bx - u, ax1 = (float)(bx + QW - 1) - u;
+        const float ay0 = (float)by - vv, ay1 = (float)(by + QH - 1) - vv;
         const float xa = ax0 * ax0, xb = ax1 * ax1, ya = ay0 * ay0, yb = ay1 * ay1;
         const float xxlo = (ax0 <= 0.0f && ax1 >= 0.0f) ? 0.0f : fminf(xa, xb), xxhi = fmaxf(xa, xb);
         const float yylo = (ay0 <= 0.0f && ay1 >= 0.0f) ? 0.0f : fminf(ya, yb), yyhi = fmaxf(ya, yb);

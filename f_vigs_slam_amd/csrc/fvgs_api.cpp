@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/fvgs/fvgs.h"
+#include "../../include/fvgs/fvgs_window.h"
 #include "fvgs_internal.h"
 #include "gn_step.h"
 
@@ -39,6 +40,8 @@ struct fvgs_ctx {
     size_t lds_limit = 65536;
     size_t allocated = 0;
     bool timing = false;
+    int wave_rows = 0;   // forward wave mapping: 0 = 8x8 quadrants, 1 = 16x4 row bands
+    const View* view_dev = nullptr;   // batched window: pose / intrinsics read from here
     hipEvent_t ev[FVGS_TIMING_STAGES + 1] = {};
     float last_ms[FVGS_TIMING_STAGES] = {};
 
@@ -245,7 +248,7 @@ int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int
     hipStream_t s = c->stream;
     c->view = v;
     mark(c, 0);
-    HIPCHK(c, launch_project(map_ref(c), n, v, nullptr, cull_emit, as<Rec>(c->rec),
+    HIPCHK(c, launch_project(map_ref(c), n, v, c->view_dev, cull_emit, as<Rec>(c->rec),
                              as<uint2>(c->rect),
                              as<uint32_t>(c->blk_counts), s));
     HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->chunk_pre), nblk, T,
@@ -260,7 +263,7 @@ int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int
     }
     uint64_t* keys = c->prm.keep_keys ? as<uint64_t>(c->keys) : nullptr;
     HIPCHK(c, launch_emit(as<Rec>(c->rec), as<uint2>(c->rect), as<uint32_t>(c->blk_counts),
-                          as<uint32_t>(c->chunk_pre), n, v, nullptr, as<uint64_t>(c->pairs),
+                          as<uint32_t>(c->chunk_pre), n, v, c->view_dev, as<uint64_t>(c->pairs),
                           c->inst_cap,
                           s));
     HIPCHK(c, launch_tile_sort(as<uint32_t>(c->ranges), T, n, as<uint64_t>(c->pairs),
@@ -314,7 +317,7 @@ int run_forward(fvgs_ctx* c) {
                              as<Rec>(c->rec), c->obs_rgb, blend_params(c->prm),
                              as<float>(c->o_rgb), as<float>(c->o_depth), as<float>(c->o_T),
                              as<uint32_t>(c->o_nc), as<float>(c->o_err), as<uint32_t>(c->o_maxc),
-                             as<float4>(c->ckpt), c->stream));
+                             as<float4>(c->ckpt), c->wave_rows, c->stream));
     c->forwarded = true;
     c->backwarded = false;
     return FVGS_OK;
@@ -324,7 +327,7 @@ int run_forward(fvgs_ctx* c) {
 // clear) and runs the backward over the forward's outputs.
 int clear_delta2d(fvgs_ctx* c) {
     TRY(ensure(c, c->d2, sizeof(Delta2D) * (size_t)c->n));
-    HIPCHK(c, hipMemsetAsync(c->d2.p, 0, sizeof(Delta2D) * (size_t)c->n, c->stream));
+    HIPCHK(c, launch_zero(c->d2.p, sizeof(Delta2D) * (size_t)c->n, c->stream));
     return FVGS_OK;
 }
 
@@ -342,7 +345,7 @@ int run_backward(fvgs_ctx* c) {
 int run_delta3d(fvgs_ctx* c) {
     if (!c->backwarded) return fail(c, FVGS_E_STATE, "call fvgs_backward first");
     TRY(ensure(c, c->d3, sizeof(Delta3D) * (size_t)c->n));
-    HIPCHK(c, launch_delta3d(map_ref(c), (int)c->n, c->view, c->prm.lambda_iso,
+    HIPCHK(c, launch_delta3d(map_ref(c), (int)c->n, c->view, c->view_dev, c->prm.lambda_iso,
                              as<Delta2D>(c->d2), as<Delta3D>(c->d3), c->stream));
     return FVGS_OK;
 }
@@ -933,6 +936,395 @@ int fvgs_last_timing(fvgs_ctx* c, float ms[FVGS_TIMING_STAGES]) {
     if (!c || !ms) return FVGS_E_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
     for (int i = 0; i < FVGS_TIMING_STAGES; i++) ms[i] = c->last_ms[i];
+    return FVGS_OK;
+}
+
+}  // extern "C"
+
+// =============================================================================================
+// Batched mapping window (include/fvgs/fvgs_window.h).
+// =============================================================================================
+
+struct fvgs_window {
+    fvgs_params prm{};
+    int max_views = 0;
+    std::vector<fvgs_ctx*> views;         // one context (buffers + stream) per view slot
+    hipStream_t master = nullptr;
+    std::string err;
+    // shared map
+    uint32_t n = 0;
+    void* map[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool map_bound = false;
+    // per-slot observations
+    std::vector<void*> rgb, depth;
+    std::vector<int> fw, fh;
+    // device-resident views, summed gradients, pointer table of the views' gradients
+    View* views_dev = nullptr;
+    View* views_host = nullptr;           // pinned
+    Delta3D* sum = nullptr;
+    const Delta3D** d3_table = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
+    std::vector<hipEvent_t> ev_done;
+    // captured schedule of the current shape
+    std::string shape;
+    hipGraphExec_t exec = nullptr;
+    fvgs_window_stats st{};
+};
+
+namespace {
+
+int wfail(fvgs_window* w, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (w) w->err = buf;
+    return code;
+}
+
+#define WCHK(w, expr)                                                                     \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            return wfail((w), FVGS_E_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+#define WTRY(w, c, x)                                          \
+    do {                                                       \
+        int _r = (x);                                          \
+        if (_r != FVGS_OK) return wfail((w), _r, "%s", (c)->err.c_str()); \
+    } while (0)
+
+// One mapping view enqueued on the view's stream with no host wait: binning with the current
+// instance capacity (the scan empties every range if it overflows; the caller checks M after
+// synchronising), clear, forward, backward, 2D->3D.
+int enqueue_mapping(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr) {
+    TRY(prepare(c, pose, intr, c->fw, c->fh, true, c->prm.cull_emit));
+    TRY(clear_delta2d(c));
+    TRY(run_forward(c));
+    TRY(run_backward(c));
+    TRY(run_delta3d(c));
+    return FVGS_OK;
+}
+
+void drop_graph(fvgs_window* w) {
+    if (w->exec) (void)hipGraphExecDestroy(w->exec);
+    w->exec = nullptr;
+    w->shape.clear();
+}
+
+// Captures the window for `nv` views with at most `conc` views in flight and the given forward
+// wave mapping.
+int capture_window(fvgs_window* w, int nv, const fvgs_pose* poses, const fvgs_intrinsics* intrs,
+                   int conc, int wave_rows, hipGraphExec_t* out) {
+    for (int v = 0; v < nv; v++) w->views[(size_t)v]->wave_rows = wave_rows;
+    WCHK(w, hipStreamBeginCapture(w->master, hipStreamCaptureModeRelaxed));
+    int rc = FVGS_OK;
+    hipError_t e = hipEventRecord(w->ev_fork, w->master);
+    for (int v = 0; v < nv && rc == FVGS_OK && e == hipSuccess; v++) {
+        fvgs_ctx* c = w->views[(size_t)v];
+        e = hipStreamWaitEvent(c->stream, w->ev_fork, 0);
+        if (e == hipSuccess && v >= conc)
+            e = hipStreamWaitEvent(c->stream, w->ev_done[(size_t)(v - conc)], 0);
+        if (e == hipSuccess) {
+            std::lock_guard<std::mutex> lk(c->mu);
+            rc = enqueue_mapping(c, &poses[v], &intrs[v]);
+            if (rc != FVGS_OK) w->err = c->err;
+        }
+        if (e == hipSuccess && rc == FVGS_OK) e = hipEventRecord(w->ev_done[(size_t)v], c->stream);
+    }
+    for (int v = 0; v < nv && e == hipSuccess && rc == FVGS_OK; v++)
+        e = hipStreamWaitEvent(w->master, w->ev_done[(size_t)v], 0);
+    if (e == hipSuccess && rc == FVGS_OK)
+        e = launch_sum_delta3d(w->d3_table, nv, w->sum, (int)w->n, w->master);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(w->master, &g);
+    if (rc != FVGS_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return wfail(w, FVGS_E_HIP, "window capture failed: %s",
+                     hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return wfail(w, FVGS_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    return FVGS_OK;
+}
+
+// Replays `exec` and returns its device time in ms.
+int replay(fvgs_window* w, hipGraphExec_t exec, float* ms) {
+    WCHK(w, hipEventRecord(w->ev_t0, w->master));
+    WCHK(w, hipGraphLaunch(exec, w->master));
+    WCHK(w, hipEventRecord(w->ev_t1, w->master));
+    WCHK(w, hipStreamSynchronize(w->master));
+    if (ms) WCHK(w, hipEventElapsedTime(ms, w->ev_t0, w->ev_t1));
+    return FVGS_OK;
+}
+
+// First run of a shape: plain (synchronous) mapping steps size every view's buffers and learn
+// its M; then each candidate schedule is captured and timed and the fastest kept.
+int build_window(fvgs_window* w, int nv, const fvgs_pose* poses, const fvgs_intrinsics* intrs) {
+    drop_graph(w);
+    std::vector<const Delta3D*> table((size_t)nv);
+    for (int v = 0; v < nv; v++) {
+        fvgs_ctx* c = w->views[(size_t)v];
+        c->view_dev = nullptr;
+        WTRY(w, c, fvgs_mapping_step(c, &poses[v], &intrs[v]));
+        {
+            std::lock_guard<std::mutex> lk(c->mu);
+            WTRY(w, c, ensure_instances(c, (uint64_t)c->m + c->m / 2));
+            table[(size_t)v] = as<Delta3D>(c->d3);
+        }
+        c->view_dev = w->views_dev + v;
+    }
+    WCHK(w, hipMemcpy(w->d3_table, table.data(), sizeof(void*) * (size_t)nv,
+                      hipMemcpyHostToDevice));
+    std::vector<int> concs;
+    for (int k : {1, 2, 4, nv})
+        if (k <= nv && std::find(concs.begin(), concs.end(), k) == concs.end()) concs.push_back(k);
+    float best = 1e30f, worst = 0.f;
+    int cands = 0;
+    for (int conc : concs)
+        for (int rows = 0; rows < 2; rows++) {
+            hipGraphExec_t ex = nullptr;
+            int r = capture_window(w, nv, poses, intrs, conc, rows, &ex);
+            if (r != FVGS_OK) {
+                if (w->exec) (void)hipGraphExecDestroy(w->exec);
+                w->exec = nullptr;
+                return r;
+            }
+            float ms = 0.f;
+            r = replay(w, ex, nullptr);   // warm
+            if (r == FVGS_OK) r = replay(w, ex, &ms);
+            if (r != FVGS_OK) {
+                (void)hipGraphExecDestroy(ex);
+                return r;
+            }
+            cands++;
+            worst = std::max(worst, ms);
+            if (ms < best) {
+                best = ms;
+                if (w->exec) (void)hipGraphExecDestroy(w->exec);
+                w->exec = ex;
+                w->st.concurrent_views = conc;
+                w->st.wave_rows = rows;
+            } else {
+                (void)hipGraphExecDestroy(ex);
+            }
+        }
+    w->st.candidates = cands;
+    w->st.best_ms = best;
+    w->st.worst_ms = worst;
+    return FVGS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvgs_window_create(const fvgs_params* params, int max_views, fvgs_window** out) {
+    if (!out || max_views < 1 || max_views > 64) return FVGS_E_ARG;
+    *out = nullptr;
+    auto* w = new (std::nothrow) fvgs_window();
+    if (!w) return FVGS_E_OOM;
+    if (params) w->prm = *params;
+    else fvgs_default_params(&w->prm);
+    w->max_views = max_views;
+    int r = FVGS_OK;
+    for (int v = 0; v < max_views && r == FVGS_OK; v++) {
+        fvgs_ctx* c = nullptr;
+        r = fvgs_create(&w->prm, &c);
+        if (r == FVGS_OK) w->views.push_back(c);
+    }
+    w->rgb.assign((size_t)max_views, nullptr);
+    w->depth.assign((size_t)max_views, nullptr);
+    w->fw.assign((size_t)max_views, 0);
+    w->fh.assign((size_t)max_views, 0);
+    w->ev_done.assign((size_t)max_views, nullptr);
+    if (r == FVGS_OK && (hipStreamCreate(&w->master) != hipSuccess ||
+                         hipMalloc((void**)&w->views_dev, sizeof(View) * max_views) != hipSuccess ||
+                         hipHostMalloc((void**)&w->views_host, sizeof(View) * max_views,
+                                       hipHostMallocDefault) != hipSuccess ||
+                         hipMalloc((void**)&w->d3_table, sizeof(void*) * max_views) != hipSuccess ||
+                         hipEventCreateWithFlags(&w->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                         hipEventCreate(&w->ev_t0) != hipSuccess ||
+                         hipEventCreate(&w->ev_t1) != hipSuccess))
+        r = FVGS_E_HIP;
+    for (int v = 0; v < max_views && r == FVGS_OK; v++)
+        if (hipEventCreateWithFlags(&w->ev_done[(size_t)v], hipEventDisableTiming) != hipSuccess)
+            r = FVGS_E_HIP;
+    if (r != FVGS_OK) {
+        (void)hipGetLastError();
+        fvgs_window_destroy(w);
+        return r;
+    }
+    *out = w;
+    return FVGS_OK;
+}
+
+int fvgs_window_destroy(fvgs_window* w) {
+    if (!w) return FVGS_E_ARG;
+    if (w->master) (void)hipStreamSynchronize(w->master);
+    drop_graph(w);
+    for (fvgs_ctx* c : w->views) fvgs_destroy(c);
+    for (void* p : w->map) if (p) (void)hipFree(p);
+    for (void* p : w->rgb) if (p) (void)hipFree(p);
+    for (void* p : w->depth) if (p) (void)hipFree(p);
+    if (w->views_dev) (void)hipFree(w->views_dev);
+    if (w->views_host) (void)hipHostFree(w->views_host);
+    if (w->sum) (void)hipFree(w->sum);
+    if (w->d3_table) (void)hipFree(w->d3_table);
+    for (hipEvent_t e : w->ev_done) if (e) (void)hipEventDestroy(e);
+    if (w->ev_fork) (void)hipEventDestroy(w->ev_fork);
+    if (w->ev_t0) (void)hipEventDestroy(w->ev_t0);
+    if (w->ev_t1) (void)hipEventDestroy(w->ev_t1);
+    if (w->master) (void)hipStreamDestroy(w->master);
+    delete w;
+    return FVGS_OK;
+}
+
+const char* fvgs_window_last_error(const fvgs_window* w) { return w ? w->err.c_str() : "null window"; }
+
+int fvgs_window_set_gaussians(fvgs_window* w, uint32_t n, const float* pos3, const float* scale3,
+                              const float* quat4, const float* color3, const float* opacity,
+                              int mem) {
+    if (!w || n == 0 || !pos3 || !scale3 || !quat4 || !color3 || !opacity) return FVGS_E_ARG;
+    if (mem != FVGS_MEM_HOST && mem != FVGS_MEM_DEVICE) return FVGS_E_ARG;
+    drop_graph(w);
+    for (void*& p : w->map) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
+    if (w->sum) (void)hipFree(w->sum);
+    w->sum = nullptr;
+    const float* src[5] = {pos3, scale3, quat4, color3, opacity};
+    const size_t bytes[5] = {12ull * n, 12ull * n, 16ull * n, 12ull * n, 4ull * n};
+    const hipMemcpyKind kind = mem == FVGS_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    for (int k = 0; k < 5; k++) {
+        if (hipMalloc(&w->map[k], bytes[k]) != hipSuccess) {
+            (void)hipGetLastError();
+            return wfail(w, FVGS_E_OOM, "map allocation failed");
+        }
+        WCHK(w, hipMemcpy(w->map[k], src[k], bytes[k], kind));
+    }
+    if (hipMalloc((void**)&w->sum, sizeof(Delta3D) * n) != hipSuccess) {
+        (void)hipGetLastError();
+        return wfail(w, FVGS_E_OOM, "gradient allocation failed");
+    }
+    w->n = n;
+    for (fvgs_ctx* c : w->views) {
+        int r = fvgs_set_gaussians(c, n, static_cast<float*>(w->map[0]),
+                                   static_cast<float*>(w->map[1]), static_cast<float*>(w->map[2]),
+                                   static_cast<float*>(w->map[3]), static_cast<float*>(w->map[4]),
+                                   FVGS_MEM_DEVICE);
+        if (r != FVGS_OK) return wfail(w, r, "%s", c->err.c_str());
+    }
+    return FVGS_OK;
+}
+
+int fvgs_window_set_view(fvgs_window* w, int slot, int width, int height, const float* rgb3,
+                         const float* depth, int mem) {
+    if (!w || slot < 0 || slot >= w->max_views || width <= 0 || height <= 0 || !rgb3 || !depth)
+        return FVGS_E_ARG;
+    if (mem != FVGS_MEM_HOST && mem != FVGS_MEM_DEVICE) return FVGS_E_ARG;
+    const size_t P = (size_t)width * height;
+    const size_t s = (size_t)slot;
+    if (w->fw[s] != width || w->fh[s] != height) {
+        drop_graph(w);   // buffer addresses change: recapture
+        if (w->rgb[s]) (void)hipFree(w->rgb[s]);
+        if (w->depth[s]) (void)hipFree(w->depth[s]);
+        w->rgb[s] = w->depth[s] = nullptr;
+        if (hipMalloc(&w->rgb[s], 12 * P) != hipSuccess ||
+            hipMalloc(&w->depth[s], 4 * P) != hipSuccess) {
+            (void)hipGetLastError();
+            return wfail(w, FVGS_E_OOM, "view allocation failed");
+        }
+        w->fw[s] = width;
+        w->fh[s] = height;
+        fvgs_ctx* c = w->views[s];
+        int r = fvgs_set_frame(c, width, height, static_cast<float*>(w->rgb[s]),
+                               static_cast<float*>(w->depth[s]), nullptr, nullptr, FVGS_MEM_DEVICE);
+        if (r != FVGS_OK) return wfail(w, r, "%s", c->err.c_str());
+    }
+    const hipMemcpyKind kind = mem == FVGS_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    WCHK(w, hipMemcpy(w->rgb[s], rgb3, 12 * P, kind));
+    WCHK(w, hipMemcpy(w->depth[s], depth, 4 * P, kind));
+    return FVGS_OK;
+}
+
+int fvgs_window_run(fvgs_window* w, int nv, const fvgs_pose* poses, const fvgs_intrinsics* intrs) {
+    if (!w || !poses || !intrs || nv < 1 || nv > w->max_views) return FVGS_E_ARG;
+    if (w->n == 0) return wfail(w, FVGS_E_STATE, "no map");
+    char key[64];
+    std::string shape = std::to_string(nv) + ":" + std::to_string(w->n);
+    for (int v = 0; v < nv; v++) {
+        if (!w->rgb[(size_t)v]) return wfail(w, FVGS_E_STATE, "view %d has no observation", v);
+        if (!finite_pose(&poses[v]) || !(intrs[v].f[0] > 0.f) || !(intrs[v].f[1] > 0.f))
+            return wfail(w, FVGS_E_ARG, "bad pose or intrinsics for view %d", v);
+        snprintf(key, sizeof key, ":%dx%d", w->fw[(size_t)v], w->fh[(size_t)v]);
+        shape += key;
+    }
+    // the views' poses and intrinsics, read by the captured kernels from device memory
+    for (int v = 0; v < nv; v++) {
+        View& x = w->views_host[v];
+        x.px = poses[v].position[0]; x.py = poses[v].position[1]; x.pz = poses[v].position[2];
+        x.qx = poses[v].orientation[0]; x.qy = poses[v].orientation[1];
+        x.qz = poses[v].orientation[2]; x.qw = poses[v].orientation[3];
+        x.fx = intrs[v].f[0]; x.fy = intrs[v].f[1]; x.cx = intrs[v].c[0]; x.cy = intrs[v].c[1];
+        x.W = w->fw[(size_t)v]; x.H = w->fh[(size_t)v];
+        x.ntx = (x.W + kTile - 1) / kTile; x.nty = (x.H + kTile - 1) / kTile;
+    }
+    WCHK(w, hipMemcpyAsync(w->views_dev, w->views_host, sizeof(View) * nv, hipMemcpyHostToDevice,
+                           w->master));
+    WCHK(w, hipStreamSynchronize(w->master));
+    for (int attempt = 0; attempt < 3; attempt++) {
+        if (shape != w->shape || !w->exec) {
+            int r = build_window(w, nv, poses, intrs);
+            if (r != FVGS_OK) return r;
+            w->shape = shape;
+        }
+        float ms = 0.f;
+        int r = replay(w, w->exec, &ms);
+        if (r != FVGS_OK) return r;
+        bool overflow = false;
+        uint64_t inst = 0;
+        for (int v = 0; v < nv; v++) {
+            fvgs_ctx* c = w->views[(size_t)v];
+            const uint32_t m = *c->h_m;
+            inst += m;
+            if (m > c->inst_cap) {
+                overflow = true;
+                std::lock_guard<std::mutex> lk(c->mu);
+                WTRY(w, c, ensure_instances(c, (uint64_t)m + m / 2));
+            }
+            c->m = m;
+        }
+        if (!overflow) {
+            w->st.window_ms = ms;
+            w->st.instances = inst;
+            return FVGS_OK;
+        }
+        drop_graph(w);   // buffers grew: recapture for the new addresses
+    }
+    return wfail(w, FVGS_E_STATE, "instance buffers kept overflowing");
+}
+
+int fvgs_window_read(fvgs_window* w, fvgs_delta3d* host, size_t bytes) {
+    if (!w || !host) return FVGS_E_ARG;
+    if (!w->sum) return wfail(w, FVGS_E_STATE, "no map");
+    WCHK(w, hipStreamSynchronize(w->master));
+    WCHK(w, hipMemcpy(host, w->sum, std::min(bytes, sizeof(Delta3D) * (size_t)w->n),
+                      hipMemcpyDeviceToHost));
+    return FVGS_OK;
+}
+
+int fvgs_window_get_stats(fvgs_window* w, fvgs_window_stats* out) {
+    if (!w || !out) return FVGS_E_ARG;
+    *out = w->st;
     return FVGS_OK;
 }
 

@@ -111,14 +111,18 @@ hipError_t launch_render(const View& v, const uint32_t* ranges, const uint32_t* 
 hipError_t launch_forward(const View& v, const uint32_t* ranges, const uint32_t* idx,
                           const Rec* rec, const float* obs_rgb, const BlendParams& bp, float* rgb,
                           float* depth, float* final_T, uint32_t* n_contrib, float* color_err,
-                          uint32_t* max_contrib, float4* ckpt, hipStream_t s);
+                          uint32_t* max_contrib, float4* ckpt, int wave_rows, hipStream_t s);
 hipError_t launch_backward(const View& v, const uint32_t* ranges, const uint32_t* idx,
                            const Rec* rec, const float* obs_depth, const BlendParams& bp,
                            const float* rgb, const float* depth, const uint32_t* n_contrib,
                            const float* color_err, const uint32_t* max_contrib,
                            const float4* ckpt, Delta2D* delta, hipStream_t s);
-hipError_t launch_delta3d(const MapRef& map, int n, const View& v, float lambda_iso,
-                          const Delta2D* d2, Delta3D* d3, hipStream_t s);
+hipError_t launch_delta3d(const MapRef& map, int n, const View& v, const View* vd,
+                          float lambda_iso, const Delta2D* d2, Delta3D* d3, hipStream_t s);
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+// dst = srcs[0] + ... + srcs[nsrc-1] (device array of device pointers)
+hipError_t launch_sum_delta3d(const Delta3D* const* srcs, int nsrc, Delta3D* dst, int n,
+                              hipStream_t s);
 
 hipError_t launch_accum_delta3d(const Delta3D* src, Delta3D* dst, int n, hipStream_t s);
 

@@ -9,6 +9,8 @@
 // the backward can use. k_backward: the per-bucket replay of the reference as a wave64 lane
 // pipeline.
 // Reference defects Q8, Q9, Q10 and Q14 are transcribed (see the oracle header).
+#include <algorithm>
+
 #include "fvgs_internal.h"
 #include "blend_common.h"
 
@@ -89,6 +91,7 @@ __global__ __launch_bounds__(256) void k_render(View v, const uint32_t* __restri
 // to HBM as float4 per (tile, bucket, pixel) — only the first 256 entries of a tile ever get
 // gradients (GSCudaKernels.cu:2329-2330), so no other bucket needs one. A checkpoint is written
 // when the walk reaches or passes its index; skipped entries do not change the state.
+template <int QW>
 __global__ __launch_bounds__(256) void k_forward(
     View v, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ idx,
     const Rec* __restrict__ rec, const float* __restrict__ obs_rgb, BlendParams bp,
@@ -101,7 +104,7 @@ __global__ __launch_bounds__(256) void k_forward(
     __shared__ uint32_t smax;
     const int tile = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int tx0 = (tile % v.ntx) * kTile, ty0 = (tile / v.ntx) * kTile;
-    const int lx = 8 * (w & 1) + (lane & 7), ly = 8 * (w >> 1) + (lane >> 3);
+    const int lx = wave_x0<QW>(w) + (lane % QW), ly = wave_y0<QW>(w) + lane / QW;
     const int x = tx0 + lx, y = ty0 + ly, pix = ly * kTile + lx;
     const bool inside = x < v.W && y < v.H;
     const uint32_t s = ranges[2 * tile], n = ranges[2 * tile + 1] - s;
@@ -134,7 +137,7 @@ __global__ __launch_bounds__(256) void k_forward(
         if (tid < cnt) {
             const Rec r = rec[idx[s + base + tid]];
             recs[tid] = r;
-            qmask[tid] = (uint8_t)quadrant_mask(r, tx0, ty0, true);
+            qmask[tid] = (uint8_t)quadrant_mask<QW>(r, tx0, ty0, true);
         }
         __syncthreads();
         int wc = 0;
@@ -352,9 +355,11 @@ __device__ __forceinline__ void qmat(float x, float y, float z, float w, float R
     R[2][2] = 1.0f - 2.0f * (x * x + y * y);
 }
 
-__global__ __launch_bounds__(256) void k_delta3d(MapRef map, int n, View v, float lambda_iso,
+__global__ __launch_bounds__(256) void k_delta3d(MapRef map, int n, View vh,
+                                                 const View* __restrict__ vd, float lambda_iso,
                                                  const Delta2D* __restrict__ d2in,
                                                  Delta3D* __restrict__ out) {
+    const View v = vd ? *vd : vh;
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const Delta2D d2 = d2in[i];
@@ -575,6 +580,34 @@ __global__ __launch_bounds__(256) void k_accum_delta3d(const Delta3D* __restrict
     dst[g] = b;
 }
 
+// Zero fill with 16-byte stores (used instead of hipMemsetAsync so that the clears can sit in
+// captured graphs).
+__global__ __launch_bounds__(256) void k_zero16(uint4* __restrict__ p, size_t n16) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (size_t)gridDim.x * 256)
+        p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+__global__ void k_zero_bytes(uint8_t* __restrict__ p, size_t n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// dst = sum over the views' 3D gradients, views in order (fixed, deterministic).
+__global__ __launch_bounds__(256) void k_sum_delta3d(const Delta3D* const* __restrict__ srcs,
+                                                     int nsrc, Delta3D* __restrict__ dst, int n) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= n) return;
+    Delta3D b = srcs[0][g];
+    for (int k = 1; k < nsrc; ++k) {
+        const Delta3D a = srcs[k][g];
+        b.px += a.px; b.py += a.py; b.pz += a.pz;
+        b.sx += a.sx; b.sy += a.sy; b.sz += a.sz;
+        b.ox += a.ox; b.oy += a.oy; b.oz += a.oz;
+        b.cr += a.cr; b.cg += a.cg; b.cb += a.cb;
+        b.alpha += a.alpha;
+        b.n += a.n;
+    }
+    dst[g] = b;
+}
+
 }  // namespace
 
 hipError_t launch_render(const View& v, const uint32_t* ranges, const uint32_t* idx,
@@ -587,9 +620,15 @@ hipError_t launch_render(const View& v, const uint32_t* ranges, const uint32_t* 
 hipError_t launch_forward(const View& v, const uint32_t* ranges, const uint32_t* idx,
                           const Rec* rec, const float* obs_rgb, const BlendParams& bp, float* rgb,
                           float* depth, float* final_T, uint32_t* n_contrib, float* color_err,
-                          uint32_t* max_contrib, float4* ckpt, hipStream_t s) {
-    hipLaunchKernelGGL(k_forward, dim3(v.ntx * v.nty), dim3(256), 0, s, v, ranges, idx, rec,
-                       obs_rgb, bp, rgb, depth, final_T, n_contrib, color_err, max_contrib, ckpt);
+                          uint32_t* max_contrib, float4* ckpt, int wave_rows, hipStream_t s) {
+    if (wave_rows)   // four 16x4 row bands per tile
+        hipLaunchKernelGGL(k_forward<16>, dim3(v.ntx * v.nty), dim3(256), 0, s, v, ranges, idx,
+                           rec, obs_rgb, bp, rgb, depth, final_T, n_contrib, color_err,
+                           max_contrib, ckpt);
+    else             // 2x2 quadrants of 8x8
+        hipLaunchKernelGGL(k_forward<8>, dim3(v.ntx * v.nty), dim3(256), 0, s, v, ranges, idx,
+                           rec, obs_rgb, bp, rgb, depth, final_T, n_contrib, color_err,
+                           max_contrib, ckpt);
     return hipGetLastError();
 }
 
@@ -603,10 +642,28 @@ hipError_t launch_backward(const View& v, const uint32_t* ranges, const uint32_t
     return hipGetLastError();
 }
 
-hipError_t launch_delta3d(const MapRef& map, int n, const View& v, float lambda_iso,
-                          const Delta2D* d2, Delta3D* d3, hipStream_t s) {
-    hipLaunchKernelGGL(k_delta3d, dim3((n + 255) / 256), dim3(256), 0, s, map, n, v, lambda_iso,
-                       d2, d3);
+hipError_t launch_delta3d(const MapRef& map, int n, const View& v, const View* vd,
+                          float lambda_iso, const Delta2D* d2, Delta3D* d3, hipStream_t s) {
+    hipLaunchKernelGGL(k_delta3d, dim3((n + 255) / 256), dim3(256), 0, s, map, n, v, vd,
+                       lambda_iso, d2, d3);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
+    const size_t n16 = bytes / 16;
+    if (n16)
+        hipLaunchKernelGGL(k_zero16, dim3((unsigned)std::min<size_t>((n16 + 255) / 256, 65535)),
+                           dim3(256), 0, s, static_cast<uint4*>(p), n16);
+    if (bytes % 16)
+        hipLaunchKernelGGL(k_zero_bytes, dim3(1), dim3(64), 0, s, static_cast<uint8_t*>(p) + n16 * 16,
+                           bytes % 16);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_delta3d(const Delta3D* const* srcs, int nsrc, Delta3D* dst, int n,
+                              hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sum_delta3d, dim3((n + 255) / 256), dim3(256), 0, s, srcs, nsrc, dst, n);
     return hipGetLastError();
 }
 

@@ -1,0 +1,63 @@
+/* fvgs_window: a batched mapping window — forward, backward and 2D->3D gradients of several
+ * keyframe views of one shared map, with the per-Gaussian 3D gradients summed over the views.
+ *
+ * The reference runs GSSlam::optimizeGaussiansKeyframe once per keyframe with host syncs in
+ * between (GSSlam.cu:1020-1081, 1083-1264); this entry point has no direct counterpart and is
+ * for callers that optimise over a window of keyframes. Every view slot owns its buffers and
+ * stream; the whole window (all views' projection, binning, blend forward / backward, 2D->3D,
+ * and the sum over views) is captured once per window shape into a hipGraph and replayed, with
+ * each view's pose and intrinsics read from device memory. On the first run of a shape an
+ * autotuner times the candidate schedules — how many views run concurrently (1, 2, 4 or all) and
+ * the forward's wave mapping (8x8 quadrants or 16x4 row bands) — and keeps the fastest. Every
+ * candidate gives the same result up to the order of the backward's float atomics. The tile
+ * size stays 16x16: tile lists and the first-256-entries rule of the backward depend on it.
+ *
+ * All functions return FVGS_OK (0) or a negative FVGS_E_* code (fvgs.h).
+ */
+#ifndef FVGS_WINDOW_H
+#define FVGS_WINDOW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "fvgs.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fvgs_window fvgs_window;
+
+typedef struct fvgs_window_stats {
+    int concurrent_views;  /* chosen schedule: views in flight at once */
+    int wave_rows;         /* chosen forward wave mapping: 0 = 8x8 quadrants, 1 = 16x4 rows */
+    int candidates;        /* schedules timed for this shape */
+    float window_ms;       /* device time of the last window run (graph replay) */
+    float best_ms;         /* autotuner's time of the chosen schedule */
+    float worst_ms;        /* autotuner's time of the slowest candidate */
+    uint64_t instances;    /* tile instances over all views of the last run */
+} fvgs_window_stats;
+
+int fvgs_window_create(const fvgs_params* params, int max_views, fvgs_window** out);
+int fvgs_window_destroy(fvgs_window* win);
+const char* fvgs_window_last_error(const fvgs_window* win);
+
+/* The shared map (copied into window-owned device memory; host or device arrays). */
+int fvgs_window_set_gaussians(fvgs_window* win, uint32_t n, const float* pos3, const float* scale3,
+                              const float* quat4, const float* color3, const float* opacity,
+                              int mem);
+/* Observation of view `slot` (0 <= slot < max_views): colour and depth, copied to the device. */
+int fvgs_window_set_view(fvgs_window* win, int slot, int width, int height, const float* rgb3,
+                         const float* depth, int mem);
+/* Runs views 0 .. n_views-1 at the given poses / intrinsics and sums their 3D gradients. */
+int fvgs_window_run(fvgs_window* win, int n_views, const fvgs_pose* poses,
+                    const fvgs_intrinsics* intrinsics);
+/* The summed 3D gradients of the last run (fvgs_delta3d[n]) into host memory. */
+int fvgs_window_read(fvgs_window* win, fvgs_delta3d* host, size_t bytes);
+int fvgs_window_get_stats(fvgs_window* win, fvgs_window_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FVGS_WINDOW_H */

@@ -15,7 +15,8 @@ from f_vigs_slam_amd import context as C
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "fvgs", "fvgs.h")
-HEADERS = [HEADER, os.path.join(ROOT, "include", "fvgs", "fvgs_server.h")]
+HEADERS = [HEADER, os.path.join(ROOT, "include", "fvgs", "fvgs_server.h"),
+           os.path.join(ROOT, "include", "fvgs", "fvgs_window.h")]
 
 
 def declared_functions():
