@@ -45,7 +45,9 @@ __device__ __forceinline__ uint32_t quadrant_mask(const Rec& r, int tx0, int ty0
         const float qlo = t0lo + t1lo + t2lo, qhi = t0hi + t1hi + t2hi;
         const float slack = 1e-5f * (fabsf(t0lo) + fabsf(t0hi) + fabsf(t1lo) + fabsf(t1hi) +
                                      fabsf(t2lo) + fabsf(t2hi)) + 1e-30f;
-        const bool skip = (need_pos_q && qhi + slack <= 0.0f) || (qlo - slack > qcut);
+        // A zero form (the culled Gaussians' zero record) gives q = 0 at every pixel exactly.
+        const bool zero_form = c0 == 0.0f && c1x2 == 0.0f && c2 == 0.0f;
+        const bool skip = (need_pos_q && (zero_form || qhi + slack <= 0.0f)) || (qlo - slack > qcut);
         if (!skip) mask |= 1u << w;
     }
     return mask;
