@@ -144,6 +144,10 @@ __device__ __forceinline__ uint2 tile_rect_exact(float u, float v, float cxx, fl
 }
 
 // Depth at the tile centre (GSCudaKernels.cu:643-654), raw float bits.
+// Rectangle marker of a culled Gaussian's zero-segment instance (rows 0..0xffff cannot occur:
+// a view has at most kMaxTiles tiles).
+constexpr uint32_t kCulledY = 0xffff0000u;
+
 __device__ __forceinline__ uint32_t tile_depth_bits(float u, float v, float z, float phx,
                                                     float phy, int tx, int ty) {
 #pragma clang fp contract(off)
@@ -157,11 +161,12 @@ __device__ __forceinline__ uint32_t tile_depth_bits(float u, float v, float z, f
 
 __global__ __launch_bounds__(kBinThreads) void k_project(
     MapRef map, int n, View vh, const View* __restrict__ vd, int cull_emit,
-    Rec* __restrict__ rec, uint2* __restrict__ rect, uint32_t* __restrict__ blk_counts) {
-    extern __shared__ uint32_t hist[];
+    Rec* __restrict__ rec, uint2* __restrict__ rect, uint32_t* __restrict__ blk_counts,
+    uint32_t* __restrict__ zinfo) {   // zinfo[1]: visible tile-0 instances with key 0
+    extern __shared__ uint32_t hist[];   // [0] culled (zero segment), [1 + t] tile t
     const View v = vd ? *vd : vh;   // device-resident view (pose updated on the device) or argument
     const int T = v.ntx * v.nty;
-    for (int t = threadIdx.x; t < T; t += kBinThreads) hist[t] = 0;
+    for (int t = threadIdx.x; t <= T; t += kBinThreads) hist[t] = 0;
     __syncthreads();
     float Rc[3][3];
     quat_to_mat_exact(v.qx, v.qy, v.qz, v.qw, Rc);
@@ -187,19 +192,30 @@ __global__ __launch_bounds__(kBinThreads) void k_project(
         r.r3 = make_float4(cov[0], cov[1], cov[2], 0.f);
         rec[g] = r;
         uint2 rc;
-        if (vis || cull_emit) {
+        if (vis) {
             rc = tile_rect_exact(u, vv, cov[0], cov[1], v);
+            // A visible instance in tile 0 with key 0 (linear depth +0.0) sorts among the
+            // culled ones: flag it, tile 0 then sorts its whole list.
+            if ((rc.x & 0xffff) == 0 && (rc.y & 0xffff) == 0 &&
+                tile_depth_bits(u, vv, z, ph[0], ph[1], 0, 0) == 0u)
+                atomicAdd(zinfo + 1, 1u);
         } else {
-            rc = make_uint2(1u, 0u);  // empty rectangle: x0 = 1 > x1 = 0
+            // culled: the zero record's single instance (tile 0, key 0) goes to the zero
+            // segment at the head of tile 0's list; none when cull_emit is 0
+            rc = cull_emit ? make_uint2(0u, kCulledY) : make_uint2(1u, 0u);
         }
         rect[g] = rc;
+        if (rc.y == kCulledY) {
+            atomicAdd(&hist[0], 1u);
+            continue;
+        }
         int x0 = rc.x & 0xffff, x1 = rc.x >> 16, y0 = rc.y & 0xffff, y1 = rc.y >> 16;
         for (int ty = y0; ty <= y1; ++ty)
-            for (int tx = x0; tx <= x1; ++tx) atomicAdd(&hist[ty * v.ntx + tx], 1u);
+            for (int tx = x0; tx <= x1; ++tx) atomicAdd(&hist[1 + ty * v.ntx + tx], 1u);
     }
     __syncthreads();
-    uint32_t* row = blk_counts + (size_t)blockIdx.x * T;
-    for (int t = threadIdx.x; t < T; t += kBinThreads) row[t] = hist[t];
+    uint32_t* row = blk_counts + (size_t)blockIdx.x * (T + 1);
+    for (int t = threadIdx.x; t <= T; t += kBinThreads) row[t] = hist[t];
 }
 
 // Column prefix in chunks of kScanRows rows: rows become within-chunk exclusive prefixes,
@@ -224,9 +240,11 @@ __global__ __launch_bounds__(256) void k_colscan1(uint32_t* __restrict__ blk_cou
 // One block of 1024 threads. When the instance count exceeds the capacity of the instance
 // buffers (cap), every range is written empty so that the downstream kernels do no work and
 // touch nothing; the host sees M > cap, grows the buffers and runs the view again.
+// T here counts the columns of the count matrix: the zero segment and the tiles.
 __global__ __launch_bounds__(1024) void k_colscan2(uint32_t* __restrict__ chunk_pre, int nchunk,
                                                    int T, uint32_t* __restrict__ ranges,
-                                                   uint32_t* __restrict__ d_m, uint32_t cap) {
+                                                   uint32_t* __restrict__ d_m,
+                                                   uint32_t* __restrict__ zinfo, uint32_t cap) {
     extern __shared__ uint32_t tot[];   // T entries, then 1024 + 32 scratch
     uint32_t* part = tot + T;
     const int tid = threadIdx.x;
@@ -255,12 +273,20 @@ __global__ __launch_bounds__(1024) void k_colscan2(uint32_t* __restrict__ chunk_
     }
     uint32_t run = part[tid] - sum;   // exclusive prefix of this thread's segment
     const bool overflow = part[1023] > cap;
+    // Column 0 is the zero segment (culled instances, key 0), placed at the head of tile 0's
+    // list; column c >= 1 is tile c - 1.
     for (int i = a; i < b; ++i) {
         uint32_t x = tot[i];
-        // tiles without instances keep (0, 0), the reference's memset value (GSSlam.cu:470-471)
-        const bool empty = overflow || x == 0;
-        ranges[2 * i] = empty ? 0u : run;
-        ranges[2 * i + 1] = empty ? 0u : run + x;
+        if (i > 0) {
+            const uint32_t start = (i == 1) ? 0u : run, end = run + x;
+            // tiles without instances keep (0, 0), the reference's memset value
+            // (GSSlam.cu:470-471)
+            const bool empty = overflow || end == start;
+            ranges[2 * (i - 1)] = empty ? 0u : start;
+            ranges[2 * (i - 1) + 1] = empty ? 0u : end;
+        } else {
+            zinfo[0] = overflow ? 0u : x;   // zero-segment length
+        }
         tot[i] = run;
         run += x;
     }
@@ -276,24 +302,44 @@ __global__ __launch_bounds__(kBinThreads) void k_emit(
     const Rec* __restrict__ rec, const uint2* __restrict__ rect,
     const uint32_t* __restrict__ blk_counts, const uint32_t* __restrict__ chunk_pre, int n,
     View vh, const View* __restrict__ vd, uint64_t* __restrict__ pairs, uint64_t cap) {
-    extern __shared__ uint32_t cur[];
+    extern __shared__ uint32_t cur[];   // [0] zero segment, [1 + t] tile t
+    __shared__ uint32_t wz[kBinThreads / 64];
     const View v = vd ? *vd : vh;
     const int T = v.ntx * v.nty;
-    const uint32_t* row = blk_counts + (size_t)blockIdx.x * T;
-    const uint32_t* crow = chunk_pre + (size_t)(blockIdx.x / kScanRows) * T;
-    for (int t = threadIdx.x; t < T; t += kBinThreads) cur[t] = crow[t] + row[t];
+    const uint32_t* row = blk_counts + (size_t)blockIdx.x * (T + 1);
+    const uint32_t* crow = chunk_pre + (size_t)(blockIdx.x / kScanRows) * (T + 1);
+    for (int t = threadIdx.x; t <= T; t += kBinThreads) cur[t] = crow[t] + row[t];
     __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int it = 0; it < kBinItems; ++it) {
-        int g = blockIdx.x * kBinChunk + it * kBinThreads + threadIdx.x;
-        if (g >= n) break;
-        uint2 rc = rect[g];
-        int x0 = rc.x & 0xffff, x1 = rc.x >> 16, y0 = rc.y & 0xffff, y1 = rc.y >> 16;
+        const int g = blockIdx.x * kBinChunk + it * kBinThreads + threadIdx.x;
+        const bool valid = g < n;
+        const uint2 rc = valid ? rect[g] : make_uint2(1u, 0u);
+        // Zero segment: culled instances in Gaussian order (a block-wide ballot prefix), so
+        // that the head of tile 0's list is already sorted by (key 0, index).
+        const bool culled = valid && rc.y == kCulledY;
+        const uint64_t bal = __ballot(culled);
+        if (lane == 0) wz[w] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (int k = 0; k < kBinThreads / 64; ++k) {
+            before += k < w ? wz[k] : 0u;
+            total += wz[k];
+        }
+        if (culled) {
+            const uint32_t p = cur[0] + before + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+            if (p < cap) pairs[p] = (uint64_t)(uint32_t)g;   // key 0
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) cur[0] += total;
+        if (!valid || culled) continue;
+        const int x0 = rc.x & 0xffff, x1 = rc.x >> 16, y0 = rc.y & 0xffff, y1 = rc.y >> 16;
         if (x0 > x1) continue;
-        float4 r0 = rec[g].r0, r1 = rec[g].r1;
+        const float4 r0 = rec[g].r0, r1 = rec[g].r1;
         for (int ty = y0; ty <= y1; ++ty)
             for (int tx = x0; tx <= x1; ++tx) {
                 uint32_t bits = tile_depth_bits(r0.x, r0.y, r1.y, r1.z, r1.w, tx, ty);
-                uint32_t p = atomicAdd(&cur[ty * v.ntx + tx], 1u);
+                uint32_t p = atomicAdd(&cur[1 + ty * v.ntx + tx], 1u);
                 if (p < cap) pairs[p] = ((uint64_t)bits << 32) | (uint32_t)g;
             }
     }
@@ -450,7 +496,8 @@ __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ r
                                                   int idx_bits, uint64_t* __restrict__ pairs,
                                                   uint64_t* __restrict__ tmpbuf,
                                                   uint32_t* __restrict__ sorted_idx,
-                                                  uint64_t* __restrict__ keys_out, uint32_t lo,
+                                                  uint64_t* __restrict__ keys_out,
+                                                  const uint32_t* __restrict__ zinfo, uint32_t lo,
                                                   uint32_t hi) {
     using C = SortCfg<NT, E>;
     constexpr int NW = C::NW, CHUNK = C::CHUNK;
@@ -462,8 +509,18 @@ __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ r
     uint64_t* keysA = reinterpret_cast<uint64_t*>(smem + C::kHeader);
     uint64_t* keysB = keysA + CHUNK;
     const int tile = blockIdx.x, tid = threadIdx.x;
-    const uint32_t s = ranges[2 * tile], n = ranges[2 * tile + 1] - s;
-    if (n == 0 || n <= lo || n > hi) return;
+    const uint32_t s0 = ranges[2 * tile], n0 = ranges[2 * tile + 1] - s0;
+    // Tile 0 starts with the zero segment (culled instances, key 0, already in index order);
+    // unless a visible instance also has key 0 (zinfo[1]), only the rest is sorted.
+    const uint32_t zskip = (tile == 0 && zinfo[1] == 0u) ? min(zinfo[0], n0) : 0u;
+    const uint32_t s = s0 + zskip, n = n0 - zskip;
+    const bool mine = (n > lo && n <= hi) || (n == 0 && n0 > 0 && lo == 0);
+    if (!mine) return;
+    for (uint32_t j = tid; j < zskip; j += NT) {
+        sorted_idx[s0 + j] = (uint32_t)pairs[s0 + j];
+        if (keys_out) keys_out[s0 + j] = (uint64_t)tile << 32;
+    }
+    if (n == 0) return;
     const uint64_t idx_mask = (1ull << idx_bits) - 1ull;
     if (n == 1) {
         if (tid == 0) {
@@ -651,11 +708,12 @@ __global__ __launch_bounds__(256) void k_dequantize(MapRef map, int n, float* __
 }  // namespace
 
 hipError_t launch_project(const MapRef& map, int n, const View& v, const View* vd, int cull_emit,
-                          Rec* rec, uint2* rect, uint32_t* blk_counts, hipStream_t s) {
+                          Rec* rec, uint2* rect, uint32_t* blk_counts, uint32_t* zinfo,
+                          hipStream_t s) {
     int nblk = (n + kBinChunk - 1) / kBinChunk;
-    size_t lds = (size_t)v.ntx * v.nty * 4;
+    size_t lds = ((size_t)v.ntx * v.nty + 1) * 4;
     hipLaunchKernelGGL(k_project, dim3(nblk), dim3(kBinThreads), lds, s, map, n, v, vd,
-                       cull_emit, rec, rect, blk_counts);
+                       cull_emit, rec, rect, blk_counts, zinfo);
     return hipGetLastError();
 }
 
@@ -675,15 +733,17 @@ hipError_t launch_dequantize(const MapRef& map, int n, float* out14, hipStream_t
 }
 
 hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* chunk_pre, int nblk, int ntiles,
-                          uint32_t* ranges, uint32_t* d_m, uint32_t cap, hipStream_t s) {
+                          uint32_t* ranges, uint32_t* d_m, uint32_t* zinfo, uint32_t cap,
+                          hipStream_t s) {
+    const int cols = ntiles + 1;   // zero segment + tiles
     int nchunk = (nblk + kScanRows - 1) / kScanRows;
-    hipLaunchKernelGGL(k_colscan1, dim3((ntiles + 255) / 256, nchunk), dim3(256), 0, s,
-                       blk_counts, chunk_pre, nblk, ntiles);
+    hipLaunchKernelGGL(k_colscan1, dim3((cols + 255) / 256, nchunk), dim3(256), 0, s,
+                       blk_counts, chunk_pre, nblk, cols);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    size_t lds = ((size_t)ntiles + 1024 + 32) * 4;
-    hipLaunchKernelGGL(k_colscan2, dim3(1), dim3(1024), lds, s, chunk_pre, nchunk, ntiles,
-                       ranges, d_m, cap);
+    size_t lds = ((size_t)cols + 1024 + 32) * 4;
+    hipLaunchKernelGGL(k_colscan2, dim3(1), dim3(1024), lds, s, chunk_pre, nchunk, cols,
+                       ranges, d_m, zinfo, cap);
     return hipGetLastError();
 }
 
@@ -691,7 +751,7 @@ hipError_t launch_emit(const Rec* rec, const uint2* rect, const uint32_t* blk_co
                        const uint32_t* chunk_pre, int n, const View& v, const View* vd,
                        uint64_t* pairs, uint64_t cap, hipStream_t s) {
     int nblk = (n + kBinChunk - 1) / kBinChunk;
-    size_t lds = (size_t)v.ntx * v.nty * 4;
+    size_t lds = ((size_t)v.ntx * v.nty + 1) * 4;
     hipLaunchKernelGGL(k_emit, dim3(nblk), dim3(kBinThreads), lds, s, rec, rect, blk_counts,
                        chunk_pre, n, v, vd, pairs, cap);
     return hipGetLastError();
@@ -699,7 +759,7 @@ hipError_t launch_emit(const Rec* rec, const uint2* rect, const uint32_t* blk_co
 
 hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uint64_t* pairs,
                             uint64_t* tmp, uint32_t* sorted_idx, uint64_t* keys_out,
-                            size_t lds_limit, hipStream_t s) {
+                            const uint32_t* zinfo, size_t lds_limit, hipStream_t s) {
     int idx_bits = 1;
     while (idx_bits < 32 && ((uint64_t)1 << idx_bits) < (uint64_t)n_gauss) ++idx_bits;
     // Lists up to 2048 keys: 256 threads (~36 KB LDS, several workgroups per CU); up to 4096:
@@ -719,15 +779,17 @@ hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uin
     if (Large::kLds > lds_limit) return hipErrorInvalidConfiguration;
     const uint32_t small_cap = (uint32_t)Small::CHUNK, medium_cap = (uint32_t)Medium::CHUNK;
     hipLaunchKernelGGL((k_tile_sort<256, 8>), dim3(ntiles), dim3(256), Small::kLds, s, ranges,
-                       idx_bits, pairs, tmp, sorted_idx, keys_out, 0u, small_cap);
+                       idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo, 0u, small_cap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_tile_sort<512, 8>), dim3(ntiles), dim3(512), Medium::kLds, s, ranges,
-                       idx_bits, pairs, tmp, sorted_idx, keys_out, small_cap, medium_cap);
+                       idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo, small_cap,
+                       medium_cap);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_tile_sort<1024, 8>), dim3(ntiles), dim3(1024), Large::kLds, s, ranges,
-                       idx_bits, pairs, tmp, sorted_idx, keys_out, medium_cap, 0xffffffffu);
+                       idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo, medium_cap,
+                       0xffffffffu);
     return hipGetLastError();
 }
 

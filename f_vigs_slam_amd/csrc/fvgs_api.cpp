@@ -240,19 +240,20 @@ int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int
     const int nchunk = (nblk + kScanRows - 1) / kScanRows;
     TRY(ensure(c, c->rec, sizeof(Rec) * (size_t)n));
     TRY(ensure(c, c->rect, sizeof(uint2) * (size_t)n));
-    TRY(ensure(c, c->blk_counts, 4ull * nblk * T));
-    TRY(ensure(c, c->chunk_pre, 4ull * nchunk * T));
+    TRY(ensure(c, c->blk_counts, 4ull * nblk * (T + 1)));
+    TRY(ensure(c, c->chunk_pre, 4ull * nchunk * (T + 1)));
     TRY(ensure(c, c->ranges, 8ull * T));
-    TRY(ensure(c, c->d_m, 16));
+    TRY(ensure(c, c->d_m, 16));   // M | zero-segment length | visible key-0 count in tile 0
     if (async && c->inst_cap == 0) async = false;
     hipStream_t s = c->stream;
     c->view = v;
     mark(c, 0);
+    uint32_t* zinfo = as<uint32_t>(c->d_m) + 1;
+    HIPCHK(c, launch_zero(zinfo + 1, 4, s));
     HIPCHK(c, launch_project(map_ref(c), n, v, c->view_dev, cull_emit, as<Rec>(c->rec),
-                             as<uint2>(c->rect),
-                             as<uint32_t>(c->blk_counts), s));
+                             as<uint2>(c->rect), as<uint32_t>(c->blk_counts), zinfo, s));
     HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->chunk_pre), nblk, T,
-                             as<uint32_t>(c->ranges), as<uint32_t>(c->d_m),
+                             as<uint32_t>(c->ranges), as<uint32_t>(c->d_m), zinfo,
                              async ? c->inst_cap : 0xffffffffu, s));
     HIPCHK(c, hipMemcpyAsync(c->h_m, c->d_m.p, 4, hipMemcpyDeviceToHost, s));
     if (!async) {
@@ -267,7 +268,7 @@ int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int
                           c->inst_cap,
                           s));
     HIPCHK(c, launch_tile_sort(as<uint32_t>(c->ranges), T, n, as<uint64_t>(c->pairs),
-                               as<uint64_t>(c->tmp), as<uint32_t>(c->sorted_idx), keys,
+                               as<uint64_t>(c->tmp), as<uint32_t>(c->sorted_idx), keys, zinfo,
                                c->lds_limit, s));
     mark(c, 1);
     c->prepared = true;
@@ -317,7 +318,8 @@ int run_forward(fvgs_ctx* c) {
                              as<Rec>(c->rec), c->obs_rgb, blend_params(c->prm),
                              as<float>(c->o_rgb), as<float>(c->o_depth), as<float>(c->o_T),
                              as<uint32_t>(c->o_nc), as<float>(c->o_err), as<uint32_t>(c->o_maxc),
-                             as<float4>(c->ckpt), c->wave_rows, c->stream));
+                             as<float4>(c->ckpt), as<uint32_t>(c->d_m) + 1, c->wave_rows,
+                             c->stream));
     c->forwarded = true;
     c->backwarded = false;
     return FVGS_OK;
@@ -793,18 +795,20 @@ int fvgs_track(fvgs_ctx* c, const fvgs_intrinsics* intr, int iters, double pose7
         std::memcpy(c->h_track + off_pose, pose7, 7 * 8);
         std::memset(c->h_track + off_fail, 0, 8);
         HIPCHK(c, hipMemcpyAsync(d, c->h_track, off_status, hipMemcpyHostToDevice, s));
+        uint32_t* zinfo = as<uint32_t>(c->d_m) + 1;
         for (int it = 0; it < iters; it++) {
             HIPCHK(c, hipEventRecord(c->track_ev[it], s));
+            HIPCHK(c, launch_zero(zinfo + 1, 4, s));
             HIPCHK(c, launch_project(map_ref(c), n, v0, vd, 0, as<Rec>(c->rec), as<uint2>(c->rect),
-                                     as<uint32_t>(c->blk_counts), s));
+                                     as<uint32_t>(c->blk_counts), zinfo, s));
             HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->chunk_pre), nblk,
-                                     T, as<uint32_t>(c->ranges), dm + it, c->inst_cap, s));
+                                     T, as<uint32_t>(c->ranges), dm + it, zinfo, c->inst_cap, s));
             HIPCHK(c, launch_emit(as<Rec>(c->rec), as<uint2>(c->rect), as<uint32_t>(c->blk_counts),
                                   as<uint32_t>(c->chunk_pre), n, v0, vd, as<uint64_t>(c->pairs),
                                   c->inst_cap, s));
             HIPCHK(c, launch_tile_sort(as<uint32_t>(c->ranges), T, n, as<uint64_t>(c->pairs),
                                        as<uint64_t>(c->tmp), as<uint32_t>(c->sorted_idx), nullptr,
-                                       c->lds_limit, s));
+                                       zinfo, c->lds_limit, s));
             HIPCHK(c, launch_pose(v0, vd, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
                                   as<Rec>(c->rec), c->obs_rgb, c->obs_depth, c->gx, c->gy, bp,
                                   as<double>(c->pose_tiles), nullptr, nullptr, pc, ps, s));

@@ -88,7 +88,8 @@ struct BlendParams {
 
 // ---- binning (binning.hip) ----
 hipError_t launch_project(const MapRef& map, int n, const View& v, const View* vd, int cull_emit,
-                          Rec* rec, uint2* rect, uint32_t* blk_counts, hipStream_t s);
+                          Rec* rec, uint2* rect, uint32_t* blk_counts, uint32_t* zinfo,
+                          hipStream_t s);
 // FP32 map -> precision-form storage (origins, FP16 offsets and attributes).
 hipError_t launch_quantize(const float* pos, const float* scale, const float* quat,
                            const float* color, const float* opac, int n, float4* origin,
@@ -96,14 +97,17 @@ hipError_t launch_quantize(const float* pos, const float* scale, const float* qu
                            uint16_t* hopac, hipStream_t s);
 // The map as the kernels read it, FP32 [n][14]: pos3 scale3 quat4 color3 opacity.
 hipError_t launch_dequantize(const MapRef& map, int n, float* out14, hipStream_t s);
+// Count matrix columns: 0 = zero segment (culled instances, head of tile 0's list), 1 + t =
+// tile t. Writes the tile ranges, *d_m = M and zinfo[0] = zero-segment length.
 hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* chunk_pre, int nblk, int ntiles,
-                          uint32_t* ranges, uint32_t* d_m, uint32_t cap, hipStream_t s);
+                          uint32_t* ranges, uint32_t* d_m, uint32_t* zinfo, uint32_t cap,
+                          hipStream_t s);
 hipError_t launch_emit(const Rec* rec, const uint2* rect, const uint32_t* blk_counts,
                        const uint32_t* chunk_pre, int n, const View& v, const View* vd,
                        uint64_t* pairs, uint64_t cap, hipStream_t s);
 hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uint64_t* pairs,
                             uint64_t* tmp, uint32_t* sorted_idx, uint64_t* keys_out,
-                            size_t lds_limit, hipStream_t s);
+                            const uint32_t* zinfo, size_t lds_limit, hipStream_t s);
 
 // ---- blending (raster.hip) ----
 hipError_t launch_render(const View& v, const uint32_t* ranges, const uint32_t* idx,
@@ -111,7 +115,8 @@ hipError_t launch_render(const View& v, const uint32_t* ranges, const uint32_t* 
 hipError_t launch_forward(const View& v, const uint32_t* ranges, const uint32_t* idx,
                           const Rec* rec, const float* obs_rgb, const BlendParams& bp, float* rgb,
                           float* depth, float* final_T, uint32_t* n_contrib, float* color_err,
-                          uint32_t* max_contrib, float4* ckpt, int wave_rows, hipStream_t s);
+                          uint32_t* max_contrib, float4* ckpt, const uint32_t* zinfo, int wave_rows,
+                          hipStream_t s);
 hipError_t launch_backward(const View& v, const uint32_t* ranges, const uint32_t* idx,
                            const Rec* rec, const float* obs_depth, const BlendParams& bp,
                            const float* rgb, const float* depth, const uint32_t* n_contrib,

@@ -97,7 +97,7 @@ __global__ __launch_bounds__(256) void k_forward(
     const Rec* __restrict__ rec, const float* __restrict__ obs_rgb, BlendParams bp,
     float* __restrict__ out_rgb, float* __restrict__ out_depth, float* __restrict__ out_T,
     uint32_t* __restrict__ out_nc, float* __restrict__ out_err, uint32_t* __restrict__ max_contrib,
-    float4* __restrict__ ckpt) {
+    float4* __restrict__ ckpt, const uint32_t* __restrict__ zinfo) {
     __shared__ Rec recs[256];
     __shared__ uint8_t qmask[256];
     __shared__ uint8_t wlist[4][256];
@@ -131,7 +131,14 @@ __global__ __launch_bounds__(256) void k_forward(
         return tT < 0.0001f;
     };
     float4* ck = ckpt + (size_t)tile * kMaxBuckets * 256 + pix;
-    for (uint32_t base = 0; base < n; base += 256) {
+    // Tile 0 starts with the zero segment (culled zero records: q = 0, never contributing) unless
+    // a visible key-0 instance is mixed in; its whole batches are not staged. The state stays
+    // (T = 1, colour 0) over it, which is also every checkpoint inside it.
+    const uint32_t zlead = (tile == 0 && zinfo[1] == 0u) ? min(zinfo[0], n) : 0u;
+    const uint32_t base0 = zlead & ~255u;
+    if (base0 > 0 && !done)
+        for (int b = 0; b < kMaxBuckets; ++b) ck[b * 256] = make_float4(1.f, 0.f, 0.f, 0.f);
+    for (uint32_t base = base0; base < n; base += 256) {
         if (__syncthreads_count(!done) == 0) break;
         const int cnt = (int)min(256u, n - base);
         if (tid < cnt) {
@@ -620,15 +627,16 @@ hipError_t launch_render(const View& v, const uint32_t* ranges, const uint32_t* 
 hipError_t launch_forward(const View& v, const uint32_t* ranges, const uint32_t* idx,
                           const Rec* rec, const float* obs_rgb, const BlendParams& bp, float* rgb,
                           float* depth, float* final_T, uint32_t* n_contrib, float* color_err,
-                          uint32_t* max_contrib, float4* ckpt, int wave_rows, hipStream_t s) {
+                          uint32_t* max_contrib, float4* ckpt, const uint32_t* zinfo, int wave_rows,
+                          hipStream_t s) {
     if (wave_rows)   // four 16x4 row bands per tile
         hipLaunchKernelGGL(k_forward<16>, dim3(v.ntx * v.nty), dim3(256), 0, s, v, ranges, idx,
                            rec, obs_rgb, bp, rgb, depth, final_T, n_contrib, color_err,
-                           max_contrib, ckpt);
+                           max_contrib, ckpt, zinfo);
     else             // 2x2 quadrants of 8x8
         hipLaunchKernelGGL(k_forward<8>, dim3(v.ntx * v.nty), dim3(256), 0, s, v, ranges, idx,
                            rec, obs_rgb, bp, rgb, depth, final_T, n_contrib, color_err,
-                           max_contrib, ckpt);
+                           max_contrib, ckpt, zinfo);
     return hipGetLastError();
 }
 
