@@ -108,6 +108,25 @@ def copy_probe(mib):
     log(f"copy probe: 3 x {mib} MiB")
 
 
+def measured_traffic(kernel, n, w, h):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    (profiles/hbm_traffic_1M_640x480.md, scaled by a plain copy of known size); None when the
+    workload differs from the profiled one or the file is absent."""
+    if (n, w, h) != (1_000_000, 640, 480):
+        return None
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "hbm_traffic_1M_640x480.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            for item in line.split(":", 1)[1].split(";"):
+                name, _, val = item.strip().partition("=")
+                if name == kernel or name == kernel + "<8>":
+                    return int(val)
+    return None
+
+
 def tracking(args, log_fn):
     from f_vigs_slam_amd import Context
     from f_vigs_slam_amd import scenes as S
@@ -506,6 +525,11 @@ def main():
     stage /= args.steps
     st = ctx.stats()
     M, K, P = st["M"], st["K"], st["pixels"]
+    # (entry, pixel) pairs the backward's lane pipeline evaluates: the first min(max_contrib,
+    # list length, 256) entries of every tile against its 256 pixels
+    lens = np.diff(ctx.read("RANGES").astype(np.int64), axis=1).ravel()
+    pairs_bwd = int(np.minimum(np.minimum(ctx.read("MAX_CONTRIB").astype(np.int64), lens),
+                               256).sum() * 256)
     if args.breakdown:
         ctx.prepare_rasterization(pose, intr, w, h)
         for _ in range(args.steps):
@@ -526,6 +550,7 @@ def main():
     else:
         kname, kbytes, kms = "k_forward (blend forward, A14)", fwd_bytes, stage[2]
     achieved = kbytes / (kms * 1e-3) / 1e9
+    traffic = measured_traffic(kname.split(" ")[0], n, w, h)
     step_bytes = 108.0 * n + 44.0 * M + 104.0 * K + 36.0 * P
     result = {
         "metric": "fwd+bwd Mpix/s @1M Gaussians 640x480; p99 tracking-step ms, served vs plain",
@@ -550,8 +575,16 @@ def main():
                    "step_algorithmic_GBs": round(step_bytes / t / 1e9, 2)},
         "roofline": {"kernel": kname, "bound": "hbm",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "algorithmic_bytes": int(kbytes), "avg_ms": round(kms, 4)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "algorithmic_bytes": int(kbytes), "avg_ms": round(kms, 4),
+                     # the blend kernels are VALU-bound: FP32 work of the backward, counted as
+                     # 87 FLOP per evaluated (entry, pixel) pair (the reference's per-pair
+                     # gradient arithmetic), against the FP32 vector peak
+                     "compute": {"pairs": pairs_bwd, "flop_per_pair": 87,
+                                 "achieved_tflops": round(pairs_bwd * 87 / (stage[3] * 1e-3) / 1e12,
+                                                          3),
+                                 "peak_tflops": 157.3,
+                                 "frac": round(pairs_bwd * 87 / (stage[3] * 1e-3) / 157.3e12, 4)}},
     }
     ctx.close()
     if not args.no_tracking:
