@@ -28,7 +28,7 @@ PRECISION_F16 = 1
 OUT = dict(POS2D=1, COV2D=2, CONIC=3, DEPTH_G=4, PHAT=5, KEYS=6, INDICES=7, RANGES=8,
            RENDER_RGB=10, RENDER_DEPTH=11, RENDER_ALPHA=12, RGB=20, DEPTH=21, FINAL_T=22,
            N_CONTRIB=23, COLOR_ERROR=24, MAX_CONTRIB=25, DELTA2D=30, DELTA3D=31,
-           POSE_TILES=40, TRACK_RGB=41, TRACK_DEPTH=42, MAP=50)
+           POSE_TILES=40, TRACK_RGB=41, TRACK_DEPTH=42, MAP=50, ADAM_STATE=51)
 
 # Every function the header declares (checked by the CPU test suite against the library).
 EXPORTS = [
@@ -36,7 +36,8 @@ EXPORTS = [
     "fvgs_set_gaussians", "fvgs_gaussian_count", "fvgs_set_frame", "fvgs_prepare_rasterization",
     "fvgs_render", "fvgs_forward", "fvgs_backward", "fvgs_gradients_3d", "fvgs_mapping_step",
     "fvgs_pose_jacobians", "fvgs_render_tracking", "fvgs_compute_gradients", "fvgs_gn_step", "fvgs_track", "fvgs_read", "fvgs_device_ptr",
-    "fvgs_sync", "fvgs_stats", "fvgs_set_timing", "fvgs_last_timing",
+    "fvgs_sync", "fvgs_stats", "fvgs_set_timing", "fvgs_last_timing", "fvgs_adam_step",
+    "fvgs_optimize_keyframe",
     # fvgs_server.h
     "fvgs_server_default_params", "fvgs_server_create", "fvgs_server_destroy",
     "fvgs_server_last_error", "fvgs_server_open_session", "fvgs_server_close_session",
@@ -45,7 +46,7 @@ EXPORTS = [
     # fvgs_window.h
     "fvgs_window_create", "fvgs_window_destroy", "fvgs_window_last_error",
     "fvgs_window_set_gaussians", "fvgs_window_set_view", "fvgs_window_run", "fvgs_window_read",
-    "fvgs_window_get_stats",
+    "fvgs_window_get_stats", "fvgs_window_optimize", "fvgs_window_read_map",
 ]
 
 
@@ -83,7 +84,8 @@ class WindowStats(ctypes.Structure):
     _fields_ = [("concurrent_views", ctypes.c_int), ("wave_rows", ctypes.c_int),
                 ("candidates", ctypes.c_int), ("window_ms", ctypes.c_float),
                 ("best_ms", ctypes.c_float), ("worst_ms", ctypes.c_float),
-                ("instances", ctypes.c_uint64)]
+                ("instances", ctypes.c_uint64), ("optimize_ms", ctypes.c_float),
+                ("optimize_graph", ctypes.c_int)]
 
 
 DELTA2D_DTYPE = np.dtype([("meanImg", "<f4", 2), ("invSigmaImg", "<f4", 3), ("color", "<f4", 3),
@@ -132,6 +134,8 @@ def load():
         "fvgs_stats": ([vp, P(ctypes.c_uint64), P(ctypes.c_uint64), P(u32), P(u32)], i),
         "fvgs_set_timing": ([vp, i], i),
         "fvgs_last_timing": ([vp, P(f)], i),
+        "fvgs_adam_step": ([vp, f], i),
+        "fvgs_optimize_keyframe": ([vp, P(Pose), P(Intrinsics), f], i),
         "fvgs_server_default_params": ([P(ServerParams)], i),
         "fvgs_server_create": ([P(ServerParams), P(vp)], i),
         "fvgs_server_destroy": ([vp], i),
@@ -154,6 +158,8 @@ def load():
         "fvgs_window_run": ([vp, i, vp, vp], i),
         "fvgs_window_read": ([vp, vp, ctypes.c_size_t], i),
         "fvgs_window_get_stats": ([vp, P(WindowStats)], i),
+        "fvgs_window_optimize": ([vp, i, vp, vp, vp, f], i),
+        "fvgs_window_read_map": ([vp, vp, vp, vp, vp, vp], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

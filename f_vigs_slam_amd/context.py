@@ -176,6 +176,16 @@ class Context:
         """Device Sobel gradients of the current frame's colour, attached to the frame."""
         self._check(self._lib.fvgs_compute_gradients(self._h))
 
+    def adam_step(self, eta):
+        """fvgs_adam_step: Adam update of the map from the last mapping step's 3D gradients."""
+        self._check(self._lib.fvgs_adam_step(self._h, float(eta)))
+
+    def optimize_keyframe(self, pose7, intr, eta):
+        """fvgs_optimize_keyframe: mapping step + Adam (optimizeGaussiansKeyframe)."""
+        self._check(self._lib.fvgs_optimize_keyframe(self._h, ctypes.byref(make_pose(pose7)),
+                                                     ctypes.byref(make_intr(intr)), float(eta)))
+        self.view = self.frame_size
+
     def track(self, intr, iters, pose7):
         pose = np.ascontiguousarray(pose7, np.float64).copy()
         ms = np.zeros(max(1, iters))
@@ -227,7 +237,7 @@ class Context:
                       MAX_CONTRIB=((T,), np.uint32), DELTA2D=((n,), L.DELTA2D_DTYPE),
                       DELTA3D=((n,), L.DELTA3D_DTYPE), POSE_TILES=((T, 27), np.float64),
                       TRACK_RGB=((h, w, 3), np.float32), TRACK_DEPTH=((h, w), np.float32),
-                      MAP=((n, 14), np.float32))
+                      MAP=((n, 14), np.float32), ADAM_STATE=((n, 27), np.float32))
         shape, dt = shapes[what]
         out = np.zeros(shape, dt)
         self._check(self._lib.fvgs_read(self._h, L.OUT[what], L.ptr(out), out.nbytes))

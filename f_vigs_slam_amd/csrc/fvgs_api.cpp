@@ -49,6 +49,9 @@ struct fvgs_ctx {
     uint32_t n = 0;
     DevBuf own_pos, own_scale, own_quat, own_color, own_opac;
     DevBuf q_origin, q_pos, q_scale, q_quat, q_color, q_opac, map_out;   // precision form
+    DevBuf adam;                   // AdamState[n], zeroed when the map is set
+    bool adam_zeroed = false;
+    bool grads3d = false;          // d3 holds the 3D gradients of the current view
     const float *pos = nullptr, *scale = nullptr, *quat = nullptr, *color = nullptr,
                 *opac = nullptr;
     // frame
@@ -222,7 +225,7 @@ int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int
     if (!finite_pose(pose)) return fail(c, FVGS_E_ARG, "non-finite pose");
     if (!(intr->f[0] > 0.f) || !(intr->f[1] > 0.f))
         return fail(c, FVGS_E_ARG, "invalid focal length");
-    c->prepared = c->forwarded = c->backwarded = false;
+    c->prepared = c->forwarded = c->backwarded = c->grads3d = false;
     if (c->n == 0) return fail(c, FVGS_E_EMPTY, "no Gaussians");
     View v;
     v.px = pose->position[0]; v.py = pose->position[1]; v.pz = pose->position[2];
@@ -349,6 +352,7 @@ int run_delta3d(fvgs_ctx* c) {
     TRY(ensure(c, c->d3, sizeof(Delta3D) * (size_t)c->n));
     HIPCHK(c, launch_delta3d(map_ref(c), (int)c->n, c->view, c->view_dev, c->prm.lambda_iso,
                              as<Delta2D>(c->d2), as<Delta3D>(c->d3), c->stream));
+    c->grads3d = true;
     return FVGS_OK;
 }
 
@@ -428,6 +432,7 @@ OutDesc describe(fvgs_ctx* c, int what) {
         case FVGS_OUT_TRACK_RGB: return {c->t_rgb.p, 12 * P};
         case FVGS_OUT_TRACK_DEPTH: return {c->t_depth.p, 4 * P};
         case FVGS_OUT_MAP: return {c->map_out.p, 56 * n};
+        case FVGS_OUT_ADAM_STATE: return {c->adam_zeroed ? c->adam.p : nullptr, 108 * n};
         default: return {nullptr, 0};
     }
 }
@@ -502,7 +507,7 @@ int fvgs_destroy(fvgs_ctx* c) {
         if (c->stream) (void)hipStreamSynchronize(c->stream);
         DevBuf* bufs[] = {&c->own_pos, &c->own_scale, &c->own_quat, &c->own_color, &c->own_opac,
                           &c->q_origin, &c->q_pos, &c->q_scale, &c->q_quat, &c->q_color,
-                          &c->q_opac, &c->map_out,
+                          &c->q_opac, &c->map_out, &c->adam,
                           &c->own_rgb, &c->own_depth, &c->own_gx, &c->own_gy, &c->rec, &c->rect,
                           &c->blk_counts, &c->chunk_pre, &c->ranges, &c->d_m, &c->pairs,
                           &c->tmp, &c->sorted_idx, &c->keys, &c->r_rgb, &c->r_depth, &c->r_alpha,
@@ -533,7 +538,8 @@ int fvgs_set_gaussians(fvgs_ctx* c, uint32_t n, const float* pos3, const float* 
         return fail(c, FVGS_E_ARG, "null Gaussian array");
     if (n >= (1u << 31)) return fail(c, FVGS_E_ARG, "too many Gaussians");
     (void)hipSetDevice(c->prm.device);
-    c->prepared = c->forwarded = c->backwarded = false;
+    c->prepared = c->forwarded = c->backwarded = c->grads3d = false;
+    c->adam_zeroed = false;
     c->n = 0;
     if (n == 0) return FVGS_OK;
     TRY(upload(c, c->own_pos, c->pos, pos3, 12ull * n, mem));
@@ -729,6 +735,38 @@ int fvgs_compute_gradients(fvgs_ctx* c) {
     c->gy = as<float>(c->own_gy);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return FVGS_OK;
+}
+
+int fvgs_adam_step(fvgs_ctx* c, float eta) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->prm.device);
+    if (c->prm.precision != FVGS_PRECISION_F32)
+        return fail(c, FVGS_E_STATE, "the Adam update needs the FP32 map (plain form)");
+    if (!c->grads3d) return fail(c, FVGS_E_STATE, "no 3D gradients: run a mapping step first");
+    if (!(eta >= 0.f)) return fail(c, FVGS_E_ARG, "invalid learning rate");
+    TRY(ensure(c, c->adam, sizeof(AdamState) * (size_t)c->n));
+    if (!c->adam_zeroed) {
+        HIPCHK(c, launch_zero(c->adam.p, sizeof(AdamState) * (size_t)c->n, c->stream));
+        c->adam_zeroed = true;
+    }
+    // The map is updated in place: the context's copy, or the caller's arrays for maps set
+    // with FVGS_MEM_DEVICE (as the reference updates gaussians_ in place).
+    HIPCHK(c, launch_adam(const_cast<float*>(c->pos), const_cast<float*>(c->scale),
+                          const_cast<float*>(c->quat), const_cast<float*>(c->color),
+                          const_cast<float*>(c->opac), as<AdamState>(c->adam), as<Delta3D>(c->d3),
+                          (int)c->n, eta, 0.9f, 0.999f, 1e-8f, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->grads3d = false;   // consumed; the screen records no longer match the map
+    c->prepared = c->forwarded = c->backwarded = false;
+    return FVGS_OK;
+}
+
+int fvgs_optimize_keyframe(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr,
+                           float eta) {
+    if (!c) return FVGS_E_ARG;
+    TRY(fvgs_mapping_step(c, pose, intr));
+    return fvgs_adam_step(c, eta);
 }
 
 int fvgs_gn_step(const double jtj21[21], const double jtr6[6], double pose7[7],
@@ -973,6 +1011,16 @@ struct fvgs_window {
     std::string shape;
     hipGraphExec_t exec = nullptr;
     fvgs_window_stats st{};
+    // sequential optimisation (fvgs_window_optimize)
+    AdamState* adam = nullptr;            // shared Adam state, zeroed when the map is set
+    void* backup = nullptr;               // map + Adam state before a replay (overflow recovery)
+    View* steps_dev = nullptr;            // per step: pose and intrinsics read by the kernels
+    View* steps_host = nullptr;           // pinned
+    int steps_cap = 0;
+    uint32_t* mmax_dev = nullptr;         // per slot: the largest M over the replayed sequence
+    uint32_t* mmax_host = nullptr;        // pinned
+    std::string seq_shape;
+    hipGraphExec_t seq_exec = nullptr;
 };
 
 namespace {
@@ -1012,10 +1060,111 @@ int enqueue_mapping(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* i
     return FVGS_OK;
 }
 
+// Drops both captured graphs (the summed-gradient window and the optimisation sequence): they
+// hold the views' buffer addresses, which any eager step or resize may change.
 void drop_graph(fvgs_window* w) {
     if (w->exec) (void)hipGraphExecDestroy(w->exec);
     w->exec = nullptr;
     w->shape.clear();
+    if (w->seq_exec) (void)hipGraphExecDestroy(w->seq_exec);
+    w->seq_exec = nullptr;
+    w->seq_shape.clear();
+}
+
+View make_view(const fvgs_pose& p, const fvgs_intrinsics& in, int W, int H) {
+    View x;
+    x.px = p.position[0]; x.py = p.position[1]; x.pz = p.position[2];
+    x.qx = p.orientation[0]; x.qy = p.orientation[1];
+    x.qz = p.orientation[2]; x.qw = p.orientation[3];
+    x.fx = in.f[0]; x.fy = in.f[1]; x.cx = in.c[0]; x.cy = in.c[1];
+    x.W = W; x.H = H;
+    x.ntx = (W + kTile - 1) / kTile; x.nty = (H + kTile - 1) / kTile;
+    return x;
+}
+
+float* wmap(fvgs_window* w, int k) { return static_cast<float*>(w->map[k]); }
+
+hipError_t enqueue_adam(fvgs_window* w, fvgs_ctx* c, float eta) {
+    return launch_adam(wmap(w, 0), wmap(w, 1), wmap(w, 2), wmap(w, 3), wmap(w, 4), w->adam,
+                       as<Delta3D>(c->d3), (int)w->n, eta, 0.9f, 0.999f, 1e-8f, c->stream);
+}
+
+// The optimisation sequence run step by step with host waits (the reference's order): each
+// step's binning sizes its buffers synchronously. Records each slot's largest M. Returns
+// FVGS_OK, or the first error; `all_bound` is cleared when a step had no instances (skipped, as
+// optimizeGaussiansKeyframe returns early on last_nb_instances_ == 0, GSSlam.cu:1090-1092).
+int optimize_eager(fvgs_window* w, int ns, const int* slots, const fvgs_pose* poses,
+                   const fvgs_intrinsics* intrs, float eta, std::vector<uint32_t>& mmax,
+                   bool& all_bound) {
+    drop_graph(w);
+    all_bound = true;
+    for (int s = 0; s < ns; s++) {
+        fvgs_ctx* c = w->views[(size_t)slots[s]];
+        c->view_dev = nullptr;
+        const int r = fvgs_mapping_step(c, &poses[s], &intrs[s]);
+        if (r == FVGS_E_EMPTY) {
+            all_bound = false;
+            continue;
+        }
+        if (r != FVGS_OK) return wfail(w, r, "step %d: %s", s, c->err.c_str());
+        mmax[(size_t)slots[s]] = std::max(mmax[(size_t)slots[s]], c->m);
+        WCHK(w, enqueue_adam(w, c, eta));
+        WCHK(w, hipStreamSynchronize(c->stream));
+    }
+    return FVGS_OK;
+}
+
+// Captures the sequence on the window's own stream: the steps are strictly ordered (each reads
+// the map the previous one updated), so every step's launches go to w->master in turn and the
+// capture involves no other stream. Step s reads its view from steps_dev[s]; after its binning
+// the slot's M is folded into mmax_dev; then the Adam update of the shared map.
+int capture_sequence(fvgs_window* w, int ns, const int* slots, const fvgs_pose* poses,
+                     const fvgs_intrinsics* intrs, float eta, hipGraphExec_t* out) {
+    WCHK(w, hipStreamBeginCapture(w->master, hipStreamCaptureModeRelaxed));
+    int rc = FVGS_OK;
+    hipError_t e = launch_zero(w->mmax_dev, 4ull * w->max_views, w->master);
+    for (int s = 0; s < ns && rc == FVGS_OK && e == hipSuccess; s++) {
+        fvgs_ctx* c = w->views[(size_t)slots[s]];
+        std::lock_guard<std::mutex> lk(c->mu);
+        hipStream_t own = c->stream;
+        c->stream = w->master;
+        c->view_dev = w->steps_dev + s;
+        rc = enqueue_mapping(c, &poses[s], &intrs[s]);
+        if (rc != FVGS_OK) w->err = c->err;
+        if (rc == FVGS_OK)
+            e = launch_max_count(as<uint32_t>(c->d_m), w->mmax_dev + slots[s], w->master);
+        if (rc == FVGS_OK && e == hipSuccess) e = enqueue_adam(w, c, eta);
+        c->stream = own;
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(w->master, &g);
+    if (rc != FVGS_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return wfail(w, FVGS_E_HIP, "sequence capture failed: %s",
+                     hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return wfail(w, FVGS_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    return FVGS_OK;
+}
+
+// Copies the map and the Adam state into `backup` (to_backup) or back.
+int backup_state(fvgs_window* w, bool to_backup) {
+    const size_t n = w->n;
+    const size_t bytes[6] = {12 * n, 12 * n, 16 * n, 12 * n, 4 * n, sizeof(AdamState) * n};
+    char* b = static_cast<char*>(w->backup);
+    for (int k = 0; k < 6; k++) {
+        void* live = k < 5 ? w->map[k] : static_cast<void*>(w->adam);
+        WCHK(w, hipMemcpyAsync(to_backup ? b : live, to_backup ? live : b, bytes[k],
+                               hipMemcpyDeviceToDevice, w->master));
+        b += bytes[k];
+    }
+    return FVGS_OK;
 }
 
 // Captures the window for `nv` views with at most `conc` views in flight and the given forward
@@ -1154,6 +1303,9 @@ int fvgs_window_create(const fvgs_params* params, int max_views, fvgs_window** o
                          hipHostMalloc((void**)&w->views_host, sizeof(View) * max_views,
                                        hipHostMallocDefault) != hipSuccess ||
                          hipMalloc((void**)&w->d3_table, sizeof(void*) * max_views) != hipSuccess ||
+                         hipMalloc((void**)&w->mmax_dev, 4ull * max_views) != hipSuccess ||
+                         hipHostMalloc((void**)&w->mmax_host, 4ull * max_views,
+                                       hipHostMallocDefault) != hipSuccess ||
                          hipEventCreateWithFlags(&w->ev_fork, hipEventDisableTiming) != hipSuccess ||
                          hipEventCreate(&w->ev_t0) != hipSuccess ||
                          hipEventCreate(&w->ev_t1) != hipSuccess))
@@ -1182,6 +1334,12 @@ int fvgs_window_destroy(fvgs_window* w) {
     if (w->views_host) (void)hipHostFree(w->views_host);
     if (w->sum) (void)hipFree(w->sum);
     if (w->d3_table) (void)hipFree(w->d3_table);
+    if (w->adam) (void)hipFree(w->adam);
+    if (w->backup) (void)hipFree(w->backup);
+    if (w->steps_dev) (void)hipFree(w->steps_dev);
+    if (w->steps_host) (void)hipHostFree(w->steps_host);
+    if (w->mmax_dev) (void)hipFree(w->mmax_dev);
+    if (w->mmax_host) (void)hipHostFree(w->mmax_host);
     for (hipEvent_t e : w->ev_done) if (e) (void)hipEventDestroy(e);
     if (w->ev_fork) (void)hipEventDestroy(w->ev_fork);
     if (w->ev_t0) (void)hipEventDestroy(w->ev_t0);
@@ -1198,16 +1356,30 @@ int fvgs_window_set_gaussians(fvgs_window* w, uint32_t n, const float* pos3, con
                               int mem) {
     if (!w || n == 0 || !pos3 || !scale3 || !quat4 || !color3 || !opacity) return FVGS_E_ARG;
     if (mem != FVGS_MEM_HOST && mem != FVGS_MEM_DEVICE) return FVGS_E_ARG;
+    const float* src[5] = {pos3, scale3, quat4, color3, opacity};
+    const size_t bytes[5] = {12ull * n, 12ull * n, 16ull * n, 12ull * n, 4ull * n};
+    const hipMemcpyKind kind = mem == FVGS_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (n == w->n && w->map[0] && w->prm.precision == FVGS_PRECISION_F32) {
+        // same size: overwrite in place, so the captured graphs (which hold the map's
+        // addresses) stay valid; the Adam state restarts at zero
+        WCHK(w, hipStreamSynchronize(w->master));
+        for (int k = 0; k < 5; k++) WCHK(w, hipMemcpy(w->map[k], src[k], bytes[k], kind));
+        WCHK(w, launch_zero(w->adam, sizeof(AdamState) * n, w->master));
+        WCHK(w, hipStreamSynchronize(w->master));
+        return FVGS_OK;
+    }
     drop_graph(w);
     for (void*& p : w->map) {
         if (p) (void)hipFree(p);
         p = nullptr;
     }
     if (w->sum) (void)hipFree(w->sum);
+    if (w->adam) (void)hipFree(w->adam);
+    if (w->backup) (void)hipFree(w->backup);
     w->sum = nullptr;
-    const float* src[5] = {pos3, scale3, quat4, color3, opacity};
-    const size_t bytes[5] = {12ull * n, 12ull * n, 16ull * n, 12ull * n, 4ull * n};
-    const hipMemcpyKind kind = mem == FVGS_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    w->adam = nullptr;
+    w->backup = nullptr;
+    w->n = 0;
     for (int k = 0; k < 5; k++) {
         if (hipMalloc(&w->map[k], bytes[k]) != hipSuccess) {
             (void)hipGetLastError();
@@ -1215,10 +1387,14 @@ int fvgs_window_set_gaussians(fvgs_window* w, uint32_t n, const float* pos3, con
         }
         WCHK(w, hipMemcpy(w->map[k], src[k], bytes[k], kind));
     }
-    if (hipMalloc((void**)&w->sum, sizeof(Delta3D) * n) != hipSuccess) {
+    if (hipMalloc((void**)&w->sum, sizeof(Delta3D) * n) != hipSuccess ||
+        hipMalloc((void**)&w->adam, sizeof(AdamState) * n) != hipSuccess ||
+        hipMalloc(&w->backup, (56 + sizeof(AdamState)) * (size_t)n) != hipSuccess) {
         (void)hipGetLastError();
-        return wfail(w, FVGS_E_OOM, "gradient allocation failed");
+        return wfail(w, FVGS_E_OOM, "gradient / optimiser allocation failed");
     }
+    WCHK(w, launch_zero(w->adam, sizeof(AdamState) * n, w->master));
+    WCHK(w, hipStreamSynchronize(w->master));
     w->n = n;
     for (fvgs_ctx* c : w->views) {
         int r = fvgs_set_gaussians(c, n, static_cast<float*>(w->map[0]),
@@ -1273,15 +1449,8 @@ int fvgs_window_run(fvgs_window* w, int nv, const fvgs_pose* poses, const fvgs_i
         shape += key;
     }
     // the views' poses and intrinsics, read by the captured kernels from device memory
-    for (int v = 0; v < nv; v++) {
-        View& x = w->views_host[v];
-        x.px = poses[v].position[0]; x.py = poses[v].position[1]; x.pz = poses[v].position[2];
-        x.qx = poses[v].orientation[0]; x.qy = poses[v].orientation[1];
-        x.qz = poses[v].orientation[2]; x.qw = poses[v].orientation[3];
-        x.fx = intrs[v].f[0]; x.fy = intrs[v].f[1]; x.cx = intrs[v].c[0]; x.cy = intrs[v].c[1];
-        x.W = w->fw[(size_t)v]; x.H = w->fh[(size_t)v];
-        x.ntx = (x.W + kTile - 1) / kTile; x.nty = (x.H + kTile - 1) / kTile;
-    }
+    for (int v = 0; v < nv; v++)
+        w->views_host[v] = make_view(poses[v], intrs[v], w->fw[(size_t)v], w->fh[(size_t)v]);
     WCHK(w, hipMemcpyAsync(w->views_dev, w->views_host, sizeof(View) * nv, hipMemcpyHostToDevice,
                            w->master));
     WCHK(w, hipStreamSynchronize(w->master));
@@ -1315,6 +1484,104 @@ int fvgs_window_run(fvgs_window* w, int nv, const fvgs_pose* poses, const fvgs_i
         drop_graph(w);   // buffers grew: recapture for the new addresses
     }
     return wfail(w, FVGS_E_STATE, "instance buffers kept overflowing");
+}
+
+int fvgs_window_optimize(fvgs_window* w, int ns, const int* slots, const fvgs_pose* poses,
+                         const fvgs_intrinsics* intrs, float eta) {
+    if (!w || !slots || !poses || !intrs || ns < 1) return FVGS_E_ARG;
+    if (w->n == 0) return wfail(w, FVGS_E_STATE, "no map");
+    if (w->prm.precision != FVGS_PRECISION_F32)
+        return wfail(w, FVGS_E_STATE, "the Adam update needs the FP32 map (plain form)");
+    if (!(eta >= 0.f)) return wfail(w, FVGS_E_ARG, "invalid learning rate");
+    std::string shape = std::to_string(w->n);
+    char key[96];
+    snprintf(key, sizeof key, ":%08x", [&] { uint32_t b; memcpy(&b, &eta, 4); return b; }());
+    shape += key;
+    for (int s = 0; s < ns; s++) {
+        const int v = slots[s];
+        if (v < 0 || v >= w->max_views) return wfail(w, FVGS_E_ARG, "step %d: bad slot %d", s, v);
+        if (!w->rgb[(size_t)v]) return wfail(w, FVGS_E_STATE, "slot %d has no observation", v);
+        if (!finite_pose(&poses[s]) || !(intrs[s].f[0] > 0.f) || !(intrs[s].f[1] > 0.f))
+            return wfail(w, FVGS_E_ARG, "bad pose or intrinsics for step %d", s);
+        snprintf(key, sizeof key, ",%d:%dx%d", v, w->fw[(size_t)v], w->fh[(size_t)v]);
+        shape += key;
+    }
+    if (ns > w->steps_cap) {
+        drop_graph(w);
+        if (w->steps_dev) (void)hipFree(w->steps_dev);
+        if (w->steps_host) (void)hipHostFree(w->steps_host);
+        w->steps_dev = nullptr;
+        w->steps_host = nullptr;
+        w->steps_cap = 0;
+        if (hipMalloc((void**)&w->steps_dev, sizeof(View) * ns) != hipSuccess ||
+            hipHostMalloc((void**)&w->steps_host, sizeof(View) * ns, hipHostMallocDefault) !=
+                hipSuccess) {
+            (void)hipGetLastError();
+            return wfail(w, FVGS_E_OOM, "step table allocation failed");
+        }
+        w->steps_cap = ns;
+    }
+    for (int s = 0; s < ns; s++)
+        w->steps_host[s] = make_view(poses[s], intrs[s], w->fw[(size_t)slots[s]],
+                                     w->fh[(size_t)slots[s]]);
+    WCHK(w, hipMemcpyAsync(w->steps_dev, w->steps_host, sizeof(View) * ns, hipMemcpyHostToDevice,
+                           w->master));
+    WCHK(w, hipStreamSynchronize(w->master));
+    w->st.optimize_graph = 0;
+    if (shape == w->seq_shape && w->seq_exec) {
+        TRY(backup_state(w, true));
+        float ms = 0.f;
+        TRY(replay(w, w->seq_exec, &ms));
+        WCHK(w, hipMemcpy(w->mmax_host, w->mmax_dev, 4ull * w->max_views, hipMemcpyDeviceToHost));
+        bool overflow = false;
+        for (int v = 0; v < w->max_views; v++) {
+            fvgs_ctx* c = w->views[(size_t)v];
+            const uint32_t m = w->mmax_host[v];
+            if (m > c->inst_cap) {
+                overflow = true;
+                std::lock_guard<std::mutex> lk(c->mu);
+                WTRY(w, c, ensure_instances(c, (uint64_t)m + m / 2));
+            }
+        }
+        if (!overflow) {
+            w->st.optimize_ms = ms;
+            w->st.optimize_graph = 1;
+            return FVGS_OK;
+        }
+        // a step outgrew its instance buffers: undo the sequence and run it step by step
+        TRY(backup_state(w, false));
+        WCHK(w, hipStreamSynchronize(w->master));
+    }
+    std::vector<uint32_t> mmax((size_t)w->max_views, 0);
+    bool all_bound = true;
+    const auto t0 = std::chrono::steady_clock::now();
+    TRY(optimize_eager(w, ns, slots, poses, intrs, eta, mmax, all_bound));
+    w->st.optimize_ms = std::chrono::duration<float, std::milli>(
+                            std::chrono::steady_clock::now() - t0).count();
+    if (!all_bound) return FVGS_OK;   // a step without instances: stay step by step
+    for (int s = 0; s < ns; s++) {
+        fvgs_ctx* c = w->views[(size_t)slots[s]];
+        std::lock_guard<std::mutex> lk(c->mu);
+        const uint64_t m = mmax[(size_t)slots[s]];
+        WTRY(w, c, ensure_instances(c, m + m / 2));
+    }
+    hipGraphExec_t ex = nullptr;
+    TRY(capture_sequence(w, ns, slots, poses, intrs, eta, &ex));
+    w->seq_exec = ex;
+    w->seq_shape = shape;
+    return FVGS_OK;
+}
+
+int fvgs_window_read_map(fvgs_window* w, float* pos3, float* scale3, float* quat4, float* color3,
+                         float* opacity) {
+    if (!w || !pos3 || !scale3 || !quat4 || !color3 || !opacity) return FVGS_E_ARG;
+    if (w->n == 0) return wfail(w, FVGS_E_STATE, "no map");
+    WCHK(w, hipStreamSynchronize(w->master));
+    float* dst[5] = {pos3, scale3, quat4, color3, opacity};
+    const size_t bytes[5] = {12ull * w->n, 12ull * w->n, 16ull * w->n, 12ull * w->n, 4ull * w->n};
+    for (int k = 0; k < 5; k++)
+        WCHK(w, hipMemcpy(dst[k], w->map[k], bytes[k], hipMemcpyDeviceToHost));
+    return FVGS_OK;
 }
 
 int fvgs_window_read(fvgs_window* w, fvgs_delta3d* host, size_t bytes) {

@@ -79,6 +79,11 @@ struct MapRef {
     }
 };
 
+struct AdamState {   // AdamStateGaussian3D layout (GSCudaKernels.cuh:32-45), 108 bytes
+    float m_pos[3], v_pos[3], m_scale[3], v_scale[3], m_rot[3], v_rot[3], m_col[3], v_col[3];
+    float m_alpha, v_alpha, k;
+};
+
 struct BlendParams {
     float bg[3];
     float alpha_thresh, color_thresh, depth_thresh, depth_weight;
@@ -125,11 +130,17 @@ hipError_t launch_backward(const View& v, const uint32_t* ranges, const uint32_t
 hipError_t launch_delta3d(const MapRef& map, int n, const View& v, const View* vd,
                           float lambda_iso, const Delta2D* d2, Delta3D* d3, hipStream_t s);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+hipError_t launch_max_count(const uint32_t* src, uint32_t* dst, hipStream_t s);
 // dst = srcs[0] + ... + srcs[nsrc-1] (device array of device pointers)
 hipError_t launch_sum_delta3d(const Delta3D* const* srcs, int nsrc, Delta3D* dst, int n,
                               hipStream_t s);
 
 hipError_t launch_accum_delta3d(const Delta3D* src, Delta3D* dst, int n, hipStream_t s);
+
+// Adam update of the FP32 map from the 3D gradients (updateGaussiansParametersAdam_kernel).
+hipError_t launch_adam(float* pos, float* scale, float* quat, float* color, float* opac,
+                       AdamState* state, const Delta3D* d3, int n, float eta, float beta1,
+                       float beta2, float eps, hipStream_t s);
 
 // ---- tracking (track.hip) ----
 hipError_t launch_pose(const View& v, const View* vd, const uint32_t* ranges,

@@ -597,6 +597,12 @@ __global__ void k_zero_bytes(uint8_t* __restrict__ p, size_t n) {
     if (threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
+// *dst = max(*dst, *src): the largest instance count a view slot needed over a captured
+// sequence of steps (checked against the slot's capacity after the replay).
+__global__ void k_max_count(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
+    if (threadIdx.x == 0) atomicMax(dst, *src);
+}
+
 // dst = sum over the views' 3D gradients, views in order (fixed, deterministic).
 __global__ __launch_bounds__(256) void k_sum_delta3d(const Delta3D* const* __restrict__ srcs,
                                                      int nsrc, Delta3D* __restrict__ dst, int n) {
@@ -613,6 +619,58 @@ __global__ __launch_bounds__(256) void k_sum_delta3d(const Delta3D* const* __res
         b.n += a.n;
     }
     dst[g] = b;
+}
+
+// updateGaussiansParametersAdam_kernel (GSCudaKernels.cu:2722-2882), one thread per Gaussian:
+// skipped when its gradient count is 0; k += 1; per parameter group
+// m = (1-b1) g + b1 m, v = (1-b2) g^2 + b2 v, step = eta (m / (1 - b1^k)) rsqrt(v / (1 - b2^k) + eps);
+// position, scale, colour += step; q = normalize(q * [1, step / 2]); opacity stepped in logit
+// space (s = log(a / (1 - a)) + step / (a - a^2), a = clamp(sigmoid(s), 0.01, 0.99)); scale
+// clamped to >= 0.001, colour to [0, 1].
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ pos, float* __restrict__ scale,
+                                              float* __restrict__ quat, float* __restrict__ color,
+                                              float* __restrict__ opac, AdamState* __restrict__ st,
+                                              const Delta3D* __restrict__ d3, int n, float eta,
+                                              float b1, float b2, float eps) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= n) return;
+    const Delta3D d = d3[g];
+    if (d.n == 0) return;
+    AdamState a = st[g];
+    a.k += 1.0f;
+    const float a1 = 1.0f - b1, a2 = 1.0f - b2;
+    const float c1 = __frcp_rn(1.0f - __powf(b1, a.k)), c2 = __frcp_rn(1.0f - __powf(b2, a.k));
+    auto step = [&](float& m, float& v, float gr) {
+        m = a1 * gr + b1 * m;
+        v = a2 * gr * gr + b2 * v;
+        return eta * (c1 * m) * __frsqrt_rn(c2 * v + eps);
+    };
+    const float dp[3] = {d.px, d.py, d.pz}, ds[3] = {d.sx, d.sy, d.sz},
+                dr[3] = {d.ox, d.oy, d.oz}, dc[3] = {d.cr, d.cg, d.cb};
+    const size_t g3 = 3 * (size_t)g;
+    for (int k = 0; k < 3; ++k) pos[g3 + k] += step(a.m_pos[k], a.v_pos[k], dp[k]);
+    float dq[3];
+    for (int k = 0; k < 3; ++k) dq[k] = step(a.m_rot[k], a.v_rot[k], dr[k]);
+    {
+        float4* q4 = reinterpret_cast<float4*>(quat) + g;
+        const float4 q = *q4;
+        const float bx = 0.5f * dq[0], by = 0.5f * dq[1], bz = 0.5f * dq[2];
+        float4 r = make_float4(q.w * bx + q.x + q.y * bz - q.z * by,
+                               q.w * by - q.x * bz + q.y + q.z * bx,
+                               q.w * bz + q.x * by - q.y * bx + q.z,
+                               q.w - q.x * bx - q.y * by - q.z * bz);
+        const float inv = rsqrtf(r.x * r.x + r.y * r.y + r.z * r.z + r.w * r.w);
+        *q4 = make_float4(r.x * inv, r.y * inv, r.z * inv, r.w * inv);
+    }
+    for (int k = 0; k < 3; ++k)
+        scale[g3 + k] = fmaxf(0.001f, scale[g3 + k] + step(a.m_scale[k], a.v_scale[k], ds[k]));
+    for (int k = 0; k < 3; ++k)
+        color[g3 + k] = fminf(1.0f, fmaxf(0.0f, color[g3 + k] + step(a.m_col[k], a.v_col[k], dc[k])));
+    const float da = step(a.m_alpha, a.v_alpha, d.alpha);
+    const float al = opac[g];
+    const float sl = __logf(al / (1.0f - al)) + da / (al - al * al);
+    opac[g] = fmaxf(0.01f, fminf(0.99f, 1.0f / (1.0f + __expf(-sl))));
+    st[g] = a;
 }
 
 }  // namespace
@@ -668,6 +726,11 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_max_count(const uint32_t* src, uint32_t* dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_max_count, dim3(1), dim3(64), 0, s, src, dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_sum_delta3d(const Delta3D* const* srcs, int nsrc, Delta3D* dst, int n,
                               hipStream_t s) {
     if (n <= 0) return hipSuccess;
@@ -678,6 +741,15 @@ hipError_t launch_sum_delta3d(const Delta3D* const* srcs, int nsrc, Delta3D* dst
 hipError_t launch_accum_delta3d(const Delta3D* src, Delta3D* dst, int n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_accum_delta3d, dim3((n + 255) / 256), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float* pos, float* scale, float* quat, float* color, float* opac,
+                       AdamState* state, const Delta3D* d3, int n, float eta, float beta1,
+                       float beta2, float eps, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, s, pos, scale, quat, color,
+                       opac, state, d3, n, eta, beta1, beta2, eps);
     return hipGetLastError();
 }
 

@@ -1,7 +1,9 @@
 """Python handle over fvgs_window (include/fvgs/fvgs_window.h): a batched mapping window.
 
 Several keyframe views of one map run forward + backward + 2D->3D as one captured hipGraph
-(autotuned schedule per window shape); the per-Gaussian 3D gradients are summed over the views.
+(autotuned schedule per window shape); the per-Gaussian 3D gradients are summed over the views
+(run), or the views are optimised in sequence with Adam on the shared map, the reference's
+optimizeGaussians loop, replayed as one captured hipGraph per sequence shape (optimize).
 """
 from __future__ import annotations
 
@@ -58,6 +60,26 @@ class Window:
         I = (L.Intrinsics * nv)(*[make_intr(i) for i in intrs])
         self._check(self._lib.fvgs_window_run(self._h, nv, ctypes.cast(P, ctypes.c_void_p),
                                               ctypes.cast(I, ctypes.c_void_p)))
+
+    def optimize(self, slots, poses, intrs, eta):
+        """fvgs_window_optimize: optimizeGaussiansKeyframe on slots[s] at poses[s] for each
+        step s in order, Adam on the shared map (GSSlam::optimizeGaussians' loop)."""
+        ns = len(slots)
+        S = (ctypes.c_int * ns)(*[int(v) for v in slots])
+        P = (L.Pose * ns)(*[make_pose(p) for p in poses])
+        I = (L.Intrinsics * ns)(*[make_intr(i) for i in intrs])
+        self._check(self._lib.fvgs_window_optimize(self._h, ns, ctypes.cast(S, ctypes.c_void_p),
+                                                   ctypes.cast(P, ctypes.c_void_p),
+                                                   ctypes.cast(I, ctypes.c_void_p), float(eta)))
+
+    def read_map(self):
+        """The window's map as a scene dict (pos, scale, quat, color, opacity)."""
+        out = dict(pos=np.zeros((self.n, 3), np.float32), scale=np.zeros((self.n, 3), np.float32),
+                   quat=np.zeros((self.n, 4), np.float32), color=np.zeros((self.n, 3), np.float32),
+                   opacity=np.zeros(self.n, np.float32))
+        self._check(self._lib.fvgs_window_read_map(
+            self._h, *(L.ptr(out[k]) for k in ("pos", "scale", "quat", "color", "opacity"))))
+        return out
 
     def gradients(self):
         out = np.zeros(self.n, L.DELTA3D_DTYPE)

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define FVGS_ABI_VERSION 3
+#define FVGS_ABI_VERSION 4
 
 /* Status codes. */
 #define FVGS_OK 0
@@ -148,8 +148,9 @@ enum fvgs_out {
                                   order of fvgs_pose_data (Jtr[6], then the JtJ upper triangle) */
     FVGS_OUT_TRACK_RGB = 41,   /* float3[P]  pass-1 render of the last pose call */
     FVGS_OUT_TRACK_DEPTH = 42, /* float[P]   pass-1 median depth of the last pose call */
-    FVGS_OUT_MAP = 50          /* float[N][14] the map as the kernels read it (dequantised in the
+    FVGS_OUT_MAP = 50,         /* float[N][14] the map as the kernels read it (dequantised in the
                                   precision form): pos3 scale3 quat4 (x,y,z,w) color3 opacity */
+    FVGS_OUT_ADAM_STATE = 51   /* float[N][27] AdamStateGaussian3D (GSCudaKernels.cuh:32-45) */
 };
 
 int fvgs_default_params(fvgs_params* params);
@@ -200,6 +201,18 @@ int fvgs_gradients_3d(fvgs_ctx* ctx);
  * fvgs_gradients_3d. Replaces GSSlam::optimizeGaussiansKeyframe minus updateGaussiansParametersAdam_kernel
  * (GSSlam.cu:1083-1264). */
 int fvgs_mapping_step(fvgs_ctx* ctx, const fvgs_pose* pose, const fvgs_intrinsics* intr);
+
+/* Adam update of the map from the 3D gradients of the last mapping step (replaces
+ * updateGaussiansParametersAdam_kernel, GSCudaKernels.cu:2722-2882, with the reference's
+ * beta1 0.9, beta2 0.999, eps 1e-8 and per-Gaussian step counts, GSSlam.cu:1265-1312).
+ * Gaussians without gradient (n = 0) are untouched. The map is updated in place: the context's
+ * copy, or the caller's arrays for a map set with FVGS_MEM_DEVICE. The Adam state starts at zero
+ * when the map is set. Plain form only (FVGS_E_STATE in the precision form). */
+int fvgs_adam_step(fvgs_ctx* ctx, float eta);
+
+/* GSSlam::optimizeGaussiansKeyframe (GSSlam.cu:1083-1312): fvgs_mapping_step + fvgs_adam_step. */
+int fvgs_optimize_keyframe(fvgs_ctx* ctx, const fvgs_pose* pose, const fvgs_intrinsics* intr,
+                           float eta);
 
 /* RGB-D pose Jacobians at a kernel-frame pose against the current frame (which needs gx, gy):
  * prepare + getRgbdPoseJacobians (GSCudaKernels.cu:922-1392) + an FP64 reduction of the per-tile

@@ -1,5 +1,7 @@
 /* fvgs_window: a batched mapping window — forward, backward and 2D->3D gradients of several
- * keyframe views of one shared map, with the per-Gaussian 3D gradients summed over the views.
+ * keyframe views of one shared map, with the per-Gaussian 3D gradients summed over the views
+ * (fvgs_window_run), or the reference's sequential keyframe optimisation with Adam
+ * (fvgs_window_optimize).
  *
  * The reference runs GSSlam::optimizeGaussiansKeyframe once per keyframe with host syncs in
  * between (GSSlam.cu:1020-1081, 1083-1264); this entry point has no direct counterpart and is
@@ -36,13 +38,18 @@ typedef struct fvgs_window_stats {
     float best_ms;         /* autotuner's time of the chosen schedule */
     float worst_ms;        /* autotuner's time of the slowest candidate */
     uint64_t instances;    /* tile instances over all views of the last run */
+    float optimize_ms;     /* last fvgs_window_optimize: graph replay device time, or host wall
+                              time when it ran step by step */
+    int optimize_graph;    /* 1 when the last fvgs_window_optimize replayed the captured graph */
 } fvgs_window_stats;
 
 int fvgs_window_create(const fvgs_params* params, int max_views, fvgs_window** out);
 int fvgs_window_destroy(fvgs_window* win);
 const char* fvgs_window_last_error(const fvgs_window* win);
 
-/* The shared map (copied into window-owned device memory; host or device arrays). */
+/* The shared map (copied into window-owned device memory; host or device arrays). A map of the
+ * same size as the current one is copied over it in place and keeps the captured graphs; either
+ * way the Adam state of fvgs_window_optimize restarts at zero. */
 int fvgs_window_set_gaussians(fvgs_window* win, uint32_t n, const float* pos3, const float* scale3,
                               const float* quat4, const float* color3, const float* opacity,
                               int mem);
@@ -52,6 +59,23 @@ int fvgs_window_set_view(fvgs_window* win, int slot, int width, int height, cons
 /* Runs views 0 .. n_views-1 at the given poses / intrinsics and sums their 3D gradients. */
 int fvgs_window_run(fvgs_window* win, int n_views, const fvgs_pose* poses,
                     const fvgs_intrinsics* intrinsics);
+/* GSSlam::optimizeGaussians' keyframe loop (GSSlam.cu:1041-1078): for s = 0 .. n_steps-1 in
+ * order, optimizeGaussiansKeyframe (GSSlam.cu:1083-1312) on view slot slots[s] at poses[s] /
+ * intrinsics[s] with learning rate eta — projection, binning, forward, backward, 2D->3D and the
+ * Adam update of the window's map, each step seeing the map the previous one left. The caller
+ * chooses the steps (the reference takes the last two keyframes plus KeyframeSelector samples per
+ * iteration, GSSlam.cu:1046-1071). The Adam state is shared by all steps and starts at zero when
+ * the map is set; a step whose view has no tile instances is skipped, as in the reference.
+ * The first call of a sequence shape (slots, image sizes, map size, eta) runs step by step and
+ * captures the sequence into a hipGraph; later calls with that shape replay it with the new
+ * poses (read from device memory). If a replayed step outgrows its instance buffers the map and
+ * Adam state are restored and the sequence is run step by step. Plain form only (FVGS_E_STATE
+ * in the precision form). */
+int fvgs_window_optimize(fvgs_window* win, int n_steps, const int* slots, const fvgs_pose* poses,
+                         const fvgs_intrinsics* intrinsics, float eta);
+/* The window's map (after fvgs_window_optimize) into host arrays of n entries each. */
+int fvgs_window_read_map(fvgs_window* win, float* pos3, float* scale3, float* quat4,
+                         float* color3, float* opacity);
 /* The summed 3D gradients of the last run (fvgs_delta3d[n]) into host memory. */
 int fvgs_window_read(fvgs_window* win, fvgs_delta3d* host, size_t bytes);
 int fvgs_window_get_stats(fvgs_window* win, fvgs_window_stats* out);

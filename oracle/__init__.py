@@ -265,3 +265,16 @@ def pipeline(scene, cam_pos, cam_quat, intr, width, height):
     rec = project_f64(scene["pos"], scene["scale"], scene["quat"], cam_pos, cam_quat, intr,
                       width, height)
     return pf, b, rec
+
+
+def adam(scene, state, delta3d, eta, beta1=0.9, beta2=0.999, eps=1e-8):
+    """updateGaussiansParametersAdam_kernel (GSCudaKernels.cu:2722-2882) in FP64.
+    scene: dict of float arrays (not modified); state: (N, 27) float64 (updated in place);
+    delta3d: (N, 16) as returned by delta3d(). Returns the updated map (float64 dict)."""
+    out = {k: np.ascontiguousarray(scene[k], np.float64).copy()
+           for k in ("pos", "scale", "quat", "color", "opacity")}
+    lib().orc_adam(ctypes.c_int(len(out["opacity"])), _p(out["pos"]), _p(out["scale"]),
+                   _p(out["quat"]), _p(out["color"]), _p(out["opacity"]), _p(state),
+                   _p(_f64(delta3d)), ctypes.c_double(eta), ctypes.c_double(beta1),
+                   ctypes.c_double(beta2), ctypes.c_double(eps))
+    return out

@@ -967,6 +967,59 @@ ORC_API void orc_pose_jacobians(int width, int height, const uint32_t* ranges, c
         for (int k = 0; k < 27; k++) out_total[k] += out_tiles[27 * (size_t)t + k];
 }
 
+// Next row: updateGaussiansParametersAdam_kernel (GSCudaKernels.cu:2722-2882) in FP64 over the
+// FP32 map. Per Gaussian with delta n > 0: k += 1; m = (1-b1) g + b1 m; v = (1-b2) g^2 + b2 v;
+// step = eta (m / (1 - b1^k)) / sqrt(v / (1 - b2^k) + eps); position, scale, colour += step;
+// q = normalize(q * [1, step_theta / 2]); opacity through the logit:
+// s = log(a / (1 - a)) + step_a / (a - a^2), a = clamp(sigmoid(s), 0.01, 0.99); then
+// scale >= 0.001 and colour in [0, 1]. state per Gaussian: 27 doubles
+// (m_pos3 v_pos3 m_scale3 v_scale3 m_rot3 v_rot3 m_col3 v_col3 m_a v_a k); delta: orc_delta3d's
+// 16-double layout. Parameters are updated in place (double copies of the map).
+ORC_API void orc_adam(int n, double* pos, double* scale, double* quat, double* color,
+                      double* opac, double* state, const double* delta, double eta, double b1,
+                      double b2, double eps) {
+#pragma omp parallel for schedule(static)
+    for (int g = 0; g < n; g++) {
+        const double* d = delta + 16 * (size_t)g;
+        if (d[13] == 0.0) continue;
+        double* st = state + 27 * (size_t)g;
+        st[26] += 1.0;
+        const double k = st[26];
+        const double c1 = 1.0 / (1.0 - std::pow(b1, k)), c2 = 1.0 / (1.0 - std::pow(b2, k));
+        auto step = [&](double& m, double& v, double gr) {
+            m = (1.0 - b1) * gr + b1 * m;
+            v = (1.0 - b2) * gr * gr + b2 * v;
+            return eta * (c1 * m) / std::sqrt(c2 * v + eps);
+        };
+        for (int a = 0; a < 3; a++) pos[3 * (size_t)g + a] += step(st[a], st[3 + a], d[a]);
+        double dq[3];
+        for (int a = 0; a < 3; a++) dq[a] = step(st[12 + a], st[15 + a], d[6 + a]);
+        {   // q (x, y, z, w) * (x, y, z, w) = (0.5 dq, 1)
+            double* q = quat + 4 * (size_t)g;
+            const double x = q[0], y = q[1], z = q[2], w = q[3];
+            const double bx = 0.5 * dq[0], by = 0.5 * dq[1], bz = 0.5 * dq[2];
+            double r[4] = {w * bx + x + y * bz - z * by, w * by - x * bz + y + z * bx,
+                           w * bz + x * by - y * bx + z, w - x * bx - y * by - z * bz};
+            const double nr = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3]);
+            for (int a = 0; a < 4; a++) q[a] = r[a] / nr;
+        }
+        for (int a = 0; a < 3; a++) {
+            double& s = scale[3 * (size_t)g + a];
+            s += step(st[6 + a], st[9 + a], d[3 + a]);
+            s = std::max(0.001, s);
+        }
+        for (int a = 0; a < 3; a++) {
+            double& c = color[3 * (size_t)g + a];
+            c += step(st[18 + a], st[21 + a], d[9 + a]);
+            c = std::min(1.0, std::max(0.0, c));
+        }
+        const double da = step(st[24], st[25], d[12]);
+        const double al = opac[g];
+        const double sl = std::log(al / (1.0 - al)) + da / (al - al * al);
+        opac[g] = std::max(0.01, std::min(0.99, 1.0 / (1.0 + std::exp(-sl))));
+    }
+}
+
 // A21: FP64 Gauss-Newton step. JtJ from the 21-entry upper triangle (row-major r<=c), Jtr the
 // kernel-convention sum (the host negates it, GSSlam.cu:2225, and Ceres' step is -JtJ^-1 *
 // (-Jtr)). Solves JtJ * delta = Jtr by Cholesky, then PoseLocalParameterization::Plus
