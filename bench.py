@@ -500,10 +500,15 @@ def window_optimize(args, log_fn, scene, poses, views):
     """The reference's keyframe optimisation loop with Adam (8 steps of optimizeGaussiansKeyframe
     over the window's keyframes, OPT_SEQ): plain = one context, frames resident in HBM, one
     fvgs_optimize_keyframe per step (host waits between steps, as the reference's loop);
-    window = fvgs_window_optimize, the whole sequence replayed as one captured graph."""
-    import torch
+    window = fvgs_window_optimize, the whole sequence replayed as one captured graph.
+    Frames stay in HBM in both (holder contexts lend theirs as device arrays)."""
     from f_vigs_slam_amd import Context, Window
-    dev = [(torch.from_numpy(rgb).cuda(), torch.from_numpy(depth).cuda()) for _, rgb, depth in views]
+    holders = []          # one context per keyframe holds its frame in HBM
+    for _, rgb, depth in views:
+        hc = Context()
+        hc.set_frame(rgb, depth)
+        holders.append(hc)
+    dev = [hc.frame_device() for hc in holders]
     ctx = Context()
     ctx.set_scene(scene)
 
@@ -519,6 +524,8 @@ def window_optimize(args, log_fn, scene, poses, views):
         plain_seq()
         plain.append(time.perf_counter() - t0)
     ctx.close()
+    for hc in holders:
+        hc.close()
     win = Window(max_views=len(views))
     win.set_scene(scene)
     for k, (intr, rgb, depth) in enumerate(views):

@@ -28,7 +28,9 @@ PRECISION_F16 = 1
 OUT = dict(POS2D=1, COV2D=2, CONIC=3, DEPTH_G=4, PHAT=5, KEYS=6, INDICES=7, RANGES=8,
            RENDER_RGB=10, RENDER_DEPTH=11, RENDER_ALPHA=12, RGB=20, DEPTH=21, FINAL_T=22,
            N_CONTRIB=23, COLOR_ERROR=24, MAX_CONTRIB=25, DELTA2D=30, DELTA3D=31,
-           POSE_TILES=40, TRACK_RGB=41, TRACK_DEPTH=42, MAP=50, ADAM_STATE=51)
+           POSE_TILES=40, TRACK_RGB=41, TRACK_DEPTH=42, MAP=50, ADAM_STATE=51,
+           FRAME_RGB=52, FRAME_DEPTH=53, POSITIONS=54, SCALES=55, ORIENTATIONS=56,
+           COLORS=57, OPACITIES=58)
 
 # Every function the header declares (checked by the CPU test suite against the library).
 EXPORTS = [

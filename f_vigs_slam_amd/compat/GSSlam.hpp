@@ -8,7 +8,9 @@
 //
 //   prepareRasterization(pose, intrinsics, w, h)        GSSlam.cu:408-646
 //   renderView(pose, intrinsics, w, h, rgb, depth)       GSSlam.cu:775-799  (device aliases)
-//   optimizeGaussiansKeyframe(keyframe)                  GSSlam.cu:1083-1264 (without Adam)
+//   optimizeGaussiansKeyframe(keyframe, eta)             GSSlam.cu:1083-1312 (with Adam)
+//   optimizeGaussians(iterations, eta, sampler)          GSSlam.cu:1020-1081
+//   getGaussianData(positions, colors)                   GSSlam.cuh:59
 //   computeRgbdPoseJacobians(JtJ, Jtr, level, P, Q)      GSSlam.cu:2103-2244
 //
 // It depends on nothing but the C ABI and the C++ standard library; with Eigen on the include
@@ -19,6 +21,7 @@
 // the kernels read as (x, y, z, w)).
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -142,9 +145,12 @@ public:
         return true;
     }
 
-    // Forward + backward + 2D->3D gradients for one keyframe; the fvgs_delta3d[N] result stays
-    // on the device (deltas3d_dev). The Adam update of the reference is not part of this path.
-    void optimizeGaussiansKeyframe(const KeyframeData& kf, fvgs_delta3d** deltas3d_dev = nullptr) {
+    // Forward + backward + 2D->3D gradients for one keyframe, then the Adam update of the map
+    // with learning rate eta (the Adam state lives as long as the map); the step's
+    // fvgs_delta3d[N] stays on the device (deltas3d_dev). A keyframe whose view has no tile
+    // instances is skipped, as in the reference (GSSlam.cu:1090-1092).
+    void optimizeGaussiansKeyframe(const KeyframeData& kf, float eta = 0.002f,
+                                   fvgs_delta3d** deltas3d_dev = nullptr) {
         std::lock_guard<std::mutex> lk(mutex_);
         if (!ctx_ || getGaussiansCount() == 0 || !kf.frame.rgb || !kf.frame.depth) return;
         if (!check(fvgs_set_frame(ctx_, kf.frame.width, kf.frame.height, kf.frame.rgb,
@@ -153,11 +159,59 @@ public:
             return;
         fvgs_pose p = to_pose(kf.pose);
         fvgs_intrinsics i = to_intr(kf.intrinsics);
-        if (!check(fvgs_mapping_step(ctx_, &p, &i), "optimizeGaussiansKeyframe")) return;
+        const int r = fvgs_mapping_step(ctx_, &p, &i);
+        if (r == FVGS_E_EMPTY) return;
+        if (!check(r, "optimizeGaussiansKeyframe")) return;
         if (deltas3d_dev) {
             size_t b = 0;
             fvgs_device_ptr(ctx_, FVGS_OUT_DELTA3D, (void**)deltas3d_dev, &b);
         }
+        check(fvgs_adam_step(ctx_, eta), "optimizeGaussiansKeyframe");
+    }
+
+    // GSSlam::optimizeGaussians (GSSlam.cu:1020-1081): per iteration the last keyframe, the one
+    // before it, and keyframes drawn by `sample(count, total)` (the reference's KeyframeSelector,
+    // host-side and the caller's to provide), each through optimizeGaussiansKeyframe. For a
+    // whole window replayed as one captured graph use fvgs_window_optimize (fvgs_window.h).
+    void addKeyframe(const KeyframeData& kf) { keyframes_.push_back(kf); }
+    size_t keyframeCount() const { return keyframes_.size(); }
+    template <class Sampler>
+    void optimizeGaussians(int nb_iterations, float eta, Sampler&& sample) {
+        if (getGaussiansCount() == 0 || keyframes_.empty()) return;
+        const int total = (int)keyframes_.size();
+        for (int it = 0; it < std::max(1, nb_iterations); ++it) {
+            std::vector<int> sel;
+            auto add = [&](int idx) {
+                if (idx < 0 || idx >= total) return;
+                for (int e : sel)
+                    if (e == idx) return;
+                sel.push_back(idx);
+            };
+            add(total - 1);
+            if (total > 1) add(total - 2);
+            if (sel.size() < 4 && total > (int)sel.size())
+                for (int idx : sample(2, total)) add(idx);
+            for (int idx : sel) optimizeGaussiansKeyframe(keyframes_[(size_t)idx], eta);
+        }
+    }
+
+    // GSSlam::getGaussianData (GSSlam.cuh:59): positions and colours of the map (after any
+    // Adam updates); returns the count.
+    uint32_t getGaussianData(std::vector<std::array<float, 3>>& positions,
+                             std::vector<std::array<float, 3>>& colors) {
+        std::lock_guard<std::mutex> lk(mutex_);
+        const uint32_t n = getGaussiansCount();
+        positions.resize(n);
+        colors.resize(n);
+        if (n == 0) return 0;
+        if (!check(fvgs_read(ctx_, FVGS_OUT_POSITIONS, positions.data(), 12ull * n),
+                   "getGaussianData") ||
+            !check(fvgs_read(ctx_, FVGS_OUT_COLORS, colors.data(), 12ull * n), "getGaussianData")) {
+            positions.clear();
+            colors.clear();
+            return 0;
+        }
+        return n;
     }
 
     // RGB-D pose Jacobians at the IMU pose (P_imu, Q_imu as (x, y, z, w)) for a pyramid level:
@@ -299,6 +353,7 @@ private:
     double t_imu_cam_[3] = {0, 0, 0};
     double q_imu_cam_[4] = {0, 0, 0, 1};
     std::vector<FrameLevel> pyramid_;
+    std::vector<KeyframeData> keyframes_;
     uint32_t last_nb_instances_ = 0;
 };
 

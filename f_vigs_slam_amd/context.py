@@ -62,6 +62,21 @@ def gn_step(jtj21, jtr6, pose7):
     return pose, d
 
 
+class DeviceArray:
+    """A float32 device array owned elsewhere (e.g. another context's frame): passed to the
+    library as FVGS_MEM_DEVICE, like a torch CUDA tensor."""
+    is_cuda = True
+
+    def __init__(self, ptr, shape):
+        self.ptr, self.shape = int(ptr), tuple(shape)
+
+    def data_ptr(self):
+        return self.ptr
+
+    def __len__(self):
+        return self.shape[0]
+
+
 class Context:
     def __init__(self, **params):
         self._lib = L.load()
@@ -213,6 +228,26 @@ class Context:
                                          ctypes.byref(t), ctypes.byref(p)))
         return dict(M=int(m.value), K=int(k.value), tiles=int(t.value), pixels=int(p.value))
 
+    def frame_device(self):
+        """The current frame's device arrays (colour, depth) as DeviceArray handles, usable as
+        FVGS_MEM_DEVICE frames of another context while this one lives."""
+        w, h = self.frame_size
+        return (DeviceArray(self.device_ptr("FRAME_RGB")[0], (h, w, 3)),
+                DeviceArray(self.device_ptr("FRAME_DEPTH")[0], (h, w)))
+
+    MAP_FIELDS = (("pos", "POSITIONS", 3), ("scale", "SCALES", 3), ("quat", "ORIENTATIONS", 4),
+                  ("color", "COLORS", 3), ("opacity", "OPACITIES", 1))
+
+    def map_device(self):
+        """The map arrays the kernels read, as a scene dict of DeviceArray handles (usable as
+        another context's FVGS_MEM_DEVICE map while this one lives)."""
+        return {k: DeviceArray(self.device_ptr(o)[0], (self.n, c) if c > 1 else (self.n,))
+                for k, o, c in self.MAP_FIELDS}
+
+    def read_map(self):
+        """The map arrays the kernels read (after fvgs_adam_step: the updated map)."""
+        return {k: self.read(o) for k, o, _ in self.MAP_FIELDS}
+
     def device_ptr(self, what):
         p, b = ctypes.c_void_p(), ctypes.c_size_t(0)
         self._check(self._lib.fvgs_device_ptr(self._h, L.OUT[what], ctypes.byref(p),
@@ -222,6 +257,7 @@ class Context:
     # -- results -------------------------------------------------------------------------
     def read(self, what, count=None):
         w, h = self.view if self.view else (0, 0)
+        fw, fh = getattr(self, "frame_size", (0, 0))
         P = w * h
         T = ((w + 15) // 16) * ((h + 15) // 16)
         n = self.n
@@ -237,7 +273,11 @@ class Context:
                       MAX_CONTRIB=((T,), np.uint32), DELTA2D=((n,), L.DELTA2D_DTYPE),
                       DELTA3D=((n,), L.DELTA3D_DTYPE), POSE_TILES=((T, 27), np.float64),
                       TRACK_RGB=((h, w, 3), np.float32), TRACK_DEPTH=((h, w), np.float32),
-                      MAP=((n, 14), np.float32), ADAM_STATE=((n, 27), np.float32))
+                      MAP=((n, 14), np.float32), ADAM_STATE=((n, 27), np.float32),
+                      FRAME_RGB=((fh, fw, 3), np.float32), FRAME_DEPTH=((fh, fw), np.float32),
+                      POSITIONS=((n, 3), np.float32), SCALES=((n, 3), np.float32),
+                      ORIENTATIONS=((n, 4), np.float32), COLORS=((n, 3), np.float32),
+                      OPACITIES=((n,), np.float32))
         shape, dt = shapes[what]
         out = np.zeros(shape, dt)
         self._check(self._lib.fvgs_read(self._h, L.OUT[what], L.ptr(out), out.nbytes))

@@ -432,6 +432,15 @@ OutDesc describe(fvgs_ctx* c, int what) {
         case FVGS_OUT_TRACK_RGB: return {c->t_rgb.p, 12 * P};
         case FVGS_OUT_TRACK_DEPTH: return {c->t_depth.p, 4 * P};
         case FVGS_OUT_MAP: return {c->map_out.p, 56 * n};
+        case FVGS_OUT_FRAME_RGB:
+            return {const_cast<float*>(c->obs_rgb), 12ull * c->fw * c->fh};
+        case FVGS_OUT_FRAME_DEPTH:
+            return {const_cast<float*>(c->obs_depth), 4ull * c->fw * c->fh};
+        case FVGS_OUT_POSITIONS: return {const_cast<float*>(c->pos), 12 * n};
+        case FVGS_OUT_SCALES: return {const_cast<float*>(c->scale), 12 * n};
+        case FVGS_OUT_ORIENTATIONS: return {const_cast<float*>(c->quat), 16 * n};
+        case FVGS_OUT_COLORS: return {const_cast<float*>(c->color), 12 * n};
+        case FVGS_OUT_OPACITIES: return {const_cast<float*>(c->opac), 4 * n};
         case FVGS_OUT_ADAM_STATE: return {c->adam_zeroed ? c->adam.p : nullptr, 108 * n};
         default: return {nullptr, 0};
     }

@@ -150,7 +150,17 @@ enum fvgs_out {
     FVGS_OUT_TRACK_DEPTH = 42, /* float[P]   pass-1 median depth of the last pose call */
     FVGS_OUT_MAP = 50,         /* float[N][14] the map as the kernels read it (dequantised in the
                                   precision form): pos3 scale3 quat4 (x,y,z,w) color3 opacity */
-    FVGS_OUT_ADAM_STATE = 51   /* float[N][27] AdamStateGaussian3D (GSCudaKernels.cuh:32-45) */
+    FVGS_OUT_ADAM_STATE = 51,  /* float[N][27] AdamStateGaussian3D (GSCudaKernels.cuh:32-45) */
+    FVGS_OUT_FRAME_RGB = 52,   /* float[H][W][3] the frame's colour as the kernels read it (the
+                                  context's device copy, or the caller's FVGS_MEM_DEVICE array) */
+    FVGS_OUT_FRAME_DEPTH = 53, /* float[H][W] the frame's depth, likewise */
+    /* The map arrays the kernels read and fvgs_adam_step updates (the context's copies, or the
+     * caller's FVGS_MEM_DEVICE arrays); plain form only. */
+    FVGS_OUT_POSITIONS = 54,   /* float[N][3] */
+    FVGS_OUT_SCALES = 55,      /* float[N][3] */
+    FVGS_OUT_ORIENTATIONS = 56,/* float[N][4] (x, y, z, w) */
+    FVGS_OUT_COLORS = 57,      /* float[N][3] */
+    FVGS_OUT_OPACITIES = 58    /* float[N] */
 };
 
 int fvgs_default_params(fvgs_params* params);

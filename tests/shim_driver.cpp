@@ -1,0 +1,125 @@
+// Test driver for the C++ GSSlam shim (f_vigs_slam_amd/compat/GSSlam.hpp): a C++ caller that
+// links libfvgs.so and calls the shim's methods, as the reference's RgbdPoseCost and mapping
+// thread would. Built by __graft_entry__.build() into tests/_bin/shim_driver; driven by
+// tests/test_gpu_shim.py through files in a scratch directory.
+//
+//   shim_driver DIR pose      scene.bin frame.bin pose.bin -> out.bin: JtJ[36], Jtr[6] (double)
+//   shim_driver DIR optimize  scene.bin frame.bin pose.bin -> out.bin: for each of two
+//                             optimizeGaussiansKeyframe calls, positions[N][3] and colors[N][3]
+//                             (float) after it, then the call's delta3d[N]
+//
+// scene.bin: uint32 n; float pos[3n], scale[3n], quat[4n], color[3n], opacity[n]
+// frame.bin: int32 w, h; float rgb[3wh], depth[wh]
+// pose.bin:  float intr[4]; double t_imu_cam[3], q_imu_cam[4] (x,y,z,w), P_imu[3], Q_imu[4];
+//            float eta; float cam_pose[7] (position, float4 slot as stored)
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../f_vigs_slam_amd/compat/GSSlam.hpp"
+
+namespace {
+
+bool read_all(const std::string& path, std::vector<char>& buf) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    std::fseek(f, 0, SEEK_END);
+    const long len = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    buf.resize((size_t)len);
+    const bool ok = std::fread(buf.data(), 1, buf.size(), f) == buf.size();
+    std::fclose(f);
+    return ok;
+}
+
+template <class T>
+const T* take(const std::vector<char>& buf, size_t& off, size_t count) {
+    const T* p = reinterpret_cast<const T*>(buf.data() + off);
+    off += sizeof(T) * count;
+    return off <= buf.size() ? p : nullptr;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc != 3) {
+        std::fprintf(stderr, "usage: shim_driver DIR pose|optimize\n");
+        return 2;
+    }
+    const std::string dir = argv[1], mode = argv[2];
+    std::vector<char> sb, fb, pb;
+    if (!read_all(dir + "/scene.bin", sb) || !read_all(dir + "/frame.bin", fb) ||
+        !read_all(dir + "/pose.bin", pb)) {
+        std::fprintf(stderr, "shim_driver: missing input files in %s\n", dir.c_str());
+        return 2;
+    }
+    size_t o = 0;
+    const uint32_t n = *take<uint32_t>(sb, o, 1);
+    const float* pos = take<float>(sb, o, 3ull * n);
+    const float* scale = take<float>(sb, o, 3ull * n);
+    const float* quat = take<float>(sb, o, 4ull * n);
+    const float* color = take<float>(sb, o, 3ull * n);
+    const float* opac = take<float>(sb, o, n);
+    size_t q = 0;
+    const int32_t* wh = take<int32_t>(fb, q, 2);
+    const int w = wh[0], h = wh[1];
+    const float* rgb = take<float>(fb, q, 3ull * w * h);
+    const float* depth = take<float>(fb, q, (size_t)w * h);
+    size_t r = 0;
+    const float* intr = take<float>(pb, r, 4);
+    const double* t_ic = take<double>(pb, r, 3);
+    const double* q_ic = take<double>(pb, r, 4);
+    const double* P_imu = take<double>(pb, r, 3);
+    const double* Q_imu = take<double>(pb, r, 4);
+    const float* eta = take<float>(pb, r, 1);
+    const float* cam = take<float>(pb, r, 7);
+    if (!opac || !depth || !cam) {
+        std::fprintf(stderr, "shim_driver: truncated input\n");
+        return 2;
+    }
+
+    fvgs_compat::GSSlam slam;
+    if (!slam.ok()) return 3;
+    fvgs_compat::IntrinsicParameters ip;
+    ip.f[0] = intr[0]; ip.f[1] = intr[1]; ip.c[0] = intr[2]; ip.c[1] = intr[3];
+    slam.setIntrinsics(ip);
+    slam.setImuToCamExtrinsics(t_ic, q_ic);
+    if (!slam.setGaussians(n, pos, scale, quat, color, opac)) return 4;
+    fvgs_compat::FrameLevel lv;
+    lv.width = w; lv.height = h; lv.rgb = rgb; lv.depth = depth;
+
+    FILE* out = std::fopen((dir + "/out.bin").c_str(), "wb");
+    if (!out) return 2;
+    int rc = 0;
+    if (mode == "pose") {
+        slam.setFramePyramid({lv});
+        double JtJ[36], Jtr[6];
+        slam.computeRgbdPoseJacobians(JtJ, Jtr, 0, P_imu, Q_imu);
+        std::fwrite(JtJ, sizeof(double), 36, out);
+        std::fwrite(Jtr, sizeof(double), 6, out);
+    } else if (mode == "optimize") {
+        fvgs_compat::KeyframeData kf;
+        kf.frame = lv;
+        std::memcpy(kf.pose.position, cam, 12);
+        std::memcpy(kf.pose.orientation, cam + 3, 16);
+        kf.intrinsics = ip;
+        for (int step = 0; step < 2 && rc == 0; step++) {
+            fvgs_delta3d* d3 = nullptr;
+            slam.optimizeGaussiansKeyframe(kf, *eta, &d3);
+            std::vector<std::array<float, 3>> P, C;
+            if (slam.getGaussianData(P, C) != n || !d3) rc = 5;
+            std::vector<fvgs_delta3d> d(n);
+            if (fvgs_read(slam.context(), FVGS_OUT_DELTA3D, d.data(),
+                          sizeof(fvgs_delta3d) * n) != FVGS_OK)
+                rc = 6;
+            std::fwrite(P.data(), 12, P.size(), out);
+            std::fwrite(C.data(), 12, C.size(), out);
+            std::fwrite(d.data(), sizeof(fvgs_delta3d), n, out);
+        }
+    } else {
+        rc = 2;
+    }
+    std::fclose(out);
+    return rc;
+}

@@ -11,7 +11,6 @@ consecutive steps, the second from a new pose (first moments no longer equal to 
 """
 import numpy as np
 import pytest
-import torch
 
 import oracle as O
 from f_vigs_slam_amd import Context, FvgsError
@@ -110,27 +109,30 @@ def test_adam_two_steps_match_oracle():
 
 def test_optimize_keyframe_updates_device_map_in_place():
     """fvgs_optimize_keyframe on a map set with FVGS_MEM_DEVICE updates the caller's arrays in
-    place (as the reference updates gaussians_); the result meets the oracle's bar for the
-    gradients of that same step. (Two contexts' gradients differ in the order of the backward's
-    float atomics, which Adam's normalisation amplifies where a Gaussian's gradient nearly
-    cancels, so runs are not compared with each other.)"""
+    place (as the reference updates gaussians_); here the arrays are another context's map
+    copies. The result meets the oracle's bar for the gradients of that same step. (Two
+    contexts' gradients differ in the order of the backward's float atomics, which Adam's
+    normalisation amplifies where a Gaussian's gradient nearly cancels, so runs are not
+    compared with each other.)"""
     scene = S.make_scene(N, seed=12, width=W, height=H)
     intr = S.intrinsics(W, H)
     pose = S.perturbed_pose(3, rot_deg=1.0, trans_m=0.02)
     rgb, depth = _observation(scene, np.array([0, 0, 0, 0, 0, 0, 1.0]), intr)
-    dev = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in scene.items()}
+    owner = Context()
+    owner.set_scene(scene)
     ctx = Context()
-    ctx.set_scene(dev)
+    ctx.set_scene(owner.map_device())
     ctx.set_frame(rgb, depth)
     ctx.optimize_keyframe(pose, intr, ETA)
-    torch.cuda.synchronize()
     d = _d16(ctx.read("DELTA3D"))
     want = O.adam(scene, np.zeros((N, 27)), d, ETA)
-    got = np.concatenate([dev[k].cpu().numpy().reshape(N, -1) for k, _, _ in FIELDS], axis=1)
+    m = owner.read_map()
+    got = np.concatenate([m[k].reshape(N, -1) for k, _, _ in FIELDS], axis=1)
     prev = np.concatenate([scene[k].reshape(N, -1) for k, _, _ in FIELDS], axis=1)
     _check_map("device map", got, want, prev, d)
     np.testing.assert_array_equal(ctx.read("MAP"), got)
     ctx.close()
+    owner.close()
 
 
 def test_adam_rejected_in_precision_form():

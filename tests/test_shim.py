@@ -1,7 +1,7 @@
 """The C++ GSSlam shim (f_vigs_slam_amd/compat/GSSlam.hpp) compiles against the C ABI with a
 plain host compiler, and its Ceres-cost conditioning (RgbdPoseCost.cu:57-102) matches a numpy
-eigen-decomposition. The GPU half (computeRgbdPoseJacobians through the shim) is in
-test_gpu_shim.py."""
+eigen-decomposition. The GPU half (computeRgbdPoseJacobians and optimizeGaussiansKeyframe
+through the shim, driven by tests/shim_driver.cpp) is in test_gpu_shim.py."""
 import os
 import subprocess
 
@@ -14,6 +14,14 @@ LIBDIR = os.path.join(ROOT, "f_vigs_slam_amd")
 PROG = r'''
 #include <cstdio>
 #include "GSSLAM_HPP"
+// instantiates the keyframe loop (GSSlam::optimizeGaussians) with a caller-side sampler
+void keyframe_loop(fvgs_compat::GSSlam& s) {
+    s.optimizeGaussians(2, 0.002f, [](int count, int total) {
+        std::vector<int> v;
+        for (int i = 0; i < count && i < total; i++) v.push_back(i);
+        return v;
+    });
+}
 int main() {
     double A[36], b[6], r[6], J[42];
     for (int i = 0; i < 36; i++) if (scanf("%lf", &A[i]) != 1) return 2;
