@@ -416,6 +416,39 @@ def precision(args, log_fn):
     return out
 
 
+def frame_pyramid(args, log_fn, levels=3, reps=20):
+    """Tracking input pipeline per frame (GSSlam::initAndCopyImgs): 8-bit colour + 16-bit
+    millimetre depth at the bench resolution -> 3 levels of float colour, depth and Sobel
+    gradients. GPU: fvgs_set_frame_raw from host memory (upload included), p50 wall time. CPU:
+    the oracle's restatement (C pyrDown / Sobel + numpy conversions, single-threaded pyrDown)."""
+    import oracle as O
+    from f_vigs_slam_amd import Context
+    w, h = args.width, args.height
+    rng = np.random.default_rng(9)
+    col = rng.integers(0, 256, (h, w, 3), dtype=np.uint8)
+    dmm = rng.integers(300, 6000, (h, w), dtype=np.uint16)
+    ctx = Context()
+    ctx.set_frame_raw(col, dmm, levels=levels)
+    gpu = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ctx.set_frame_raw(col, dmm, levels=levels)
+        gpu.append(time.perf_counter() - t0)
+    ctx.close()
+    cpu = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.frame_pyramid(col, dmm, levels)
+        cpu.append(time.perf_counter() - t0)
+    out = dict(workload=f"{w}x{h} 8-bit colour + 16-bit depth, {levels} levels",
+               gpu_ms_p50=round(1e3 * float(np.median(gpu)), 4),
+               oracle_cpu_ms=round(1e3 * float(np.median(cpu)), 3),
+               note="GPU time includes the host-to-device upload of both images")
+    log_fn(f"frame pyramid: {out['gpu_ms_p50']} ms on the GPU (upload included) vs "
+           f"{out['oracle_cpu_ms']} ms oracle CPU")
+    return out
+
+
 def window(args, log_fn):
     """Batched mapping window: 8 keyframes (3x640x480, 3x848x480, 2x1280x720) on a 2 m arc
     facing a shared 1M-Gaussian map (means uniform in a 2 m cube), observation = each view's
@@ -656,6 +689,11 @@ def main():
             result["tracking"] = tracking(args, log)
         except Exception as e:  # report, do not hide the main number
             result["tracking"] = {"error": str(e)}
+    if not args.no_tracking:
+        try:
+            result["frame_pyramid"] = frame_pyramid(args, log)
+        except Exception as e:  # report, do not hide the main number
+            result["frame_pyramid"] = {"error": repr(e)}
     if not args.no_window:
         try:
             result["window"] = window(args, log)

@@ -11,6 +11,7 @@
 //   optimizeGaussiansKeyframe(keyframe, eta)             GSSlam.cu:1083-1312 (with Adam)
 //   optimizeGaussians(iterations, eta, sampler)          GSSlam.cu:1020-1081
 //   getGaussianData(positions, colors)                   GSSlam.cuh:59
+//   initAndCopyImgs(w, h, color, depth, format)          GSSlam.cu:1913-1969 (on the device)
 //   computeRgbdPoseJacobians(JtJ, Jtr, level, P, Q)      GSSlam.cu:2103-2244
 //
 // It depends on nothing but the C ABI and the C++ standard library; with Eigen on the include
@@ -113,7 +114,29 @@ public:
     bool hasGaussians() const { return getGaussiansCount() > 0; }
 
     // Pyramid of the current frame (GSSlam::initAndCopyImgs output); level 0 first.
-    void setFramePyramid(const std::vector<FrameLevel>& levels) { pyramid_ = levels; }
+    void setFramePyramid(const std::vector<FrameLevel>& levels) {
+        pyramid_ = levels;
+        raw_levels_ = 0;
+    }
+
+    // GSSlam::initAndCopyImgs (GSSlam.cu:1913-1969) on the device: 8-bit 3-channel colour and
+    // depth (FVGS_DEPTH_F32 metres or FVGS_DEPTH_U16_MM millimetres) -> nb_pyr_levels levels of
+    // float colour, depth and Sobel gradients in HBM, used by computeRgbdPoseJacobians.
+    void initAndCopyImgs(int width, int height, const uint8_t* color, const void* depth,
+                         int depth_format, int mem = FVGS_MEM_HOST) {
+        std::lock_guard<std::mutex> lk(mutex_);
+        if (!ctx_ || !color || !depth) return;
+        raw_levels_ = 0;
+        if (!check(fvgs_set_frame_raw(ctx_, width, height, color, depth, depth_format,
+                                      nb_pyr_levels_, mem),
+                   "initAndCopyImgs"))
+            return;
+        pyramid_.clear();
+        raw_levels_ = nb_pyr_levels_;
+    }
+    void setPyramidLevels(int levels) {
+        nb_pyr_levels_ = std::max(1, std::min(levels, FVGS_MAX_PYRAMID_LEVELS));
+    }
 
     void prepareRasterization(const CameraPose& pose, const IntrinsicParameters& intr, int w,
                               int h) {
@@ -222,7 +245,7 @@ public:
         std::lock_guard<std::mutex> lk(mutex_);
         for (int k = 0; k < 36; k++) JtJ[k] = 0.0;
         for (int k = 0; k < 6; k++) Jtr[k] = 0.0;
-        if (!ctx_ || pyramid_.empty() || getGaussiansCount() == 0) return;
+        if (!ctx_ || (pyramid_.empty() && raw_levels_ == 0) || getGaussiansCount() == 0) return;
         // IMU -> camera (GSSlam.cu:2113-2121)
         double R_imu[9];
         quat_to_mat(Q_imu, R_imu);
@@ -240,16 +263,23 @@ public:
         pose.orientation[2] = (float)Q_cam[1];
         pose.orientation[3] = (float)Q_cam[2];
         // pyramid level and scaled intrinsics (GSSlam.cu:2126-2138)
-        const int lv = std::min(level, (int)pyramid_.size() - 1);
-        const FrameLevel& fl = pyramid_[lv];
+        const int nlv = raw_levels_ > 0 ? raw_levels_ : (int)pyramid_.size();
+        const int lv = std::max(0, std::min(level, nlv - 1));
         const float s = 1.0f / (float)(1 << lv);
         fvgs_intrinsics intr = to_intr(intrinsics_);
         intr.f[0] *= s; intr.f[1] *= s; intr.c[0] *= s; intr.c[1] *= s;
-        if (!check(fvgs_set_frame(ctx_, fl.width, fl.height, fl.rgb, fl.depth, fl.gx, fl.gy,
-                                  fl.mem), "computeRgbdPoseJacobians"))
-            return;
-        if ((!fl.gx || !fl.gy) && !check(fvgs_compute_gradients(ctx_), "computeRgbdPoseJacobians"))
-            return;
+        if (raw_levels_ > 0) {
+            if (!check(fvgs_select_level(ctx_, lv, nullptr, nullptr), "computeRgbdPoseJacobians"))
+                return;
+        } else {
+            const FrameLevel& fl = pyramid_[(size_t)lv];
+            if (!check(fvgs_set_frame(ctx_, fl.width, fl.height, fl.rgb, fl.depth, fl.gx, fl.gy,
+                                      fl.mem), "computeRgbdPoseJacobians"))
+                return;
+            if ((!fl.gx || !fl.gy) &&
+                !check(fvgs_compute_gradients(ctx_), "computeRgbdPoseJacobians"))
+                return;
+        }
         double jtj21[21], jtr6[6];
         int r = fvgs_pose_jacobians(ctx_, &pose, &intr, jtj21, jtr6);
         if (r == FVGS_E_EMPTY) return;
@@ -354,6 +384,8 @@ private:
     double q_imu_cam_[4] = {0, 0, 0, 1};
     std::vector<FrameLevel> pyramid_;
     std::vector<KeyframeData> keyframes_;
+    int nb_pyr_levels_ = 3;    // GSSlam.cuh:275
+    int raw_levels_ = 0;       // levels built by initAndCopyImgs (0: pyramid_ is used)
     uint32_t last_nb_instances_ = 0;
 };
 

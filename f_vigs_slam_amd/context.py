@@ -146,6 +146,30 @@ class Context:
         self._keep_frame = [k for _, _, k in args]
         self.frame_size = (w, h)
 
+    def set_frame_raw(self, color_u8, depth, levels=3):
+        """fvgs_set_frame_raw: 8-bit colour (H, W, 3) and depth (float32 metres or uint16
+        millimetres) -> a `levels`-level pyramid built on the device; level 0 becomes current."""
+        color_u8 = np.ascontiguousarray(color_u8, np.uint8)
+        depth = np.ascontiguousarray(depth)
+        if depth.dtype == np.uint16:
+            fmt = L.DEPTH_U16_MM
+        else:
+            depth = np.ascontiguousarray(depth, np.float32)
+            fmt = L.DEPTH_F32
+        h, w = depth.shape[:2]
+        self._check(self._lib.fvgs_set_frame_raw(self._h, w, h, L.ptr(color_u8), L.ptr(depth), fmt,
+                                                 int(levels), L.MEM_HOST))
+        self._keep_frame = [color_u8, depth]
+        self.frame_size = (w, h)
+
+    def select_level(self, level):
+        """fvgs_select_level: pyramid level `level` becomes the current frame; returns (w, h)."""
+        w, h = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self._lib.fvgs_select_level(self._h, int(level), ctypes.byref(w),
+                                                ctypes.byref(h)))
+        self.frame_size = (w.value, h.value)
+        return self.frame_size
+
     # -- compute -------------------------------------------------------------------------
     def prepare_rasterization(self, pose7, intr, width, height):
         m = ctypes.c_uint32(0)
@@ -277,7 +301,8 @@ class Context:
                       FRAME_RGB=((fh, fw, 3), np.float32), FRAME_DEPTH=((fh, fw), np.float32),
                       POSITIONS=((n, 3), np.float32), SCALES=((n, 3), np.float32),
                       ORIENTATIONS=((n, 4), np.float32), COLORS=((n, 3), np.float32),
-                      OPACITIES=((n,), np.float32))
+                      OPACITIES=((n,), np.float32), FRAME_GX=((fh, fw, 3), np.float32),
+                      FRAME_GY=((fh, fw, 3), np.float32))
         shape, dt = shapes[what]
         out = np.zeros(shape, dt)
         self._check(self._lib.fvgs_read(self._h, L.OUT[what], L.ptr(out), out.nbytes))

@@ -58,6 +58,14 @@ struct fvgs_ctx {
     int fw = 0, fh = 0;
     DevBuf own_rgb, own_depth, own_gx, own_gy;
     const float *obs_rgb = nullptr, *obs_depth = nullptr, *gx = nullptr, *gy = nullptr;
+    // frame pyramid built from raw images (fvgs_set_frame_raw)
+    static constexpr int kLevels = FVGS_MAX_PYRAMID_LEVELS;
+    DevBuf raw_color, raw_depth;                       // host inputs staged in HBM
+    DevBuf pyr_u8[kLevels], pyr_rgb[kLevels], pyr_depth[kLevels], pyr_gx[kLevels],
+        pyr_gy[kLevels];
+    const float* pyr_depth_ptr[kLevels] = {};          // level 0 may be the caller's array
+    int pyr_w[kLevels] = {}, pyr_h[kLevels] = {};
+    int pyr_levels = 0;
     // view
     bool prepared = false, forwarded = false, backwarded = false;
     View view{};
@@ -436,6 +444,8 @@ OutDesc describe(fvgs_ctx* c, int what) {
             return {const_cast<float*>(c->obs_rgb), 12ull * c->fw * c->fh};
         case FVGS_OUT_FRAME_DEPTH:
             return {const_cast<float*>(c->obs_depth), 4ull * c->fw * c->fh};
+        case FVGS_OUT_FRAME_GX: return {const_cast<float*>(c->gx), 12ull * c->fw * c->fh};
+        case FVGS_OUT_FRAME_GY: return {const_cast<float*>(c->gy), 12ull * c->fw * c->fh};
         case FVGS_OUT_POSITIONS: return {const_cast<float*>(c->pos), 12 * n};
         case FVGS_OUT_SCALES: return {const_cast<float*>(c->scale), 12 * n};
         case FVGS_OUT_ORIENTATIONS: return {const_cast<float*>(c->quat), 16 * n};
@@ -523,6 +533,15 @@ int fvgs_destroy(fvgs_ctx* c) {
                           &c->o_rgb, &c->o_depth, &c->o_T, &c->o_nc, &c->o_err, &c->o_maxc,
                           &c->d2, &c->d3, &c->ckpt, &c->pose_tiles, &c->pose_sum, &c->t_rgb, &c->t_depth, &c->pose_pix};
         for (DevBuf* b : bufs) release(c, *b);
+        release(c, c->raw_color);
+        release(c, c->raw_depth);
+        for (int l = 0; l < fvgs_ctx::kLevels; l++) {
+            release(c, c->pyr_u8[l]);
+            release(c, c->pyr_rgb[l]);
+            release(c, c->pyr_depth[l]);
+            release(c, c->pyr_gx[l]);
+            release(c, c->pyr_gy[l]);
+        }
         for (auto& e : c->ev)
             if (e) (void)hipEventDestroy(e);
         release(c, c->track);
@@ -605,6 +624,104 @@ int fvgs_set_frame(fvgs_ctx* c, int w, int h, const float* rgb3, const float* de
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->fw = w;
     c->fh = h;
+    return FVGS_OK;
+}
+
+namespace {
+
+// Makes pyramid level l the current frame (colour, depth and Sobel gradients, all in HBM).
+void attach_level(fvgs_ctx* c, int l) {
+    c->obs_rgb = as<float>(c->pyr_rgb[l]);
+    c->obs_depth = c->pyr_depth_ptr[l];
+    c->gx = as<float>(c->pyr_gx[l]);
+    c->gy = as<float>(c->pyr_gy[l]);
+    c->fw = c->pyr_w[l];
+    c->fh = c->pyr_h[l];
+    c->forwarded = c->backwarded = false;
+}
+
+}  // namespace
+
+int fvgs_set_frame_raw(fvgs_ctx* c, int w, int h, const uint8_t* color3, const void* depth,
+                       int depth_format, int levels, int mem) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (w <= 0 || h <= 0) return fail(c, FVGS_E_ARG, "invalid frame size %d x %d", w, h);
+    if (!color3 || !depth) return fail(c, FVGS_E_ARG, "null frame image");
+    if (mem != FVGS_MEM_HOST && mem != FVGS_MEM_DEVICE) return fail(c, FVGS_E_ARG, "bad mem kind");
+    if (depth_format != FVGS_DEPTH_F32 && depth_format != FVGS_DEPTH_U16_MM)
+        return fail(c, FVGS_E_ARG, "bad depth format %d", depth_format);
+    if (levels < 1 || levels > fvgs_ctx::kLevels)
+        return fail(c, FVGS_E_ARG, "levels must be 1..%d", fvgs_ctx::kLevels);
+    (void)hipSetDevice(c->prm.device);
+    hipStream_t s = c->stream;
+    const size_t P = (size_t)w * h;
+    c->obs_rgb = c->obs_depth = c->gx = c->gy = nullptr;
+    c->pyr_levels = 0;
+    // level 0 sources in HBM
+    const uint8_t* col0 = color3;
+    if (mem == FVGS_MEM_HOST) {
+        TRY(ensure(c, c->raw_color, 3 * P));
+        HIPCHK(c, hipMemcpyAsync(c->raw_color.p, color3, 3 * P, hipMemcpyHostToDevice, s));
+        col0 = as<uint8_t>(c->raw_color);
+    }
+    if (depth_format == FVGS_DEPTH_U16_MM) {
+        const uint16_t* d16 = static_cast<const uint16_t*>(depth);
+        if (mem == FVGS_MEM_HOST) {
+            TRY(ensure(c, c->raw_depth, 2 * P));
+            HIPCHK(c, hipMemcpyAsync(c->raw_depth.p, depth, 2 * P, hipMemcpyHostToDevice, s));
+            d16 = as<uint16_t>(c->raw_depth);
+        }
+        TRY(ensure(c, c->pyr_depth[0], 4 * P));
+        HIPCHK(c, launch_u16_to_f32(d16, as<float>(c->pyr_depth[0]), P, s));
+        c->pyr_depth_ptr[0] = as<float>(c->pyr_depth[0]);
+    } else if (mem == FVGS_MEM_HOST) {
+        TRY(ensure(c, c->pyr_depth[0], 4 * P));
+        HIPCHK(c, hipMemcpyAsync(c->pyr_depth[0].p, depth, 4 * P, hipMemcpyHostToDevice, s));
+        c->pyr_depth_ptr[0] = as<float>(c->pyr_depth[0]);
+    } else {
+        c->pyr_depth_ptr[0] = static_cast<const float*>(depth);   // referenced in place
+    }
+    int lw = w, lh = h;
+    const uint8_t* col = col0;
+    for (int l = 0; l < levels; l++) {
+        if (l > 0) {
+            const int nw = (lw + 1) / 2, nh = (lh + 1) / 2;
+            TRY(ensure(c, c->pyr_u8[l], 3ull * nw * nh));
+            TRY(ensure(c, c->pyr_depth[l], 4ull * nw * nh));
+            HIPCHK(c, launch_pyrdown_u8x3(col, lw, lh, as<uint8_t>(c->pyr_u8[l]), nw, nh, s));
+            HIPCHK(c, launch_pyrdown_f32(c->pyr_depth_ptr[l - 1], lw, lh,
+                                         as<float>(c->pyr_depth[l]), nw, nh, s));
+            c->pyr_depth_ptr[l] = as<float>(c->pyr_depth[l]);
+            col = as<uint8_t>(c->pyr_u8[l]);
+            lw = nw;
+            lh = nh;
+        }
+        const size_t lp = (size_t)lw * lh;
+        TRY(ensure(c, c->pyr_rgb[l], 12 * lp));
+        TRY(ensure(c, c->pyr_gx[l], 12 * lp));
+        TRY(ensure(c, c->pyr_gy[l], 12 * lp));
+        HIPCHK(c, launch_u8_to_f32(col, as<float>(c->pyr_rgb[l]), 3 * lp, s));
+        HIPCHK(c, launch_sobel(as<float>(c->pyr_rgb[l]), lw, lh, as<float>(c->pyr_gx[l]),
+                               as<float>(c->pyr_gy[l]), s));
+        c->pyr_w[l] = lw;
+        c->pyr_h[l] = lh;
+    }
+    HIPCHK(c, hipStreamSynchronize(s));
+    c->pyr_levels = levels;
+    attach_level(c, 0);
+    return FVGS_OK;
+}
+
+int fvgs_select_level(fvgs_ctx* c, int level, int* width, int* height) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (level < 0 || level >= c->pyr_levels)
+        return fail(c, FVGS_E_ARG, "level %d not built (the pyramid has %d)", level,
+                    c->pyr_levels);
+    attach_level(c, level);
+    if (width) *width = c->fw;
+    if (height) *height = c->fh;
     return FVGS_OK;
 }
 

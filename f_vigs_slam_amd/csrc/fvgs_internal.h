@@ -143,6 +143,14 @@ hipError_t launch_adam(float* pos, float* scale, float* quat, float* color, floa
                        float beta2, float eps, hipStream_t s);
 
 // ---- tracking (track.hip) ----
+// Frame pyramid (GSSlam::initAndCopyImgs): cv::pyrDown of 8-bit 3-channel colour and of float
+// depth, and the 8-bit / 16-bit conversions to float.
+hipError_t launch_pyrdown_u8x3(const uint8_t* src, int sw, int sh, uint8_t* dst, int dw, int dh,
+                              hipStream_t s);
+hipError_t launch_pyrdown_f32(const float* src, int sw, int sh, float* dst, int dw, int dh,
+                              hipStream_t s);
+hipError_t launch_u8_to_f32(const uint8_t* src, float* dst, size_t n, hipStream_t s);
+hipError_t launch_u16_to_f32(const uint16_t* src, float* dst, size_t n, hipStream_t s);
 hipError_t launch_pose(const View& v, const View* vd, const uint32_t* ranges,
                        const uint32_t* idx, const Rec* rec, const float* obs_rgb,
                        const float* obs_depth, const float* gx, const float* gy,

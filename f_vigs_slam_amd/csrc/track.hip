@@ -474,7 +474,115 @@ __global__ __launch_bounds__(64) void k_gn_step(const double* __restrict__ sum27
     *status = 0;
 }
 
+// ---- tracking input pipeline: raw camera images -> float frame pyramid --------------------
+// GSSlam::initAndCopyImgs (GSSlam.cu:1913-1969): 8-bit colour and float (or 16-bit mm) depth,
+// cv::pyrDown per level (5x5 kernel [1 4 6 4 1]^T [1 4 6 4 1] / 256, reflect-101 borders, output
+// ((w+1)/2, (h+1)/2)), then colour x (1/255) as float. One thread per output pixel.
+
+// Reflect-101 index for any offset (loops for offsets beyond one image width).
+__device__ __forceinline__ int reflect101_any(int i, int n) {
+    if (n == 1) return 0;
+    while (i < 0 || i >= n) {
+        if (i < 0) i = -i;
+        if (i >= n) i = 2 * n - 2 - i;
+    }
+    return i;
+}
+
+// 8-bit, 3 channels: exact integer sums, rounded half up as OpenCV's fixed-point path does:
+// (sum + 128) >> 8.
+__global__ __launch_bounds__(256) void k_pyrdown_u8x3(const uint8_t* __restrict__ src, int sw,
+                                                      int sh, uint8_t* __restrict__ dst, int dw,
+                                                      int dh) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= dw || y >= dh) return;
+    int cx[5], ry[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        cx[k] = reflect101_any(2 * x + k - 2, sw);
+        ry[k] = reflect101_any(2 * y + k - 2, sh);
+    }
+    const int w5[5] = {1, 4, 6, 4, 1};
+    int acc[3] = {0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        const uint8_t* row = src + (size_t)ry[j] * sw * 3;
+        int r[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) r[c] += w5[k] * (int)row[3 * cx[k] + c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[c] += w5[j] * r[c];
+    }
+    uint8_t* o = dst + 3 * ((size_t)y * dw + x);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = (uint8_t)((acc[c] + 128) >> 8);
+}
+
+// float, 1 channel: per source row r = s0*6 + (s-1 + s+1)*4 + s-2 + s+2, then
+// (r0*6 + (r-1 + r+1)*4 + r-2 + r+2) * (1/256), in that order without contraction.
+__global__ __launch_bounds__(256) void k_pyrdown_f32(const float* __restrict__ src, int sw,
+                                                     int sh, float* __restrict__ dst, int dw,
+                                                     int dh) {
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= dw || y >= dh) return;
+    int cx[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) cx[k] = reflect101_any(2 * x + k - 2, sw);
+    float r[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        const float* row = src + (size_t)reflect101_any(2 * y + j - 2, sh) * sw;
+        r[j] = row[cx[2]] * 6.0f + (row[cx[1]] + row[cx[3]]) * 4.0f + row[cx[0]] + row[cx[4]];
+    }
+    dst[(size_t)y * dw + x] =
+        (r[2] * 6.0f + (r[1] + r[3]) * 4.0f + r[0] + r[4]) * (1.0f / 256.0f);
+}
+
+// 8-bit colour -> float: value x (1/255), one rounding (cv::Mat::convertTo(CV_32F, 1/255)).
+__global__ __launch_bounds__(256) void k_u8_to_f32(const uint8_t* __restrict__ src,
+                                                   float* __restrict__ dst, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = (float)src[i] * (1.0f / 255.0f);
+}
+
+// 16-bit millimetre depth -> float metres: value x 0.001f (convertTo(CV_32FC1, 0.001f)).
+__global__ __launch_bounds__(256) void k_u16_to_f32(const uint16_t* __restrict__ src,
+                                                    float* __restrict__ dst, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = (float)src[i] * 0.001f;
+}
+
 }  // namespace
+
+hipError_t launch_pyrdown_u8x3(const uint8_t* src, int sw, int sh, uint8_t* dst, int dw, int dh,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_pyrdown_u8x3, dim3((dw + 15) / 16, (dh + 15) / 16), dim3(256), 0, s, src,
+                       sw, sh, dst, dw, dh);
+    return hipGetLastError();
+}
+
+hipError_t launch_pyrdown_f32(const float* src, int sw, int sh, float* dst, int dw, int dh,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_pyrdown_f32, dim3((dw + 15) / 16, (dh + 15) / 16), dim3(256), 0, s, src,
+                       sw, sh, dst, dw, dh);
+    return hipGetLastError();
+}
+
+hipError_t launch_u8_to_f32(const uint8_t* src, float* dst, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_u8_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_u16_to_f32(const uint16_t* src, float* dst, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_u16_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst,
+                       n);
+    return hipGetLastError();
+}
 
 hipError_t launch_sobel(const float* img, int W, int H, float* gx, float* gy, hipStream_t s) {
     hipLaunchKernelGGL(k_sobel, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, img, W, H,

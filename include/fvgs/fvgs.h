@@ -160,7 +160,9 @@ enum fvgs_out {
     FVGS_OUT_SCALES = 55,      /* float[N][3] */
     FVGS_OUT_ORIENTATIONS = 56,/* float[N][4] (x, y, z, w) */
     FVGS_OUT_COLORS = 57,      /* float[N][3] */
-    FVGS_OUT_OPACITIES = 58    /* float[N] */
+    FVGS_OUT_OPACITIES = 58,   /* float[N] */
+    FVGS_OUT_FRAME_GX = 59,    /* float[H][W][3] the frame's Sobel x gradient (if attached) */
+    FVGS_OUT_FRAME_GY = 60     /* float[H][W][3] the frame's Sobel y gradient */
 };
 
 int fvgs_default_params(fvgs_params* params);
@@ -211,6 +213,25 @@ int fvgs_gradients_3d(fvgs_ctx* ctx);
  * fvgs_gradients_3d. Replaces GSSlam::optimizeGaussiansKeyframe minus updateGaussiansParametersAdam_kernel
  * (GSSlam.cu:1083-1264). */
 int fvgs_mapping_step(fvgs_ctx* ctx, const fvgs_pose* pose, const fvgs_intrinsics* intr);
+
+/* Raw-image frame pyramid (replaces GSSlam::initAndCopyImgs, GSSlam.cu:1913-1969, which runs on
+ * the host with OpenCV): 8-bit 3-channel colour (interleaved, channel order kept) and depth as
+ * float metres (FVGS_DEPTH_F32) or 16-bit millimetres (FVGS_DEPTH_U16_MM, converted x 0.001f as
+ * GSSlam.cu:1929-1930) are staged in HBM and `levels` pyramid levels are built on the device:
+ * cv::pyrDown per level (5x5 kernel [1 4 6 4 1]^T [1 4 6 4 1] / 256, reflect-101 borders,
+ * ((w+1)/2, (h+1)/2)) on the 8-bit colour (integer sums, rounded half up) and on the float depth;
+ * each level's colour as float x (1/255) and its Sobel gradients (3x3, reflect-101,
+ * GSSlam.cu:1959-1962). Level 0 becomes the current frame (as fvgs_set_frame with gradients).
+ * Inputs with FVGS_MEM_DEVICE are read in place (float depth at level 0 is referenced). */
+#define FVGS_DEPTH_F32 0
+#define FVGS_DEPTH_U16_MM 1
+#define FVGS_MAX_PYRAMID_LEVELS 6
+int fvgs_set_frame_raw(fvgs_ctx* ctx, int width, int height, const uint8_t* color3,
+                       const void* depth, int depth_format, int levels, int mem);
+/* Makes level `level` of the last fvgs_set_frame_raw pyramid the current frame (colour, depth,
+ * gradients) and returns its size; the caller scales the intrinsics by 2^-level as the
+ * reference does (GSSlam.cu:2133-2138). */
+int fvgs_select_level(fvgs_ctx* ctx, int level, int* width, int* height);
 
 /* Adam update of the map from the 3D gradients of the last mapping step (replaces
  * updateGaussiansParametersAdam_kernel, GSCudaKernels.cu:2722-2882, with the reference's

@@ -257,6 +257,41 @@ def sobel3(img):
     return gx, gy
 
 
+def pyrdown_u8(img):
+    """cv::pyrDown of an 8-bit (H, W) or (H, W, C) image (exact integer sums, round half up)."""
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape[:2]
+    cn = 1 if img.ndim == 2 else img.shape[2]
+    out = np.zeros(((h + 1) // 2, (w + 1) // 2) + img.shape[2:], np.uint8)
+    lib().orc_pyrdown_u8(_p(img), ctypes.c_int(w), ctypes.c_int(h), ctypes.c_int(cn), _p(out))
+    return out
+
+
+def pyrdown_f32(img):
+    """cv::pyrDown of a float (H, W) image in FP32, OpenCV's scalar operation order."""
+    img = _f32(img)
+    h, w = img.shape
+    out = np.zeros(((h + 1) // 2, (w + 1) // 2), np.float32)
+    lib().orc_pyrdown_f32(_p(img), ctypes.c_int(w), ctypes.c_int(h), _p(out))
+    return out
+
+
+def frame_pyramid(color_u8, depth, levels):
+    """GSSlam::initAndCopyImgs (GSSlam.cu:1913-1969): per level (rgb float, depth float, gx, gy).
+    depth: float32 metres, or uint16 millimetres (x 0.001f)."""
+    col = np.ascontiguousarray(color_u8, np.uint8)
+    d = np.asarray(depth)
+    d = d.astype(np.float32) * np.float32(0.001) if d.dtype == np.uint16 else _f32(d)
+    out = []
+    for lv in range(levels):
+        if lv > 0:
+            col, d = pyrdown_u8(col), pyrdown_f32(d)
+        rgb = col.astype(np.float32) * np.float32(1.0 / 255.0)
+        gx, gy = sobel3(rgb)
+        out.append((rgb, d, gx, gy))
+    return out
+
+
 def pipeline(scene, cam_pos, cam_quat, intr, width, height):
     """Binning (FP32) + value-mode records (FP64) for one view."""
     pf = project_f32(scene["pos"], scene["scale"], scene["quat"], cam_pos, cam_quat, intr,

@@ -1090,4 +1090,49 @@ ORC_API void orc_sobel3(int width, int height, const float* img, float* gx, floa
         }
 }
 
+// GSSlam::initAndCopyImgs (GSSlam.cu:1913-1969): cv::pyrDown as OpenCV documents it — 5x5
+// kernel [1 4 6 4 1]^T [1 4 6 4 1] / 256, BORDER_REFLECT_101, output ((w+1)/2, (h+1)/2).
+// 8-bit: exact integer sums rounded half up ((sum + 128) >> 8, the fixed-point cast of its 8-bit
+// path). float: per source row r = s0*6 + (s-1 + s+1)*4 + s-2 + s+2, then
+// (r0*6 + (r-1 + r+1)*4 + r-2 + r+2) * (1/256), in FP32 in that order (the operation order of
+// OpenCV's scalar pyrDown loop; this file is compiled with -ffp-contract=off).
+static int orc_reflect101(int i, int n) {
+    if (n == 1) return 0;
+    while (i < 0 || i >= n) { if (i < 0) i = -i; if (i >= n) i = 2 * n - 2 - i; }
+    return i;
+}
+
+ORC_API void orc_pyrdown_u8(const uint8_t* src, int w, int h, int cn, uint8_t* dst) {
+    const int dw = (w + 1) / 2, dh = (h + 1) / 2;
+    const int k5[5] = {1, 4, 6, 4, 1};
+    for (int y = 0; y < dh; y++)
+        for (int x = 0; x < dw; x++)
+            for (int c = 0; c < cn; c++) {
+                int acc = 0;
+                for (int j = 0; j < 5; j++) {
+                    const int sy = orc_reflect101(2 * y + j - 2, h);
+                    int r = 0;
+                    for (int k = 0; k < 5; k++)
+                        r += k5[k] * src[((size_t)sy * w + orc_reflect101(2 * x + k - 2, w)) * cn + c];
+                    acc += k5[j] * r;
+                }
+                dst[((size_t)y * dw + x) * cn + c] = (uint8_t)((acc + 128) >> 8);
+            }
+}
+
+ORC_API void orc_pyrdown_f32(const float* src, int w, int h, float* dst) {
+    const int dw = (w + 1) / 2, dh = (h + 1) / 2;
+    for (int y = 0; y < dh; y++)
+        for (int x = 0; x < dw; x++) {
+            int cx[5];
+            for (int k = 0; k < 5; k++) cx[k] = orc_reflect101(2 * x + k - 2, w);
+            float r[5];
+            for (int j = 0; j < 5; j++) {
+                const float* row = src + (size_t)orc_reflect101(2 * y + j - 2, h) * w;
+                r[j] = row[cx[2]] * 6.0f + (row[cx[1]] + row[cx[3]]) * 4.0f + row[cx[0]] + row[cx[4]];
+            }
+            dst[(size_t)y * dw + x] = (r[2] * 6.0f + (r[1] + r[3]) * 4.0f + r[0] + r[4]) * (1.0f / 256.0f);
+        }
+}
+
 ORC_API int orc_num_threads(void) { return omp_get_max_threads(); }

@@ -4,6 +4,8 @@
 // tests/test_gpu_shim.py through files in a scratch directory.
 //
 //   shim_driver DIR pose      scene.bin frame.bin pose.bin -> out.bin: JtJ[36], Jtr[6] (double)
+//   shim_driver DIR pose_raw  the same at pyramid level L of initAndCopyImgs(raw.bin):
+//                             raw.bin = int32 w, h, L; uint8 colour[3wh]; uint16 depth_mm[wh]
 //   shim_driver DIR optimize  scene.bin frame.bin pose.bin -> out.bin: for each of two
 //                             optimizeGaussiansKeyframe calls, positions[N][3] and colors[N][3]
 //                             (float) after it, then the call's delta3d[N]
@@ -44,7 +46,7 @@ const T* take(const std::vector<char>& buf, size_t& off, size_t count) {
 
 int main(int argc, char** argv) {
     if (argc != 3) {
-        std::fprintf(stderr, "usage: shim_driver DIR pose|optimize\n");
+        std::fprintf(stderr, "usage: shim_driver DIR pose|pose_raw|optimize\n");
         return 2;
     }
     const std::string dir = argv[1], mode = argv[2];
@@ -96,6 +98,19 @@ int main(int argc, char** argv) {
         slam.setFramePyramid({lv});
         double JtJ[36], Jtr[6];
         slam.computeRgbdPoseJacobians(JtJ, Jtr, 0, P_imu, Q_imu);
+        std::fwrite(JtJ, sizeof(double), 36, out);
+        std::fwrite(Jtr, sizeof(double), 6, out);
+    } else if (mode == "pose_raw") {
+        std::vector<char> rb;
+        if (!read_all(dir + "/raw.bin", rb)) return 2;
+        size_t k = 0;
+        const int32_t* hdr = take<int32_t>(rb, k, 3);   // w, h, level
+        const uint8_t* col = take<uint8_t>(rb, k, 3ull * hdr[0] * hdr[1]);
+        const uint16_t* dep = take<uint16_t>(rb, k, (size_t)hdr[0] * hdr[1]);
+        if (!dep) return 2;
+        slam.initAndCopyImgs(hdr[0], hdr[1], col, dep, FVGS_DEPTH_U16_MM);
+        double JtJ[36], Jtr[6];
+        slam.computeRgbdPoseJacobians(JtJ, Jtr, hdr[2], P_imu, Q_imu);
         std::fwrite(JtJ, sizeof(double), 36, out);
         std::fwrite(Jtr, sizeof(double), 6, out);
     } else if (mode == "optimize") {

@@ -5,7 +5,8 @@ results are checked against an independent restatement.
 * computeRgbdPoseJacobians (GSSlam.cu:2103-2244): the IMU -> camera transform, the float4(w, x,
   y, z) packing of the camera quaternion, the negated Jtr and the J_cam_imu change of frame are
   restated in numpy/scipy around the Python handle's fvgs_pose_jacobians on the same frame; the
-  two agree to 1e-12 relative (the same kernels, different host arithmetic order).
+  two agree to 1e-12 relative (the same kernels, different host arithmetic order). Also from
+  initAndCopyImgs (8-bit colour, millimetre depth) at pyramid level 1.
 * optimizeGaussiansKeyframe(keyframe, eta) twice, then getGaussianData: positions and colours
   against the oracle's Adam fed each call's own 3D gradients (bar 2e-6 (1 + |x|), as for
   fvgs_adam_step).
@@ -67,10 +68,20 @@ def _skew(t):
     return np.array([[0, -t[2], t[1]], [t[2], 0, -t[0]], [-t[1], t[0], 0]])
 
 
-def test_shim_pose_jacobians_match_restated_host_conventions(tmp_path):
+@pytest.mark.parametrize("raw_level", [None, 1])
+def test_shim_pose_jacobians_match_restated_host_conventions(tmp_path, raw_level):
+    """raw_level None: setFramePyramid with float images; 1: initAndCopyImgs from 8-bit colour
+    and millimetre depth (device pyramid), Jacobians at level 1 with intrinsics scaled by 1/2."""
     scene = S.make_scene(N, seed=21, width=W, height=H)
     intr = S.intrinsics(W, H)
     rgb, depth = _frame(scene, intr)
+    if raw_level is not None:
+        col = np.clip(rgb * 255.0 + 0.5, 0, 255).astype(np.uint8)
+        dmm = np.clip(depth * 1000.0, 0, 65535).astype(np.uint16)
+        with open(os.path.join(str(tmp_path), "raw.bin"), "wb") as f:
+            f.write(np.array([W, H, raw_level], np.int32).tobytes())
+            f.write(col.tobytes())
+            f.write(dmm.tobytes())
     t_ic = np.array([0.05, -0.02, 0.01])
     q_ic = Rot.from_rotvec([0.02, -0.01, 0.03]).as_quat()          # (x, y, z, w)
     # IMU pose such that the kernels see a view near the identity: they read the packed
@@ -80,7 +91,8 @@ def test_shim_pose_jacobians_match_restated_host_conventions(tmp_path):
     P_imu = np.array([0.01, -0.01, 0.02]) - Q_imu.apply(t_ic)
     _write_inputs(str(tmp_path), scene, rgb, depth, intr, t_ic, q_ic, P_imu, Q_imu.as_quat(),
                   0.0, np.zeros(7))
-    out = np.frombuffer(_run(str(tmp_path), "pose"), np.float64)
+    out = np.frombuffer(_run(str(tmp_path), "pose" if raw_level is None else "pose_raw"),
+                        np.float64)
     JtJ_shim, Jtr_shim = out[:36].reshape(6, 6), out[36:42]
 
     # GSSlam.cu:2113-2121 restated: P_cam = P_imu + Q_imu t_ic, Q_cam = Q_imu q_ic, packed
@@ -91,9 +103,15 @@ def test_shim_pose_jacobians_match_restated_host_conventions(tmp_path):
     pose7 = np.concatenate([P_cam.astype(np.float32), slot]).astype(np.float64)
     with Context() as c:
         c.set_scene(scene)
-        c.set_frame(rgb, depth)
-        c.compute_gradients()
-        jtj21, jtr6 = c.pose_jacobians(pose7, intr)
+        if raw_level is None:
+            c.set_frame(rgb, depth)
+            c.compute_gradients()
+            jtj21, jtr6 = c.pose_jacobians(pose7, intr)
+        else:                                                        # GSSlam.cu:2133-2138
+            c.set_frame_raw(col, dmm, levels=3)
+            c.select_level(raw_level)
+            s = np.float32(1.0 / (1 << raw_level))
+            jtj21, jtr6 = c.pose_jacobians(pose7, np.asarray(intr, np.float32) * s)
     A = np.zeros((6, 6))
     A[np.triu_indices(6)] = jtj21
     A = A + np.triu(A, 1).T
