@@ -30,7 +30,8 @@ OUT = dict(POS2D=1, COV2D=2, CONIC=3, DEPTH_G=4, PHAT=5, KEYS=6, INDICES=7, RANG
            N_CONTRIB=23, COLOR_ERROR=24, MAX_CONTRIB=25, DELTA2D=30, DELTA3D=31,
            POSE_TILES=40, TRACK_RGB=41, TRACK_DEPTH=42, MAP=50, ADAM_STATE=51,
            FRAME_RGB=52, FRAME_DEPTH=53, POSITIONS=54, SCALES=55, ORIENTATIONS=56,
-           COLORS=57, OPACITIES=58, FRAME_GX=59, FRAME_GY=60)
+           COLORS=57, OPACITIES=58, FRAME_GX=59, FRAME_GY=60, VISIBILITY=61,
+           DENSITY_MASK=62, OUTLIER_PROB=63, TOTAL_ALPHA=64, NORMALS=65)
 DEPTH_F32, DEPTH_U16_MM = 0, 1
 
 # Every function the header declares (checked by the CPU test suite against the library).
@@ -40,7 +41,9 @@ EXPORTS = [
     "fvgs_render", "fvgs_forward", "fvgs_backward", "fvgs_gradients_3d", "fvgs_mapping_step",
     "fvgs_pose_jacobians", "fvgs_render_tracking", "fvgs_compute_gradients", "fvgs_gn_step", "fvgs_track", "fvgs_read", "fvgs_device_ptr",
     "fvgs_sync", "fvgs_stats", "fvgs_set_timing", "fvgs_last_timing", "fvgs_adam_step",
-    "fvgs_optimize_keyframe", "fvgs_set_frame_raw", "fvgs_select_level",
+    "fvgs_optimize_keyframe", "fvgs_set_frame_raw", "fvgs_select_level", "fvgs_visibility",
+    "fvgs_covisibility", "fvgs_prune", "fvgs_remove_outliers", "fvgs_density_mask",
+    "fvgs_densify", "fvgs_init_from_rgbd",
     # fvgs_server.h
     "fvgs_server_default_params", "fvgs_server_create", "fvgs_server_destroy",
     "fvgs_server_last_error", "fvgs_server_open_session", "fvgs_server_close_session",
@@ -140,6 +143,15 @@ def load():
         "fvgs_adam_step": ([vp, f], i),
         "fvgs_set_frame_raw": ([vp, i, i, vp, vp, i, i, i], i),
         "fvgs_select_level": ([vp, i, P(i), P(i)], i),
+        "fvgs_visibility": ([vp, P(Pose), P(Intrinsics), i, i], i),
+        "fvgs_covisibility": ([vp, P(Pose), P(Intrinsics), i, i, P(Pose), P(Intrinsics), i, i,
+                               P(u32), P(u32), P(f)], i),
+        "fvgs_prune": ([vp, f, f, P(u32)], i),
+        "fvgs_remove_outliers": ([vp, P(Pose), P(Intrinsics), f, P(u32)], i),
+        "fvgs_density_mask": ([vp, P(Pose), P(Intrinsics)], i),
+        "fvgs_densify": ([vp, P(Pose), P(Intrinsics), vp, i, i, u32, P(u32)], i),
+        "fvgs_init_from_rgbd": ([vp, i, i, vp, vp, i, i, P(Pose), P(Intrinsics), i, f, f, u32,
+                                 P(u32)], i),
         "fvgs_optimize_keyframe": ([vp, P(Pose), P(Intrinsics), f], i),
         "fvgs_server_default_params": ([P(ServerParams)], i),
         "fvgs_server_create": ([P(ServerParams), P(vp)], i),

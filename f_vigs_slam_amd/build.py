@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libfvgs.so")
-SOURCES = ["binning.hip", "raster.hip", "track.hip", "fvgs_api.cpp", "fvgs_server.cpp"]
+SOURCES = ["binning.hip", "raster.hip", "track.hip", "maint.hip", "fvgs_api.cpp",
+           "fvgs_server.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

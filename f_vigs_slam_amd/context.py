@@ -170,6 +170,85 @@ class Context:
         self.frame_size = (w.value, h.value)
         return self.frame_size
 
+    # -- map maintenance -----------------------------------------------------------------
+    def visibility(self, pose7, intr, width, height):
+        """fvgs_visibility: uint8[N] marks of the Gaussians visible in the view."""
+        self._check(self._lib.fvgs_visibility(self._h, ctypes.byref(make_pose(pose7)),
+                                              ctypes.byref(make_intr(intr)), width, height))
+        self._vis_n = self.n
+        return self.read("VISIBILITY")
+
+    def covisibility(self, kf_pose7, kf_intr, kf_size, pose7, intr, size):
+        """fvgs_covisibility: (inter, union, ratio) of the keyframe view and the current view."""
+        a, b, r = ctypes.c_uint32(0), ctypes.c_uint32(0), ctypes.c_float(0)
+        self._check(self._lib.fvgs_covisibility(
+            self._h, ctypes.byref(make_pose(kf_pose7)), ctypes.byref(make_intr(kf_intr)),
+            kf_size[0], kf_size[1], ctypes.byref(make_pose(pose7)), ctypes.byref(make_intr(intr)),
+            size[0], size[1], ctypes.byref(a), ctypes.byref(b), ctypes.byref(r)))
+        self._vis_n = self.n
+        return a.value, b.value, r.value
+
+    def _count(self):
+        n = ctypes.c_uint32(0)
+        self._check(self._lib.fvgs_gaussian_count(self._h, ctypes.byref(n)))
+        self.n = n.value
+        return self.n
+
+    def prune(self, alpha_threshold=0.05, scale_ratio_threshold=0.05):
+        """fvgs_prune (GSSlam::prune thresholds by default); returns the number removed."""
+        r = ctypes.c_uint32(0)
+        self._check(self._lib.fvgs_prune(self._h, float(alpha_threshold),
+                                         float(scale_ratio_threshold), ctypes.byref(r)))
+        self._count()
+        return r.value
+
+    def remove_outliers(self, pose7, intr, threshold=0.6):
+        """fvgs_remove_outliers against the current frame; returns the number removed."""
+        r = ctypes.c_uint32(0)
+        n_before = self.n
+        self._check(self._lib.fvgs_remove_outliers(self._h, ctypes.byref(make_pose(pose7)),
+                                                   ctypes.byref(make_intr(intr)), float(threshold),
+                                                   ctypes.byref(r)))
+        self._outl_n = n_before
+        self._count()
+        return r.value
+
+    def density_mask(self, pose7, intr):
+        """fvgs_density_mask against the current frame: float (H, W)."""
+        self._check(self._lib.fvgs_density_mask(self._h, ctypes.byref(make_pose(pose7)),
+                                                ctypes.byref(make_intr(intr))))
+        return self.read("DENSITY_MASK")
+
+    def densify(self, pose7, intr, color_u8, sample_px=10, max_gaussians=1 << 24):
+        """fvgs_densify with the keyframe's 8-bit colour; returns the number added."""
+        col = np.ascontiguousarray(color_u8, np.uint8)
+        r = ctypes.c_uint32(0)
+        self._check(self._lib.fvgs_densify(self._h, ctypes.byref(make_pose(pose7)),
+                                           ctypes.byref(make_intr(intr)), L.ptr(col), L.MEM_HOST,
+                                           int(sample_px), int(max_gaussians), ctypes.byref(r)))
+        self._count()
+        return r.value
+
+    def init_from_rgbd(self, color_u8, depth, pose7, intr, sample_px=10, init_scale=2.0,
+                       init_opacity=0.8, max_gaussians=1 << 24):
+        """fvgs_init_from_rgbd (the reference's defaults); returns the number created."""
+        col = np.ascontiguousarray(color_u8, np.uint8)
+        depth = np.ascontiguousarray(depth)
+        if depth.dtype == np.uint16:
+            fmt = L.DEPTH_U16_MM
+        else:
+            depth = np.ascontiguousarray(depth, np.float32)
+            fmt = L.DEPTH_F32
+        h, w = depth.shape
+        r = ctypes.c_uint32(0)
+        self._check(self._lib.fvgs_init_from_rgbd(
+            self._h, w, h, L.ptr(col), L.ptr(depth), fmt, L.MEM_HOST,
+            ctypes.byref(make_pose(pose7)), ctypes.byref(make_intr(intr)), int(sample_px),
+            float(init_scale), float(init_opacity), int(max_gaussians), ctypes.byref(r)))
+        self._nrm_size = (w, h)
+        self._count()
+        return r.value
+
     # -- compute -------------------------------------------------------------------------
     def prepare_rasterization(self, pose7, intr, width, height):
         m = ctypes.c_uint32(0)
@@ -302,7 +381,12 @@ class Context:
                       POSITIONS=((n, 3), np.float32), SCALES=((n, 3), np.float32),
                       ORIENTATIONS=((n, 4), np.float32), COLORS=((n, 3), np.float32),
                       OPACITIES=((n,), np.float32), FRAME_GX=((fh, fw, 3), np.float32),
-                      FRAME_GY=((fh, fw, 3), np.float32))
+                      FRAME_GY=((fh, fw, 3), np.float32),
+                      VISIBILITY=((getattr(self, "_vis_n", n),), np.uint8),
+                      DENSITY_MASK=((fh, fw), np.float32),
+                      OUTLIER_PROB=((getattr(self, "_outl_n", n),), np.float32),
+                      TOTAL_ALPHA=((getattr(self, "_outl_n", n),), np.float32),
+                      NORMALS=(getattr(self, "_nrm_size", (0, 0))[::-1] + (3,), np.float32))
         shape, dt = shapes[what]
         out = np.zeros(shape, dt)
         self._check(self._lib.fvgs_read(self._h, L.OUT[what], L.ptr(out), out.nbytes))

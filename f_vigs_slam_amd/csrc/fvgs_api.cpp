@@ -66,6 +66,12 @@ struct fvgs_ctx {
     const float* pyr_depth_ptr[kLevels] = {};          // level 0 may be the caller's array
     int pyr_w[kLevels] = {}, pyr_h[kLevels] = {};
     int pyr_levels = 0;
+    // map maintenance
+    DevBuf vis, vis2, keep, boff, outl_prob, outl_total, dmask, cand, ckeep, m_color, m_depth,
+        normals;
+    DevBuf alt_pos, alt_scale, alt_quat, alt_color, alt_opac, alt_adam;
+    uint32_t vis_n = 0, outl_n = 0;
+    int mask_w = 0, mask_h = 0, nrm_w = 0, nrm_h = 0;
     // view
     bool prepared = false, forwarded = false, backwarded = false;
     View view{};
@@ -446,6 +452,11 @@ OutDesc describe(fvgs_ctx* c, int what) {
             return {const_cast<float*>(c->obs_depth), 4ull * c->fw * c->fh};
         case FVGS_OUT_FRAME_GX: return {const_cast<float*>(c->gx), 12ull * c->fw * c->fh};
         case FVGS_OUT_FRAME_GY: return {const_cast<float*>(c->gy), 12ull * c->fw * c->fh};
+        case FVGS_OUT_VISIBILITY: return {c->vis.p, (size_t)c->vis_n};
+        case FVGS_OUT_DENSITY_MASK: return {c->dmask.p, 4ull * c->mask_w * c->mask_h};
+        case FVGS_OUT_OUTLIER_PROB: return {c->outl_prob.p, 4ull * c->outl_n};
+        case FVGS_OUT_TOTAL_ALPHA: return {c->outl_total.p, 4ull * c->outl_n};
+        case FVGS_OUT_NORMALS: return {c->normals.p, 12ull * c->nrm_w * c->nrm_h};
         case FVGS_OUT_POSITIONS: return {const_cast<float*>(c->pos), 12 * n};
         case FVGS_OUT_SCALES: return {const_cast<float*>(c->scale), 12 * n};
         case FVGS_OUT_ORIENTATIONS: return {const_cast<float*>(c->quat), 16 * n};
@@ -535,6 +546,11 @@ int fvgs_destroy(fvgs_ctx* c) {
         for (DevBuf* b : bufs) release(c, *b);
         release(c, c->raw_color);
         release(c, c->raw_depth);
+        DevBuf* mbufs[] = {&c->vis, &c->vis2, &c->keep, &c->boff, &c->outl_prob, &c->outl_total,
+                           &c->dmask, &c->cand, &c->ckeep, &c->m_color, &c->m_depth,
+                           &c->normals, &c->alt_pos, &c->alt_scale, &c->alt_quat, &c->alt_color,
+                           &c->alt_opac, &c->alt_adam};
+        for (DevBuf* b : mbufs) release(c, *b);
         for (int l = 0; l < fvgs_ctx::kLevels; l++) {
             release(c, c->pyr_u8[l]);
             release(c, c->pyr_rgb[l]);
@@ -893,6 +909,342 @@ int fvgs_optimize_keyframe(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrin
     if (!c) return FVGS_E_ARG;
     TRY(fvgs_mapping_step(c, pose, intr));
     return fvgs_adam_step(c, eta);
+}
+
+// ---- map maintenance -------------------------------------------------------------------------
+namespace {
+
+MapArrays map_arrays(fvgs_ctx* c) {
+    return MapArrays{const_cast<float*>(c->pos), const_cast<float*>(c->scale),
+                     const_cast<float*>(c->quat), const_cast<float*>(c->color),
+                     const_cast<float*>(c->opac)};
+}
+
+int need_plain_map(fvgs_ctx* c, const char* what) {
+    if (c->prm.precision != FVGS_PRECISION_F32)
+        return fail(c, FVGS_E_STATE, "%s needs the FP32 map (plain form)", what);
+    return FVGS_OK;
+}
+
+// The map in the context's own buffers with room for `cap` Gaussians, keeping the first `keep`
+// (a caller's FVGS_MEM_DEVICE map is copied in: Gaussians are then added there).
+int own_map(fvgs_ctx* c, size_t cap, size_t keep) {
+    DevBuf* own[5] = {&c->own_pos, &c->own_scale, &c->own_quat, &c->own_color, &c->own_opac};
+    const float** cur[5] = {&c->pos, &c->scale, &c->quat, &c->color, &c->opac};
+    const size_t per[5] = {12, 12, 16, 12, 4};
+    for (int k = 0; k < 5; k++) {
+        if (*cur[k] == own[k]->p && own[k]->bytes >= per[k] * cap) continue;
+        DevBuf nb;
+        TRY(ensure(c, nb, per[k] * cap));
+        if (keep && *cur[k])
+            HIPCHK(c, hipMemcpyAsync(nb.p, *cur[k], per[k] * keep, hipMemcpyDeviceToDevice,
+                                     c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        release(c, *own[k]);
+        *own[k] = nb;
+        *cur[k] = as<float>(*own[k]);
+    }
+    return FVGS_OK;
+}
+
+// Keeps the Gaussians flagged in c->keep, in order (and their Adam states). Returns the count
+// removed. The reference sorts by state and shrinks the count (GSSlam.cu:1536-1557).
+int compact_map(fvgs_ctx* c, uint32_t* removed) {
+    const int n = (int)c->n;
+    const int nb = (n + 255) / 256;
+    TRY(ensure(c, c->boff, 4ull * (nb + 1)));
+    HIPCHK(c, launch_flag_offsets(as<uint8_t>(c->keep), n, as<uint32_t>(c->boff), c->stream));
+    uint32_t kept = 0;
+    HIPCHK(c, hipMemcpyAsync(&kept, as<uint32_t>(c->boff) + nb, 4, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (removed) *removed = (uint32_t)n - kept;
+    if (kept == (uint32_t)n) return FVGS_OK;
+    DevBuf* alt[5] = {&c->alt_pos, &c->alt_scale, &c->alt_quat, &c->alt_color, &c->alt_opac};
+    const size_t per[5] = {12, 12, 16, 12, 4};
+    for (int k = 0; k < 5; k++) TRY(ensure(c, *alt[k], per[k] * n));
+    const bool with_adam = c->adam_zeroed;
+    if (with_adam) TRY(ensure(c, c->alt_adam, sizeof(AdamState) * n));
+    MapArrays dst{as<float>(c->alt_pos), as<float>(c->alt_scale), as<float>(c->alt_quat),
+                  as<float>(c->alt_color), as<float>(c->alt_opac)};
+    HIPCHK(c, launch_compact_map(as<uint8_t>(c->keep), n, as<uint32_t>(c->boff), map_arrays(c),
+                                 dst, with_adam ? as<AdamState>(c->adam) : nullptr,
+                                 with_adam ? as<AdamState>(c->alt_adam) : nullptr, c->stream));
+    DevBuf* own[5] = {&c->own_pos, &c->own_scale, &c->own_quat, &c->own_color, &c->own_opac};
+    const float** cur[5] = {&c->pos, &c->scale, &c->quat, &c->color, &c->opac};
+    for (int k = 0; k < 5; k++) {
+        if (*cur[k] == own[k]->p) {   // own storage: the compacted copy becomes the map
+            std::swap(*own[k], *alt[k]);
+            *cur[k] = as<float>(*own[k]);
+        } else {                      // the caller's arrays: compacted in place
+            HIPCHK(c, hipMemcpyAsync(const_cast<float*>(*cur[k]), alt[k]->p, per[k] * kept,
+                                     hipMemcpyDeviceToDevice, c->stream));
+        }
+    }
+    if (with_adam) std::swap(c->adam, c->alt_adam);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->n = kept;
+    c->prepared = c->forwarded = c->backwarded = c->grads3d = false;
+    return FVGS_OK;
+}
+
+// Projection + binning of a view for the maintenance walks; *empty when it has no instances
+// (the ranges are then all empty and the walks still run, as in the reference).
+int maint_view(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int w, int h,
+               bool* empty) {
+    const int r = prepare(c, pose, intr, w, h, false, c->prm.cull_emit);
+    *empty = r == FVGS_E_EMPTY && c->n > 0;
+    if (r != FVGS_OK && !*empty) return r;
+    return FVGS_OK;
+}
+
+int run_visibility(fvgs_ctx* c, DevBuf& vis, const fvgs_pose* pose, const fvgs_intrinsics* intr,
+                   int w, int h, bool* empty) {
+    TRY(ensure(c, vis, c->n));
+    HIPCHK(c, launch_zero(vis.p, c->n, c->stream));
+    TRY(maint_view(c, pose, intr, w, h, empty));
+    if (!*empty)
+        HIPCHK(c, launch_visibility(c->view, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
+                                    as<Rec>(c->rec), map_ref(c), as<uint8_t>(vis), c->stream));
+    return FVGS_OK;
+}
+
+int run_density_mask(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr) {
+    if (!c->obs_depth) return fail(c, FVGS_E_STATE, "no frame set");
+    bool empty = false;
+    TRY(maint_view(c, pose, intr, c->fw, c->fh, &empty));
+    TRY(ensure(c, c->dmask, 4ull * c->fw * c->fh));
+    HIPCHK(c, launch_density_mask(c->view, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
+                                  as<Rec>(c->rec), map_ref(c), c->obs_depth, as<float>(c->dmask),
+                                  c->stream));
+    c->mask_w = c->fw;
+    c->mask_h = c->fh;
+    return FVGS_OK;
+}
+
+// Uploads an 8-bit colour image (host) or references it (device).
+int colour_in(fvgs_ctx* c, const uint8_t* color3, size_t P, int mem, const uint8_t** out) {
+    if (mem == FVGS_MEM_DEVICE) {
+        *out = color3;
+        return FVGS_OK;
+    }
+    TRY(ensure(c, c->m_color, 3 * P));
+    HIPCHK(c, hipMemcpyAsync(c->m_color.p, color3, 3 * P, hipMemcpyHostToDevice, c->stream));
+    *out = as<uint8_t>(c->m_color);
+    return FVGS_OK;
+}
+
+// Candidates in c->cand / c->ckeep -> appended after the first n0 Gaussians, at most `cap` in
+// total. Returns the number appended.
+int append_candidates(fvgs_ctx* c, int ncell, uint32_t n0, uint32_t cap, uint32_t* added) {
+    const int nb = (ncell + 255) / 256;
+    TRY(ensure(c, c->boff, 4ull * (nb + 1)));
+    HIPCHK(c, launch_flag_offsets(as<uint8_t>(c->ckeep), ncell, as<uint32_t>(c->boff), c->stream));
+    uint32_t emitted = 0;
+    HIPCHK(c, hipMemcpyAsync(&emitted, as<uint32_t>(c->boff) + nb, 4, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const uint32_t n1 = (uint32_t)std::min<uint64_t>((uint64_t)n0 + emitted, std::max(cap, n0));
+    TRY(own_map(c, std::max<uint32_t>(n1, 1u), n0));
+    AdamState* adam = nullptr;
+    if (c->adam_zeroed) {   // existing moments kept, new Gaussians start at zero
+        DevBuf nb2;
+        TRY(ensure(c, nb2, sizeof(AdamState) * std::max<uint32_t>(n1, 1u)));
+        if (n0)
+            HIPCHK(c, hipMemcpyAsync(nb2.p, c->adam.p, sizeof(AdamState) * n0,
+                                     hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        release(c, c->adam);
+        c->adam = nb2;
+        adam = as<AdamState>(c->adam);
+    }
+    HIPCHK(c, launch_append_cands(as<uint8_t>(c->ckeep), ncell, as<uint32_t>(c->boff),
+                                  as<float>(c->cand), n0, n1, map_arrays(c), adam, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->n = n1;
+    c->prepared = c->forwarded = c->backwarded = c->grads3d = false;
+    if (added) *added = n1 - n0;
+    return FVGS_OK;
+}
+
+}  // namespace
+
+int fvgs_visibility(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int w,
+                    int h) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->prm.device);
+    if (c->n == 0) return fail(c, FVGS_E_EMPTY, "no Gaussians");
+    bool empty = false;
+    TRY(run_visibility(c, c->vis, pose, intr, w, h, &empty));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->vis_n = c->n;
+    return FVGS_OK;
+}
+
+int fvgs_covisibility(fvgs_ctx* c, const fvgs_pose* kf_pose, const fvgs_intrinsics* kf_intr,
+                      int kf_w, int kf_h, const fvgs_pose* pose, const fvgs_intrinsics* intr,
+                      int w, int h, uint32_t* inter, uint32_t* uni, float* ratio) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->prm.device);
+    if (inter) *inter = 0;
+    if (uni) *uni = 0;
+    if (ratio) *ratio = 0.f;
+    if (c->n == 0) return fail(c, FVGS_E_EMPTY, "no Gaussians");
+    bool e1 = false, e2 = false;
+    TRY(run_visibility(c, c->vis2, kf_pose, kf_intr, kf_w, kf_h, &e1));
+    TRY(run_visibility(c, c->vis, pose, intr, w, h, &e2));
+    c->vis_n = c->n;
+    uint32_t counts[2] = {0, 0};
+    if (!e1 && !e2) {   // the reference returns 0 when either view has no instances
+        TRY(ensure(c, c->boff, 16));
+        HIPCHK(c, launch_zero(c->boff.p, 8, c->stream));
+        HIPCHK(c, launch_covisibility(as<uint8_t>(c->vis2), as<uint8_t>(c->vis), (int)c->n,
+                                      as<uint32_t>(c->boff), c->stream));
+        HIPCHK(c, hipMemcpyAsync(counts, c->boff.p, 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (inter) *inter = counts[0];
+    if (uni) *uni = counts[1];
+    if (ratio) *ratio = counts[1] ? (float)counts[0] / (float)counts[1] : 0.f;
+    return FVGS_OK;
+}
+
+int fvgs_prune(fvgs_ctx* c, float alpha_threshold, float scale_ratio_threshold,
+               uint32_t* removed) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->prm.device);
+    if (removed) *removed = 0;
+    TRY(need_plain_map(c, "prune"));
+    if (c->n == 0) return FVGS_OK;
+    TRY(ensure(c, c->keep, c->n));
+    HIPCHK(c, launch_prune_keep(c->scale, c->opac, (int)c->n, alpha_threshold,
+                                scale_ratio_threshold, as<uint8_t>(c->keep), c->stream));
+    return compact_map(c, removed);
+}
+
+int fvgs_remove_outliers(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr,
+                         float threshold, uint32_t* removed) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->prm.device);
+    if (removed) *removed = 0;
+    TRY(need_plain_map(c, "outlier removal"));
+    if (c->n == 0) return FVGS_OK;
+    if (!c->obs_depth) return fail(c, FVGS_E_STATE, "no frame set");
+    const int n = (int)c->n;
+    TRY(ensure(c, c->outl_prob, 4ull * n));
+    TRY(ensure(c, c->outl_total, 4ull * n));
+    TRY(ensure(c, c->keep, n));
+    HIPCHK(c, launch_zero(c->outl_prob.p, 4ull * n, c->stream));
+    HIPCHK(c, launch_zero(c->outl_total.p, 4ull * n, c->stream));
+    bool empty = false;
+    TRY(maint_view(c, pose, intr, c->fw, c->fh, &empty));
+    if (!empty)
+        HIPCHK(c, launch_outliers(c->view, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
+                                  as<Rec>(c->rec), map_ref(c), c->obs_depth,
+                                  as<float>(c->outl_prob), as<float>(c->outl_total), c->stream));
+    c->outl_n = (uint32_t)n;
+    HIPCHK(c, launch_outlier_keep(as<float>(c->outl_prob), as<float>(c->outl_total), n,
+                                  threshold, as<uint8_t>(c->keep), c->stream));
+    return compact_map(c, removed);
+}
+
+int fvgs_density_mask(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr) {
+    if (!c) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->prm.device);
+    if (c->n == 0) return fail(c, FVGS_E_EMPTY, "no Gaussians");
+    TRY(run_density_mask(c, pose, intr));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FVGS_OK;
+}
+
+int fvgs_densify(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr,
+                 const uint8_t* color3, int mem, int sample_px, uint32_t max_gaussians,
+                 uint32_t* added) {
+    if (!c || !color3 || sample_px < 1) return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->prm.device);
+    if (added) *added = 0;
+    if (mem != FVGS_MEM_HOST && mem != FVGS_MEM_DEVICE) return fail(c, FVGS_E_ARG, "bad mem kind");
+    TRY(need_plain_map(c, "densify"));
+    if (c->n == 0) return FVGS_OK;   // the reference returns early (GSSlam.cu:1357-1360)
+    TRY(run_density_mask(c, pose, intr));
+    const size_t P = (size_t)c->fw * c->fh;
+    const uint8_t* col = nullptr;
+    TRY(colour_in(c, color3, P, mem, &col));
+    const int sw = (c->fw + sample_px - 1) / sample_px, sh = (c->fh + sample_px - 1) / sample_px;
+    const int ncell = sw * sh;
+    TRY(ensure(c, c->cand, 56ull * ncell));
+    TRY(ensure(c, c->ckeep, (size_t)ncell));
+    HIPCHK(c, launch_densify_cells(c->view, as<float>(c->dmask), col, c->obs_depth, sample_px,
+                                   sample_px, as<float>(c->cand), as<uint8_t>(c->ckeep),
+                                   c->stream));
+    return append_candidates(c, ncell, c->n, max_gaussians, added);
+}
+
+int fvgs_init_from_rgbd(fvgs_ctx* c, int w, int h, const uint8_t* color3, const void* depth,
+                        int depth_format, int mem, const fvgs_pose* pose,
+                        const fvgs_intrinsics* intr, int sample_px, float init_scale,
+                        float init_opacity, uint32_t max_gaussians, uint32_t* count) {
+    if (!c || !color3 || !depth || !pose || !intr || sample_px < 1 || w <= 0 || h <= 0)
+        return FVGS_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->prm.device);
+    if (count) *count = 0;
+    if (mem != FVGS_MEM_HOST && mem != FVGS_MEM_DEVICE) return fail(c, FVGS_E_ARG, "bad mem kind");
+    if (depth_format != FVGS_DEPTH_F32 && depth_format != FVGS_DEPTH_U16_MM)
+        return fail(c, FVGS_E_ARG, "bad depth format %d", depth_format);
+    TRY(need_plain_map(c, "initialisation"));
+    const size_t P = (size_t)w * h;
+    const uint8_t* col = nullptr;
+    TRY(colour_in(c, color3, P, mem, &col));
+    const float* dep = static_cast<const float*>(depth);
+    if (depth_format == FVGS_DEPTH_U16_MM || mem == FVGS_MEM_HOST) {
+        TRY(ensure(c, c->m_depth, 4 * P));
+        if (depth_format == FVGS_DEPTH_U16_MM) {
+            const uint16_t* d16 = static_cast<const uint16_t*>(depth);
+            if (mem == FVGS_MEM_HOST) {
+                TRY(ensure(c, c->raw_depth, 2 * P));
+                HIPCHK(c, hipMemcpyAsync(c->raw_depth.p, depth, 2 * P, hipMemcpyHostToDevice,
+                                         c->stream));
+                d16 = as<uint16_t>(c->raw_depth);
+            }
+            HIPCHK(c, launch_u16_to_f32(d16, as<float>(c->m_depth), P, c->stream));
+        } else {
+            HIPCHK(c, hipMemcpyAsync(c->m_depth.p, depth, 4 * P, hipMemcpyHostToDevice,
+                                     c->stream));
+        }
+        dep = as<float>(c->m_depth);
+    }
+    TRY(ensure(c, c->normals, 12 * P));
+    HIPCHK(c, launch_normals(dep, w, h, as<float>(c->normals), c->stream));
+    c->nrm_w = w;
+    c->nrm_h = h;
+    View v;
+    v.px = pose->position[0]; v.py = pose->position[1]; v.pz = pose->position[2];
+    v.qx = pose->orientation[0]; v.qy = pose->orientation[1];
+    v.qz = pose->orientation[2]; v.qw = pose->orientation[3];
+    v.fx = intr->f[0]; v.fy = intr->f[1]; v.cx = intr->c[0]; v.cy = intr->c[1];
+    v.W = w; v.H = h;
+    v.ntx = (w + kTile - 1) / kTile; v.nty = (h + kTile - 1) / kTile;
+    const int sw = (w + sample_px - 1) / sample_px, sh = (h + sample_px - 1) / sample_px;
+    const int ncell = sw * sh;
+    TRY(ensure(c, c->cand, 56ull * ncell));
+    TRY(ensure(c, c->ckeep, (size_t)ncell));
+    HIPCHK(c, launch_init_cells(v, col, dep, as<float>(c->normals), sample_px, sample_px,
+                                init_scale, init_opacity, as<float>(c->cand),
+                                as<uint8_t>(c->ckeep), c->stream));
+    // a new map: previous Gaussians and Adam moments are dropped
+    c->n = 0;
+    c->adam_zeroed = false;
+    uint32_t made = 0;
+    TRY(append_candidates(c, ncell, 0, max_gaussians, &made));
+    if (count) *count = made;
+    return FVGS_OK;
 }
 
 int fvgs_gn_step(const double jtj21[21], const double jtr6[6], double pose7[7],

@@ -79,6 +79,11 @@ struct MapRef {
     }
 };
 
+// Writable SoA map arrays (plain form), for kernels that move or append Gaussians.
+struct MapArrays {
+    float *pos, *scale, *quat, *color, *opac;
+};
+
 struct AdamState {   // AdamStateGaussian3D layout (GSCudaKernels.cuh:32-45), 108 bytes
     float m_pos[3], v_pos[3], m_scale[3], v_scale[3], m_rot[3], v_rot[3], m_col[3], v_col[3];
     float m_alpha, v_alpha, k;
@@ -141,6 +146,37 @@ hipError_t launch_accum_delta3d(const Delta3D* src, Delta3D* dst, int n, hipStre
 hipError_t launch_adam(float* pos, float* scale, float* quat, float* color, float* opac,
                        AdamState* state, const Delta3D* d3, int n, float eta, float beta1,
                        float beta2, float eps, hipStream_t s);
+
+// ---- map maintenance (maint.hip) ----
+hipError_t launch_visibility(const View& v, const uint32_t* ranges, const uint32_t* idx,
+                             const Rec* rec, const MapRef& map, uint8_t* vis, hipStream_t s);
+hipError_t launch_covisibility(const uint8_t* a, const uint8_t* b, int n, uint32_t* counts,
+                               hipStream_t s);
+hipError_t launch_outliers(const View& v, const uint32_t* ranges, const uint32_t* idx,
+                           const Rec* rec, const MapRef& map, const float* obs_depth, float* prob,
+                           float* total, hipStream_t s);
+hipError_t launch_density_mask(const View& v, const uint32_t* ranges, const uint32_t* idx,
+                               const Rec* rec, const MapRef& map, const float* obs_depth,
+                               float* mask, hipStream_t s);
+hipError_t launch_prune_keep(const float* scale, const float* opac, int n, float alpha_thr,
+                             float ratio_thr, uint8_t* keep, hipStream_t s);
+hipError_t launch_outlier_keep(const float* prob, const float* total, int n, float thr,
+                               uint8_t* keep, hipStream_t s);
+// boff[b] = survivors before 256-item block b (exclusive), boff[nb] = total; nb+1 entries.
+hipError_t launch_flag_offsets(const uint8_t* flag, int n, uint32_t* boff, hipStream_t s);
+hipError_t launch_compact_map(const uint8_t* keep, int n, const uint32_t* boff, MapArrays src,
+                              MapArrays dst, const AdamState* asrc, AdamState* adst,
+                              hipStream_t s);
+hipError_t launch_normals(const float* depth, int W, int H, float* nrm, hipStream_t s);
+hipError_t launch_init_cells(const View& v, const uint8_t* bgr, const float* depth,
+                             const float* nrm, int sdx, int sdy, float init_scale,
+                             float init_opac, float* cand, uint8_t* keep, hipStream_t s);
+hipError_t launch_densify_cells(const View& v, const float* mask, const uint8_t* bgr,
+                                const float* depth, int sdx, int sdy, float* cand, uint8_t* keep,
+                                hipStream_t s);
+hipError_t launch_append_cands(const uint8_t* keep, int ncell, const uint32_t* boff,
+                               const float* cand, uint32_t n0, uint32_t cap, MapArrays dst,
+                               AdamState* adst, hipStream_t s);
 
 // ---- tracking (track.hip) ----
 // Frame pyramid (GSSlam::initAndCopyImgs): cv::pyrDown of 8-bit 3-channel colour and of float

@@ -162,7 +162,14 @@ enum fvgs_out {
     FVGS_OUT_COLORS = 57,      /* float[N][3] */
     FVGS_OUT_OPACITIES = 58,   /* float[N] */
     FVGS_OUT_FRAME_GX = 59,    /* float[H][W][3] the frame's Sobel x gradient (if attached) */
-    FVGS_OUT_FRAME_GY = 60     /* float[H][W][3] the frame's Sobel y gradient */
+    FVGS_OUT_FRAME_GY = 60,    /* float[H][W][3] the frame's Sobel y gradient */
+    FVGS_OUT_VISIBILITY = 61,  /* uint8[N] marks of the last fvgs_visibility / fvgs_covisibility
+                                  (the current view) */
+    FVGS_OUT_DENSITY_MASK = 62,/* float[H][W] mask of the last fvgs_density_mask / fvgs_densify */
+    FVGS_OUT_OUTLIER_PROB = 63,/* float[N_before] outlier mass of the last fvgs_remove_outliers,
+                                  indexed by the Gaussians before the removal */
+    FVGS_OUT_TOTAL_ALPHA = 64, /* float[N_before] total alpha, likewise */
+    FVGS_OUT_NORMALS = 65      /* float[H][W][3] normals of the last fvgs_init_from_rgbd */
 };
 
 int fvgs_default_params(fvgs_params* params);
@@ -232,6 +239,55 @@ int fvgs_set_frame_raw(fvgs_ctx* ctx, int width, int height, const uint8_t* colo
  * gradients) and returns its size; the caller scales the intrinsics by 2^-level as the
  * reference does (GSSlam.cu:2133-2138). */
 int fvgs_select_level(fvgs_ctx* ctx, int level, int* width, int* height);
+
+/* ---- map maintenance (GSCudaKernels.cu:87-311, 1455-2068; GSSlam.cu:271-406, 1351-1904) ----
+ * Views are projected and binned as fvgs_prepare_rasterization does; the walks use the map's
+ * opacity, as the reference does. Removals keep the survivors in their order (the reference's
+ * sort by state) and carry their Adam states along (the reference leaves adam_states_
+ * unpermuted); additions are appended in sampling-cell order (the reference's atomic order is
+ * unspecified) up to `max_gaussians` in total, with zero Adam states. Removing or adding needs
+ * the plain form (FVGS_E_STATE otherwise); removals compact a FVGS_MEM_DEVICE map in the
+ * caller's arrays and update the count (fvgs_gaussian_count), additions first copy such a map
+ * into the context (read it back with fvgs_read / fvgs_device_ptr of FVGS_OUT_POSITIONS...). */
+
+/* computeGaussiansVisibility_kernel (GSCudaKernels.cu:1455-1537) over a view: FVGS_OUT_VISIBILITY. */
+int fvgs_visibility(fvgs_ctx* ctx, const fvgs_pose* pose, const fvgs_intrinsics* intr, int width,
+                    int height);
+/* GSSlam::computeCovisibilityRatio (GSSlam.cu:1713-1904): visibility of the keyframe view and of
+ * the current view, |both| and |either| (computeGaussiansCovisibility_kernel) and their ratio
+ * (0 when either view has no tile instances or nothing is visible). */
+int fvgs_covisibility(fvgs_ctx* ctx, const fvgs_pose* kf_pose, const fvgs_intrinsics* kf_intr,
+                      int kf_width, int kf_height, const fvgs_pose* pose,
+                      const fvgs_intrinsics* intr, int width, int height, uint32_t* inter,
+                      uint32_t* uni, float* ratio);
+/* GSSlam::prune (GSSlam.cu:1467-1579, pruneGaussians_kernel GSCudaKernels.cu:1566-1618; the
+ * reference uses alpha 0.05, scale ratio 0.05). */
+int fvgs_prune(fvgs_ctx* ctx, float alpha_threshold, float scale_ratio_threshold,
+               uint32_t* removed);
+/* GSSlam::removeOutliers (GSSlam.cu:1581-1711; computeOutliers_kernel / removeOutliers_kernel,
+ * GSCudaKernels.cu:1620-1739; the reference uses threshold 0.6) at `pose` against the current
+ * frame's depth, at the frame's size. */
+int fvgs_remove_outliers(fvgs_ctx* ctx, const fvgs_pose* pose, const fvgs_intrinsics* intr,
+                         float threshold, uint32_t* removed);
+/* computeDensityMask_kernel (GSCudaKernels.cu:1741-1857) at `pose` against the current frame's
+ * depth: FVGS_OUT_DENSITY_MASK. */
+int fvgs_density_mask(fvgs_ctx* ctx, const fvgs_pose* pose, const fvgs_intrinsics* intr);
+/* GSSlam::densify (GSSlam.cu:1351-1465): density mask against the current frame (the keyframe's
+ * depth), then densifyGaussians_kernel (GSCudaKernels.cu:1859-2068) over sample_px cells with the
+ * keyframe's 8-bit (B, G, R) colour (the size of the current frame). The camera's float4 is read
+ * (w, x, y, z) by this kernel, as in the reference. */
+int fvgs_densify(fvgs_ctx* ctx, const fvgs_pose* pose, const fvgs_intrinsics* intr,
+                 const uint8_t* color3, int mem, int sample_px, uint32_t max_gaussians,
+                 uint32_t* added);
+/* GSSlam::initializeGaussiansFromRgbd (GSSlam.cu:271-406): normals from depth
+ * (computeNormalsFromDepth_kernel) and one Gaussian per sample_px cell
+ * (initGaussiansFromRgbd_kernel, GSCudaKernels.cu:87-311; the reference uses 10 px, scale 2.0,
+ * opacity 0.8) replace the map. Depth as float metres or 16-bit millimetres (x 0.001f, the
+ * reference's depth_scale at GSSlam.cu:864). Camera float4 read (w, x, y, z) as there. */
+int fvgs_init_from_rgbd(fvgs_ctx* ctx, int width, int height, const uint8_t* color3,
+                        const void* depth, int depth_format, int mem, const fvgs_pose* pose,
+                        const fvgs_intrinsics* intr, int sample_px, float init_scale,
+                        float init_opacity, uint32_t max_gaussians, uint32_t* count);
 
 /* Adam update of the map from the 3D gradients of the last mapping step (replaces
  * updateGaussiansParametersAdam_kernel, GSCudaKernels.cu:2722-2882, with the reference's

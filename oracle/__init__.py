@@ -292,6 +292,111 @@ def frame_pyramid(color_u8, depth, levels):
     return out
 
 
+# ---- map maintenance (GSCudaKernels.cu:87-311, 1455-2068), FP32 in the reference's order ----
+def _walk_args(pf, binning):
+    return (_p(binning["ranges"]), _p(binning["idx"]), _p(_f32(pf["pos2d"])),
+            _p(_f32(pf["conic"])))
+
+
+def visibility(width, height, pf, binning, opac):
+    """computeGaussiansVisibility_kernel: uint8[N] marks (pf: project_f32 output)."""
+    vis = np.zeros(len(pf["depth"]), np.uint8)
+    lib().orc_visibility(ctypes.c_int(width), ctypes.c_int(height), *_walk_args(pf, binning),
+                         _p(_f32(opac)), _p(vis))
+    return vis
+
+
+def outliers(width, height, pf, binning, opac, obs_depth):
+    """computeOutliers_kernel: (outlier mass, total alpha) per Gaussian, FP64 sums."""
+    n = len(pf["depth"])
+    prob, total = np.zeros(n), np.zeros(n)
+    lib().orc_outliers(ctypes.c_int(width), ctypes.c_int(height), *_walk_args(pf, binning),
+                       _p(_f32(pf["phat"])), _p(_f32(pf["depth"])), _p(_f32(opac)),
+                       _p(_f32(obs_depth)), _p(prob), _p(total))
+    return prob, total
+
+
+def density_mask(width, height, pf, binning, opac, obs_depth):
+    """computeDensityMask_kernel: float (H, W)."""
+    mask = np.zeros((height, width), np.float32)
+    lib().orc_density_mask(ctypes.c_int(width), ctypes.c_int(height), *_walk_args(pf, binning),
+                           _p(_f32(pf["phat"])), _p(_f32(pf["depth"])), _p(_f32(opac)),
+                           _p(_f32(obs_depth)), _p(mask))
+    return mask
+
+
+def prune_keep(scale, opac, alpha_thr=0.05, ratio_thr=0.05):
+    """pruneGaussians_kernel: keep flags (1 = kept)."""
+    n = len(opac)
+    keep = np.zeros(n, np.uint8)
+    lib().orc_prune_keep(ctypes.c_int(n), _p(_f32(scale)), _p(_f32(opac)),
+                         ctypes.c_float(alpha_thr), ctypes.c_float(ratio_thr), _p(keep))
+    return keep
+
+
+def outlier_keep(prob, total, thr=0.6):
+    """removeOutliers_kernel on FP32 accumulators: keep flags."""
+    prob, total = _f32(prob), _f32(total)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rm = (total > np.float32(1.0)) & (prob / total > np.float32(thr))
+    return (~rm).astype(np.uint8)
+
+
+def compact(scene, keep):
+    """Survivors in their order (the reference's sort of the map by state)."""
+    k = np.asarray(keep, bool)
+    return {key: np.ascontiguousarray(v[k]) for key, v in scene.items()}
+
+
+def normals(depth):
+    """computeNormalsFromDepth_kernel: float (H, W, 3)."""
+    d = _f32(depth)
+    h, w = d.shape
+    out = np.zeros((h, w, 3), np.float32)
+    lib().orc_normals(_p(d), ctypes.c_int(w), ctypes.c_int(h), _p(out))
+    return out
+
+
+def _cands_to_scene(cand, keep, cap=None):
+    c = cand[np.asarray(keep, bool)]
+    if cap is not None:
+        c = c[:cap]
+    return dict(pos=c[:, 0:3].copy(), scale=c[:, 3:6].copy(), quat=c[:, 6:10].copy(),
+                color=c[:, 10:13].copy(), opacity=c[:, 13].copy())
+
+
+def init_from_rgbd(bgr, depth, pose7, intr, sample_px=10, init_scale=2.0, init_opacity=0.8,
+                   max_gaussians=None):
+    """initGaussiansFromRgbd_kernel: the new map in cell order. pose7: position + camera float4
+    slots (read (w, x, y, z) by this kernel); depth float metres."""
+    bgr = np.ascontiguousarray(bgr, np.uint8)
+    d = _f32(depth)
+    h, w = d.shape
+    nrm = normals(d)
+    sw, sh = (w + sample_px - 1) // sample_px, (h + sample_px - 1) // sample_px
+    cand = np.zeros((sw * sh, 14), np.float32)
+    keep = np.zeros(sw * sh, np.uint8)
+    lib().orc_init_cells(ctypes.c_int(w), ctypes.c_int(h), _p(bgr), _p(d), _p(nrm),
+                         _p(_f32(pose7)), _p(_f32(intr)), ctypes.c_int(sample_px),
+                         ctypes.c_int(sample_px), ctypes.c_float(init_scale),
+                         ctypes.c_float(init_opacity), _p(cand), _p(keep))
+    return _cands_to_scene(cand, keep, max_gaussians)
+
+
+def densify_cells(mask, bgr, depth, pose7, intr, sample_px=10):
+    """densifyGaussians_kernel: the appended Gaussians in cell order."""
+    bgr = np.ascontiguousarray(bgr, np.uint8)
+    d, m = _f32(depth), _f32(mask)
+    h, w = d.shape
+    sw, sh = (w + sample_px - 1) // sample_px, (h + sample_px - 1) // sample_px
+    cand = np.zeros((sw * sh, 14), np.float32)
+    keep = np.zeros(sw * sh, np.uint8)
+    lib().orc_densify_cells(ctypes.c_int(w), ctypes.c_int(h), _p(m), _p(bgr), _p(d),
+                            _p(_f32(pose7)), _p(_f32(intr)), ctypes.c_int(sample_px),
+                            ctypes.c_int(sample_px), _p(cand), _p(keep))
+    return _cands_to_scene(cand, keep)
+
+
 def pipeline(scene, cam_pos, cam_quat, intr, width, height):
     """Binning (FP32) + value-mode records (FP64) for one view."""
     pf = project_f32(scene["pos"], scene["scale"], scene["quat"], cam_pos, cam_quat, intr,

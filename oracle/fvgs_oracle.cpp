@@ -1135,4 +1135,280 @@ ORC_API void orc_pyrdown_f32(const float* src, int w, int h, float* dst) {
         }
 }
 
+// ---- map maintenance (GSCudaKernels.cu:87-311, 1455-2068) in FP32, the reference's operation
+// order, over the FP32 binning (pos2d, conic as stored (xx, yy, xy), depth, p-hat) and the map's
+// opacity. Culled Gaussians' zero records count as the reference counts them (Q1).
+
+// computeGaussiansVisibility_kernel (1455-1537): first min(len, 256) entries per tile.
+ORC_API void orc_visibility(int W, int H, const uint32_t* ranges, const uint32_t* idx,
+                            const float* pos2d, const float* conic, const float* opac,
+                            uint8_t* vis) {
+    const int ntx = (W + 15) / 16;
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            const int t = (y / 16) * ntx + x / 16;
+            const uint32_t s = ranges[2 * t];
+            const uint32_t n = std::min<uint32_t>(ranges[2 * t + 1] - s, 256u);
+            float T = 1.0f;
+            for (uint32_t i = 0; i < n; i++) {
+                const uint32_t g = idx[s + i];
+                const float dx = pos2d[2 * g] - (float)x, dy = pos2d[2 * g + 1] - (float)y;
+                const float q = conic[3 * g] * dx * dx + 2.0f * conic[3 * g + 1] * dx * dy +
+                                conic[3 * g + 2] * dy * dy;
+                const float a = std::min(0.99f, opac[g] * std::exp(-0.5f * q));
+                if (a < 1.0f / 255.0f) continue;
+                vis[g] = 1;
+                T *= (1.0f - a);
+                if (T < 0.5f) break;
+            }
+        }
+}
+
+// computeOutliers_kernel (1620-1715): every (entry, pixel) pair; FP32 per pair, FP64 sums.
+ORC_API void orc_outliers(int W, int H, const uint32_t* ranges, const uint32_t* idx,
+                          const float* pos2d, const float* conic, const float* phat,
+                          const float* depth, const float* opac, const float* obs_depth,
+                          double* prob, double* total) {
+    const int ntx = (W + 15) / 16;
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            const int t = (y / 16) * ntx + x / 16;
+            const float dobs = obs_depth[(size_t)y * W + x];
+            for (uint32_t k = ranges[2 * t]; k < ranges[2 * t + 1]; k++) {
+                const uint32_t g = idx[k];
+                const float dx = (float)x - pos2d[2 * g], dy = (float)y - pos2d[2 * g + 1];
+                const float dr = depth[g] + phat[2 * g] * dx + phat[2 * g + 1] * dy;
+                const float gv = std::exp(-0.5f * (conic[3 * g] * dx * dx +
+                                                   conic[3 * g + 2] * dy * dy +
+                                                   2.0f * conic[3 * g + 1] * dx * dy));
+                const float a = opac[g] * gv;
+                if (a < 1.0f / 255.0f || dr <= 0.0f) continue;
+                total[g] += a;
+                if (dobs > 0.1f && dr < 0.8f * dobs) prob[g] += a;
+            }
+        }
+}
+
+// computeDensityMask_kernel (1741-1857)
+ORC_API void orc_density_mask(int W, int H, const uint32_t* ranges, const uint32_t* idx,
+                              const float* pos2d, const float* conic, const float* phat,
+                              const float* depth, const float* opac, const float* obs_depth,
+                              float* mask) {
+    const int ntx = (W + 15) / 16;
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            const int t = (y / 16) * ntx + x / 16;
+            const float dobs = obs_depth[(size_t)y * W + x];
+            float prod = 1.0f, T = 1.0f, dr = 0.0f;
+            if (!(dobs < 0.5f))
+                for (uint32_t k = ranges[2 * t]; k < ranges[2 * t + 1]; k++) {
+                    const uint32_t g = idx[k];
+                    const float dx = pos2d[2 * g] - (float)x, dy = pos2d[2 * g + 1] - (float)y;
+                    const float q = conic[3 * g] * dx * dx + 2.0f * conic[3 * g + 1] * dx * dy +
+                                    conic[3 * g + 2] * dy * dy;
+                    const float a = std::min(0.99f, opac[g] * std::exp(-0.5f * q));
+                    if (a < 1.0f / 255.0f) continue;
+                    prod *= (1.0f - a);
+                    if (T > 0.5f && prod < 0.5f) {
+                        T = prod;
+                        dr = depth[g] + phat[2 * g] * dx + phat[2 * g + 1] * dy;
+                    }
+                    if (prod < 0.0001f) break;
+                }
+            float val = 0.0f;
+            if (dobs > 0.5f) {
+                if (prod > 0.5f) val = prod;
+                else if (dobs < dr && dr - dobs > 0.2f * dobs) val = 1.0f;
+            }
+            mask[(size_t)y * W + x] = val;
+        }
+}
+
+// pruneGaussians_kernel (1566-1618): keep = 1 unless removed
+ORC_API void orc_prune_keep(int n, const float* scale, const float* opac, float at, float rt,
+                            uint8_t* keep) {
+    for (int g = 0; g < n; g++) {
+        bool rm;
+        if (opac[g] < at) {
+            rm = true;
+        } else {
+            const float sx = scale[3 * g], sy = scale[3 * g + 1], sz = scale[3 * g + 2];
+            const float smax = std::max(sx, std::max(sy, sz)), smin = std::min(sx, std::min(sy, sz));
+            const float smed = sx + sy + sz - smax - smin;
+            rm = (smed / smax < rt) || (smax < 0.005f);
+        }
+        keep[g] = rm ? 0 : 1;
+    }
+}
+
+namespace {
+struct F3 { float x, y, z; };
+F3 cross3(F3 a, F3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+// rotateByQuaternion (20-26), q slots (w, x, y, z)
+F3 rot_wxyz(const float q[4], F3 v) {
+    const F3 qv{q[1], q[2], q[3]};
+    const F3 c = cross3(qv, v);
+    const F3 t{2.0f * c.x, 2.0f * c.y, 2.0f * c.z};
+    const F3 c2 = cross3(qv, t);
+    return {v.x + q[0] * t.x + c2.x, v.y + q[0] * t.y + c2.y, v.z + q[0] * t.z + c2.z};
+}
+F3 norm_or_zero(F3 v) {   // normalizeVec3 (28-37)
+    const float n = std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    if (n > 1e-8f) { const float inv = 1.0f / n; return {v.x * inv, v.y * inv, v.z * inv}; }
+    return {0.f, 0.f, 0.f};
+}
+void qmul_wxyz(const float a[4], const float b[4], float o[4]) {   // quatMultiply (39-50)
+    o[0] = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+    o[1] = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+    o[2] = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+    o[3] = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+}
+void q_from_two(F3 from, F3 to, float q[4]) {   // quatFromTwoVectors (52-83)
+    const F3 u = norm_or_zero(from), v = norm_or_zero(to);
+    const float dot = u.x * v.x + u.y * v.y + u.z * v.z;
+    if (dot < -0.999999f) {
+        const F3 axis = std::fabs(u.x) < 0.1f ? F3{1.f, 0.f, 0.f} : F3{0.f, 1.f, 0.f};
+        const F3 o = norm_or_zero(cross3(u, axis));
+        q[0] = 0.f; q[1] = o.x; q[2] = o.y; q[3] = o.z;
+        return;
+    }
+    const F3 c = cross3(u, v);
+    q[0] = 1.0f + dot; q[1] = c.x; q[2] = c.y; q[3] = c.z;
+    const float n = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    if (n > 1e-8f) { const float inv = 1.0f / n; for (int k = 0; k < 4; k++) q[k] *= inv; }
+    else { q[0] = 1.f; q[1] = q[2] = q[3] = 0.f; }
+}
+void put_cand(float* c, F3 p, F3 s, const float q[4], F3 col, float o) {
+    c[0] = p.x; c[1] = p.y; c[2] = p.z; c[3] = s.x; c[4] = s.y; c[5] = s.z;
+    for (int k = 0; k < 4; k++) c[6 + k] = q[k];
+    c[10] = col.x; c[11] = col.y; c[12] = col.z; c[13] = o;
+}
+}  // namespace
+
+// computeNormalsFromDepth_kernel (87-162)
+ORC_API void orc_normals(const float* depth, int W, int H, float* nrm) {
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            F3 n{0.f, 0.f, 1.f};
+            if (x > 0 && x < W - 1 && y > 0 && y < H - 1) {
+                const float zc = depth[(size_t)y * W + x], zl = depth[(size_t)y * W + x - 1],
+                            zr = depth[(size_t)y * W + x + 1], zu = depth[(size_t)(y - 1) * W + x],
+                            zd = depth[(size_t)(y + 1) * W + x];
+                if (zc > 0.01f && zl > 0.01f && zr > 0.01f && zu > 0.01f && zd > 0.01f) {
+                    const float dzdx = (zr - zl) / 2.0f, dzdy = (zd - zu) / 2.0f;
+                    n = cross3(F3{1.f, 0.f, dzdx}, F3{0.f, 1.f, dzdy});
+                    const float norm = std::sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+                    if (norm > 1e-6f) { n.x /= norm; n.y /= norm; n.z /= norm; }
+                    if (n.z < 0.0f) { n.x = -n.x; n.y = -n.y; n.z = -n.z; }
+                }
+            }
+            nrm[3 * ((size_t)y * W + x)] = n.x;
+            nrm[3 * ((size_t)y * W + x) + 1] = n.y;
+            nrm[3 * ((size_t)y * W + x) + 2] = n.z;
+        }
+}
+
+// initGaussiansFromRgbd_kernel (164-311): candidate per cell in cell order. pose7 = position,
+// then the camera float4 slots (read (w, x, y, z) here); intr = fx, fy, cx, cy.
+ORC_API void orc_init_cells(int W, int H, const uint8_t* bgr, const float* depth, const float* nrm,
+                            const float* pose7, const float* intr, int sdx, int sdy,
+                            float init_scale, float init_opac, float* cand, uint8_t* keep) {
+    const int sw = (W + sdx - 1) / sdx, sh = (H + sdy - 1) / sdy;
+    const float* cam = pose7 + 3;
+    for (int cell = 0; cell < sw * sh; cell++) {
+        keep[cell] = 0;
+        const int u = (cell % sw) * sdx, v = (cell / sw) * sdy;
+        if (u >= W || v >= H) continue;
+        const size_t p = (size_t)v * W + u;
+        const float z = depth[p];
+        if (!(z > 0.0f) || std::isinf(z) || std::isnan(z)) continue;
+        F3 n{nrm[3 * p], nrm[3 * p + 1], nrm[3 * p + 2]};
+        const float nl = std::sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+        if (nl < 1e-6f) n = F3{0.f, 0.f, 1.f};
+        else { n.x /= nl; n.y /= nl; n.z /= nl; }
+        const float cx = ((float)u - intr[2]) * z / intr[0], cy = ((float)v - intr[3]) * z / intr[1];
+        const F3 r = rot_wxyz(cam, F3{cx, cy, z});
+        const F3 pw{pose7[0] + r.x, pose7[1] + r.y, pose7[2] + r.z};
+        F3 nw = rot_wxyz(cam, n);
+        const float nwl = std::sqrt(nw.x * nw.x + nw.y * nw.y + nw.z * nw.z);
+        if (nwl > 1e-6f) { nw.x /= nwl; nw.y /= nwl; nw.z /= nwl; }
+        const F3 f{0.f, 0.f, 1.f};
+        F3 ax{f.y * nw.z - f.z * nw.y, f.z * nw.x - f.x * nw.z, f.x * nw.y - f.y * nw.x};
+        const float al = std::sqrt(ax.x * ax.x + ax.y * ax.y + ax.z * ax.z);
+        const float ac = f.x * nw.x + f.y * nw.y + f.z * nw.z;
+        float q[4];
+        if (al < 1e-6f) {
+            if (ac > 0.0f) { q[0] = 1.f; q[1] = q[2] = q[3] = 0.f; }
+            else { q[0] = 0.f; q[1] = 1.f; q[2] = q[3] = 0.f; }
+        } else {
+            ax.x /= al; ax.y /= al; ax.z /= al;
+            const float hs = std::sqrt(std::max(0.0f, (1.0f - ac) / 2.0f));
+            const float hc = std::sqrt(std::max(0.0f, (1.0f + ac) / 2.0f));
+            q[0] = hc; q[1] = hs * ax.x; q[2] = hs * ax.y; q[3] = hs * ax.z;
+        }
+        const uint8_t* px = bgr + 3 * p;
+        put_cand(cand + 14 * (size_t)cell, pw, F3{init_scale, init_scale, init_scale}, q,
+                 F3{px[2] / 255.0f, px[1] / 255.0f, px[0] / 255.0f}, init_opac);
+        keep[cell] = 1;
+    }
+}
+
+// densifyGaussians_kernel (1859-2068) without normals, candidate per cell in cell order.
+ORC_API void orc_densify_cells(int W, int H, const float* mask, const uint8_t* bgr,
+                               const float* depth, const float* pose7, const float* intr, int sdx,
+                               int sdy, float* cand, uint8_t* keep) {
+    const int sw = (W + sdx - 1) / sdx, sh = (H + sdy - 1) / sdy;
+    const float* cam = pose7 + 3;
+    for (int cell = 0; cell < sw * sh; cell++) {
+        keep[cell] = 0;
+        F3 pos_img, col;
+        float sc, dz;
+        if (sdx > 1) {
+            const int u0 = (cell % sw) * sdx, v0 = (cell / sw) * sdy;
+            if (u0 >= W || v0 >= H) continue;
+            F3 ip{0.f, 0.f, 0.f}, ra{0.f, 0.f, 0.f};
+            int cnt = 0;
+            for (int u = u0; u < u0 + sdx && u < W; u++)
+                for (int y = v0; y < v0 + sdy && y < H; y++) {
+                    const size_t p = (size_t)y * W + u;
+                    if (mask[p] > 0.0f) {
+                        const float d = depth[p];
+                        if (!(d > 0.0f)) continue;
+                        const uint8_t* px = bgr + 3 * p;
+                        ip.x += (float)u; ip.y += (float)y; ip.z += d;
+                        ra.x += px[2] / 255.0f; ra.y += px[1] / 255.0f; ra.z += px[0] / 255.0f;
+                        cnt++;
+                    }
+                }
+            if (cnt < ((sdx * sdy) >> 1)) continue;
+            const float inv = 1.0f / (float)cnt;
+            ip.x *= inv; ip.y *= inv; ip.z *= inv;
+            ra.x *= inv; ra.y *= inv; ra.z *= inv;
+            pos_img = ip; col = ra; dz = ip.z;
+            const float sx = ip.z * (float)sdx / intr[0], sy = ip.z * (float)sdy / intr[1];
+            sc = 0.5f * (sx + sy);
+        } else {
+            const int x = cell % sw, y = cell / sw;
+            const size_t p = (size_t)y * W + x;
+            if (mask[p] <= 0.0f) continue;
+            const float d = depth[p];
+            if (!(d > 0.0f)) continue;
+            const uint8_t* px = bgr + 3 * p;
+            col = F3{px[2] / 255.0f, px[1] / 255.0f, px[0] / 255.0f};
+            pos_img = F3{(float)x, (float)y, d};
+            dz = d;
+            const float sx = 0.8f * d * (float)sdx / intr[0], sy = 0.8f * d * (float)sdy / intr[1];
+            sc = 0.5f * (sx + sy);
+        }
+        const F3 pc{dz * (pos_img.x - intr[2]) / intr[0], dz * (pos_img.y - intr[3]) / intr[1], dz};
+        const F3 r = rot_wxyz(cam, pc);
+        const F3 pw{pose7[0] + r.x, pose7[1] + r.y, pose7[2] + r.z};
+        float qn[4], q[4];
+        q_from_two(F3{0.f, 0.f, 1.f}, F3{0.f, 0.f, 1.f}, qn);
+        qmul_wxyz(cam, qn, q);
+        put_cand(cand + 14 * (size_t)cell, pw, F3{sc, sc, 0.1f * sc}, q, col, 1.0f);
+        keep[cell] = 1;
+    }
+}
+
 ORC_API int orc_num_threads(void) { return omp_get_max_threads(); }
