@@ -12,6 +12,8 @@
 //   optimizeGaussians(iterations, eta, sampler)          GSSlam.cu:1020-1081
 //   getGaussianData(positions, colors)                   GSSlam.cuh:59
 //   initAndCopyImgs(w, h, color, depth, format)          GSSlam.cu:1913-1969 (on the device)
+//   initializeGaussiansFromRgbd, prune, removeOutliers,  GSSlam.cu:271-406, 1351-1904
+//   computeCovisibilityRatio, densify                    (map maintenance on the device)
 //   computeRgbdPoseJacobians(JtJ, Jtr, level, P, Q)      GSSlam.cu:2103-2244
 //
 // It depends on nothing but the C ABI and the C++ standard library; with Eigen on the include
@@ -218,6 +220,76 @@ public:
         }
     }
 
+    // ---- map maintenance (GSSlam.cu:271-406, 1351-1904) on the device ----
+    void setGaussInitSizePx(int px) { gauss_init_size_px_ = std::max(1, px); }
+    void setGaussInitScale(float s) { gauss_init_scale_ = std::max(1e-5f, s); }
+    void setMaxGaussians(uint32_t n) { max_gaussians_ = n; }
+
+    // GSSlam::initializeGaussiansFromRgbd: the map from one RGB-D frame (8-bit colour; depth
+    // FVGS_DEPTH_F32 metres or FVGS_DEPTH_U16_MM millimetres).
+    void initializeGaussiansFromRgbd(int width, int height, const uint8_t* color,
+                                     const void* depth, int depth_format,
+                                     const CameraPose& pose, int mem = FVGS_MEM_HOST) {
+        std::lock_guard<std::mutex> lk(mutex_);
+        if (!ctx_ || !intrinsics_set_ || !color || !depth) return;
+        fvgs_pose p = to_pose(pose);
+        fvgs_intrinsics i = to_intr(intrinsics_);
+        uint32_t made = 0;
+        check(fvgs_init_from_rgbd(ctx_, width, height, color, depth, depth_format, mem, &p, &i,
+                                  gauss_init_size_px_, gauss_init_scale_, gauss_init_opacity_,
+                                  max_gaussians_, &made),
+              "initializeGaussiansFromRgbd");
+    }
+    // GSSlam::prune (alpha 0.05, scale ratio 0.05)
+    uint32_t prune() {
+        std::lock_guard<std::mutex> lk(mutex_);
+        uint32_t removed = 0;
+        if (ctx_) check(fvgs_prune(ctx_, 0.05f, 0.05f, &removed), "prune");
+        return removed;
+    }
+    // GSSlam::removeOutliers at the current pose against level 0 of the current frame
+    // (threshold 0.6)
+    uint32_t removeOutliers(const CameraPose& current_pose) {
+        std::lock_guard<std::mutex> lk(mutex_);
+        uint32_t removed = 0;
+        if (!ctx_ || !intrinsics_set_ || !attach_level0("removeOutliers")) return 0;
+        fvgs_pose p = to_pose(current_pose);
+        fvgs_intrinsics i = to_intr(intrinsics_);
+        check(fvgs_remove_outliers(ctx_, &p, &i, 0.6f, &removed), "removeOutliers");
+        return removed;
+    }
+    // GSSlam::computeCovisibilityRatio between a keyframe's view and the current view
+    float computeCovisibilityRatio(const KeyframeData& kf, const CameraPose& current_pose,
+                                   int width, int height) {
+        std::lock_guard<std::mutex> lk(mutex_);
+        if (!ctx_ || getGaussiansCount() == 0) return 1.0f;
+        fvgs_pose kp = to_pose(kf.pose), p = to_pose(current_pose);
+        fvgs_intrinsics ki = to_intr(kf.intrinsics), i = to_intr(intrinsics_);
+        uint32_t inter = 0, uni = 0;
+        float ratio = 0.f;
+        if (!check(fvgs_covisibility(ctx_, &kp, &ki, kf.frame.width, kf.frame.height, &p, &i,
+                                     width, height, &inter, &uni, &ratio),
+                   "computeCovisibilityRatio"))
+            return 0.0f;
+        return ratio;
+    }
+    // GSSlam::densify with the keyframe's depth and its 8-bit (B, G, R) colour
+    uint32_t densify(const KeyframeData& kf, const uint8_t* color_bgr, int mem = FVGS_MEM_HOST) {
+        std::lock_guard<std::mutex> lk(mutex_);
+        uint32_t added = 0;
+        if (!ctx_ || !color_bgr || !kf.frame.rgb || !kf.frame.depth) return 0;
+        if (!check(fvgs_set_frame(ctx_, kf.frame.width, kf.frame.height, kf.frame.rgb,
+                                  kf.frame.depth, nullptr, nullptr, kf.frame.mem),
+                   "densify"))
+            return 0;
+        fvgs_pose p = to_pose(kf.pose);
+        fvgs_intrinsics i = to_intr(kf.intrinsics);
+        check(fvgs_densify(ctx_, &p, &i, color_bgr, mem, gauss_init_size_px_, max_gaussians_,
+                           &added),
+              "densify");
+        return added;
+    }
+
     // GSSlam::getGaussianData (GSSlam.cuh:59): positions and colours of the map (after any
     // Adam updates); returns the count.
     uint32_t getGaussianData(std::vector<std::array<float, 3>>& positions,
@@ -345,6 +417,15 @@ public:
     uint32_t lastInstances() const { return last_nb_instances_; }
 
 private:
+    // Level 0 of the current frame becomes the context's frame.
+    bool attach_level0(const char* what) {
+        if (raw_levels_ > 0) return check(fvgs_select_level(ctx_, 0, nullptr, nullptr), what);
+        if (pyramid_.empty()) return false;
+        const FrameLevel& fl = pyramid_[0];
+        return check(fvgs_set_frame(ctx_, fl.width, fl.height, fl.rgb, fl.depth, fl.gx, fl.gy,
+                                    fl.mem),
+                     what);
+    }
     bool check(int r, const char* what) {
         if (r == FVGS_OK) return true;
         std::cerr << "[ERROR " << what << "] " << fvgs_last_error(ctx_) << " (" << r << ")"
@@ -385,6 +466,10 @@ private:
     std::vector<FrameLevel> pyramid_;
     std::vector<KeyframeData> keyframes_;
     int nb_pyr_levels_ = 3;    // GSSlam.cuh:275
+    int gauss_init_size_px_ = 10;        // GSSlam.cuh:199-201
+    float gauss_init_scale_ = 2.0f;
+    float gauss_init_opacity_ = 0.8f;
+    uint32_t max_gaussians_ = 1u << 24;  // the map grows on demand (no fixed reserve)
     int raw_levels_ = 0;       // levels built by initAndCopyImgs (0: pyramid_ is used)
     uint32_t last_nb_instances_ = 0;
 };

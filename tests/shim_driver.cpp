@@ -6,6 +6,8 @@
 //   shim_driver DIR pose      scene.bin frame.bin pose.bin -> out.bin: JtJ[36], Jtr[6] (double)
 //   shim_driver DIR pose_raw  the same at pyramid level L of initAndCopyImgs(raw.bin):
 //                             raw.bin = int32 w, h, L; uint8 colour[3wh]; uint16 depth_mm[wh]
+//   shim_driver DIR init_prune initializeGaussiansFromRgbd(raw.bin, pose.bin's camera) then prune:
+//                             out.bin = uint32 created, removed, left; positions[left][3]
 //   shim_driver DIR optimize  scene.bin frame.bin pose.bin -> out.bin: for each of two
 //                             optimizeGaussiansKeyframe calls, positions[N][3] and colors[N][3]
 //                             (float) after it, then the call's delta3d[N]
@@ -46,7 +48,7 @@ const T* take(const std::vector<char>& buf, size_t& off, size_t count) {
 
 int main(int argc, char** argv) {
     if (argc != 3) {
-        std::fprintf(stderr, "usage: shim_driver DIR pose|pose_raw|optimize\n");
+        std::fprintf(stderr, "usage: shim_driver DIR pose|pose_raw|init_prune|optimize\n");
         return 2;
     }
     const std::string dir = argv[1], mode = argv[2];
@@ -113,6 +115,25 @@ int main(int argc, char** argv) {
         slam.computeRgbdPoseJacobians(JtJ, Jtr, hdr[2], P_imu, Q_imu);
         std::fwrite(JtJ, sizeof(double), 36, out);
         std::fwrite(Jtr, sizeof(double), 6, out);
+    } else if (mode == "init_prune") {
+        std::vector<char> rb;
+        if (!read_all(dir + "/raw.bin", rb)) return 2;
+        size_t k = 0;
+        const int32_t* hdr = take<int32_t>(rb, k, 3);
+        const uint8_t* col = take<uint8_t>(rb, k, 3ull * hdr[0] * hdr[1]);
+        const uint16_t* dep = take<uint16_t>(rb, k, (size_t)hdr[0] * hdr[1]);
+        if (!dep) return 2;
+        fvgs_compat::CameraPose cp;
+        std::memcpy(cp.position, cam, 12);
+        std::memcpy(cp.orientation, cam + 3, 16);
+        slam.initializeGaussiansFromRgbd(hdr[0], hdr[1], col, dep, FVGS_DEPTH_U16_MM, cp);
+        const uint32_t made = slam.getGaussiansCount();
+        const uint32_t removed = slam.prune();
+        std::vector<std::array<float, 3>> P, C;
+        const uint32_t left = slam.getGaussianData(P, C);
+        const uint32_t hdr_out[3] = {made, removed, left};
+        std::fwrite(hdr_out, 4, 3, out);
+        std::fwrite(P.data(), 12, P.size(), out);
     } else if (mode == "optimize") {
         fvgs_compat::KeyframeData kf;
         kf.frame = lv;

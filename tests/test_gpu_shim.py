@@ -160,3 +160,32 @@ def test_shim_optimize_keyframe_applies_adam(tmp_path):
             assert err.max() <= 2e-6
         moved = np.abs(P - np.asarray(scene["pos"], np.float64)).max()
         assert moved > 0.5 * eta
+
+
+def test_shim_initialisation_and_prune(tmp_path):
+    """initializeGaussiansFromRgbd (8-bit colour, millimetre depth) then prune through the shim
+    equal the same calls through the C ABI's Python handle (one map, deterministic order)."""
+    rng = np.random.default_rng(6)
+    col = rng.integers(0, 256, (H, W, 3), dtype=np.uint8)
+    yy, xx = np.mgrid[0:H, 0:W]
+    dmm = (1500 + 2 * xx + np.where(xx > 200, 600, 0)).astype(np.uint16)
+    with open(os.path.join(str(tmp_path), "raw.bin"), "wb") as f:
+        f.write(np.array([W, H, 0], np.int32).tobytes())
+        f.write(col.tobytes())
+        f.write(dmm.tobytes())
+    cam7 = np.array([0.05, 0.0, 0.1, 0.99, 0.0, 0.141, 0.0], np.float32)   # slots (w, x, y, z)
+    intr = S.intrinsics(W, H)
+    scene = S.make_scene(10, seed=1, width=W, height=H)
+    _write_inputs(str(tmp_path), scene, np.zeros((H, W, 3), np.float32),
+                  np.ones((H, W), np.float32), intr, np.zeros(3), [0, 0, 0, 1.0], np.zeros(3),
+                  [0, 0, 0, 1.0], 0.0, cam7)
+    raw = _run(str(tmp_path), "init_prune")
+    made, removed, left = np.frombuffer(raw[:12], np.uint32)
+    P = np.frombuffer(raw[12:], np.float32).reshape(-1, 3)
+    with Context() as c:
+        n = c.init_from_rgbd(col, dmm, cam7.astype(np.float64), intr)   # reference defaults
+        r = c.prune()
+        want = c.read_map()["pos"]
+    print(f"[shim] init {made} (handle {n}), pruned {removed} (handle {r})", flush=True)
+    assert (made, removed, left) == (n, r, n - r) and n > 100
+    np.testing.assert_array_equal(P, want)
