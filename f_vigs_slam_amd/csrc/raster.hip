@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_forward(
     float4* __restrict__ ckpt, const uint32_t* __restrict__ zinfo) {
     __shared__ Rec recs[256];
     __shared__ uint8_t qmask[256];
-    __shared__ uint8_t wlist[4][256];
+    __shared__ __align__(4) uint8_t wlist[4][256];
     __shared__ uint32_t smax;
     const int tile = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int tx0 = (tile % v.ntx) * kTile, ty0 = (tile / v.ntx) * kTile;
@@ -115,11 +115,9 @@ __global__ __launch_bounds__(256) void k_forward(
     uint32_t last = 0;
     bool done = !inside;
     // One list entry (GSCudaKernels.cu:2176-2228); returns true once T < 1e-4.
-    auto step = [&](int i, uint32_t j) -> bool {
-        const float4 r0 = recs[i].r0, r1 = recs[i].r1;
+    auto step = [&](const float4& r0, const float4& r1, const float4& r2, uint32_t j) -> bool {
         const float dx = px - r0.x, dy = py - r0.y;
         const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;   // Q14
-        const float4 r2 = recs[i].r2;
         const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));
         if (a < kAlphaMin || q <= 0.0f) return false;
         const float ta = T * a;
@@ -160,11 +158,31 @@ __global__ __launch_bounds__(256) void k_forward(
         if (done) continue;
         const int nb = base == 0 ? (cnt + kBucket - 1) / kBucket : 0;
         int nextb = 0;
-        for (int k = 0; k < wc; ++k) {
-            const int i = wlist[w][k];
-            for (; nextb < nb && nextb * kBucket <= i; ++nextb)
-                ck[nextb * 256] = make_float4(T, cr, cg, cb);
-            if (step(i, base + (uint32_t)i)) { done = true; break; }
+        // Four entries per round: one 32-bit read of their indices and their 12 record loads
+        // issued together (unconditionally: indices past the list are stale bytes < 256, in
+        // bounds), so the LDS latencies overlap each other and the blending.
+        const uint32_t* wl4 = reinterpret_cast<const uint32_t*>(wlist[w]);
+        uint32_t w4n = wl4[0];
+        for (int k = 0; k < wc && !done; k += 4) {
+            const uint32_t w4 = w4n;
+            w4n = wl4[((k >> 2) + 1) & 63];   // next round's indices (in bounds; unused last)
+            int ii[4];
+            float4 a0[4], a1[4], a2[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                ii[u] = (int)((w4 >> (8 * u)) & 255u);
+                a0[u] = recs[ii[u]].r0;
+                a1[u] = recs[ii[u]].r1;
+                a2[u] = recs[ii[u]].r2;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (k + u >= wc || done) break;
+                const int i = ii[u];
+                for (; nextb < nb && nextb * kBucket <= i; ++nextb)
+                    ck[nextb * 256] = make_float4(T, cr, cg, cb);
+                if (step(a0[u], a1[u], a2[u], base + (uint32_t)i)) done = true;
+            }
         }
         if (!done)
             for (; nextb < nb; ++nextb) ck[nextb * 256] = make_float4(T, cr, cg, cb);
@@ -194,20 +212,28 @@ __global__ __launch_bounds__(256) void k_forward(
 // One 512-thread workgroup per tile; wave w replays bucket w (entries 32w .. 32w+31): lane l owns
 // entry 32w + (l & 31), the lower half-wave walks pixels 0-127 and the upper half pixels
 // 128-255. At step i lane l handles pixel i - (l & 31) of its half with the transmittance and
-// accumulated colour received from lane l-1 (DPP wave shift); the first lane of each half starts
+// colour state received from lane l-1 (DPP wave shift); the first lane of each half starts
 // each pixel from the forward checkpoint of the bucket — the reference's per-bucket warp
 // pipeline, split over two half-waves so that a tile has 8 waves in flight. The halves' partial
 // sums are added across the wave before one atomicAdd per field and (entry, tile). Per-pixel
 // forward results sit in LDS.
 struct BwdLds {
     Rec recs[256];
-    float4 pix_a[256];      // rendered colour (r, g, b), median depth
-    float4 pix_b[256];      // colour error (r, g, b), observed depth
+    float4 pix_e[256];      // colour error E = (r, g, b) - observed, and E . C (C rendered colour)
+    float2 pix_d[256];      // median depth, observed depth
     uint32_t pix_last[256]; // n_contrib; 0 for pixels outside the image
 };
 
 __device__ __forceinline__ float add_halves(float v) { return v + __shfl_xor(v, 32); }
 
+// The colour state that the chain carries per pixel is the scalar S = E . acc (accumulated
+// colour dotted with the pixel's colour error): the reference's per-channel terms enter the
+// gradient only through that dot product, so
+//   sum_c -E_c (col_c T - (C_c - acc'_c) / (1 - a)) = ((E . C) - S') / (1 - a) - T (E . col),
+// with S' = S + a T (E . col) (GSCudaKernels.cu:2390-2410 rearranged). Mean, conic and depth
+// gradients are accumulated as sums over the entry's pixels (a dL/da dx, a dL/da dy, their
+// second moments, the depth-chain weight) and mapped through the entry's constant conic and
+// p-hat once at the end.
 __global__ __launch_bounds__(512, 8) void k_backward(
     View v, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ idx,
     const Rec* __restrict__ rec, const float* __restrict__ obs_depth, BlendParams bp,
@@ -229,12 +255,14 @@ __global__ __launch_bounds__(512, 8) void k_backward(
         const int x = tx0 + (t & 15), y = ty0 + (t >> 4);
         if (x < v.W && y < v.H) {
             const size_t p = (size_t)y * v.W + x;
-            L.pix_a[t] = make_float4(o_rgb[3 * p], o_rgb[3 * p + 1], o_rgb[3 * p + 2], o_depth[p]);
-            L.pix_b[t] = make_float4(o_err[3 * p], o_err[3 * p + 1], o_err[3 * p + 2], obs_depth[p]);
+            const float Er = o_err[3 * p], Eg = o_err[3 * p + 1], Eb = o_err[3 * p + 2];
+            const float EC = Er * o_rgb[3 * p] + Eg * o_rgb[3 * p + 1] + Eb * o_rgb[3 * p + 2];
+            L.pix_e[t] = make_float4(Er, Eg, Eb, EC);
+            L.pix_d[t] = make_float2(o_depth[p], obs_depth[p]);
             L.pix_last[t] = o_nc[p];
         } else {
-            L.pix_a[t] = make_float4(0.f, 0.f, 0.f, 0.f);
-            L.pix_b[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+            L.pix_e[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+            L.pix_d[t] = make_float2(0.f, 0.f);
             L.pix_last[t] = 0;
         }
     }
@@ -245,40 +273,40 @@ __global__ __launch_bounds__(512, 8) void k_backward(
     const bool valid = sidx < nsplat;
     float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
     if (valid) { r0 = L.recs[sidx].r0; r1 = L.recs[sidx].r1; r2 = L.recs[sidx].r2; }
-    // Checkpoints of this half's pixels: lane sl holds pixel 128 half + 32c + sl of chunk c and
-    // the values move one lane down per step, so the half's first lane holds pixel i of the half
-    // at step i. The next chunk is loaded 32 steps ahead. Loads are unconditional and in bounds;
-    // values of pixels outside the image are never used.
+    // Checkpoints of this half's pixels: lane sl holds pixel 128 half + 32c + sl of chunk c, as
+    // (T, E . colour), and the values move one lane down per step, so the half's first lane
+    // holds pixel i of the half at step i. The next chunk is loaded 32 steps ahead. Loads are
+    // unconditional and in bounds; values of pixels outside the image are never used.
     const float4* ck = ckpt + ((size_t)tile * kMaxBuckets + b) * 256 + 128 * half;
     float4 ck_next = ck[sl];
-    float4 ckv = ck_next;
-    float dmx = 0.f, dmy = 0.f, dsx = 0.f, dsy = 0.f, dsz = 0.f, dcr = 0.f, dcg = 0.f, dcb = 0.f,
-          ddep = 0.f, dal = 0.f, dphx = 0.f, dphy = 0.f;
+    float ckT = 0.f, ckS = 0.f;
+    float dcr = 0.f, dcg = 0.f, dcb = 0.f, dal = 0.f, sgx = 0.f, sgy = 0.f, sxx = 0.f, sxy = 0.f,
+          syy = 0.f, dz = 0.f, dphx = 0.f, dphy = 0.f;
     uint32_t dn = 0;
-    float bT = 0.f, ar = 0.f, ag = 0.f, ab = 0.f;
+    float bT = 0.f, S = 0.f;
     const bool first = sl == 0;
     const float mx0 = r0.x - (float)tx0, my0 = r0.y - (float)ty0;   // Q9: mean - pixel
+    const float wdist = bp.w_dist, wdep = bp.w_depth;
     for (int c = 0; c < 5; ++c) {   // 5 chunks of 32 steps cover steps 0 .. 128 + 30
-        ckv = ck_next;
+        {
+            const float4 ckv = ck_next;
+            const float4 pe = L.pix_e[128 * half + min(32 * c, 96) + sl];
+            ckT = ckv.x;
+            ckS = pe.x * ckv.y + pe.y * ckv.z + pe.z * ckv.w;
+        }
         ck_next = ck[min(32 * c + 32, 96) + sl];
         const int tend = c < 4 ? 32 : 31;
         for (int t = 0; t < tend; ++t) {
             const int i = 32 * c + t;
             if (t > 0) {   // uniform
-                ckv.x = lane_shift_down(ckv.x);
-                ckv.y = lane_shift_down(ckv.y);
-                ckv.z = lane_shift_down(ckv.z);
-                ckv.w = lane_shift_down(ckv.w);
+                ckT = lane_shift_down(ckT);
+                ckS = lane_shift_down(ckS);
             }
             // State of the pixel from the left lane; each half's first lane takes the checkpoint.
             bT = lane_shift_up(bT);
-            ar = lane_shift_up(ar);
-            ag = lane_shift_up(ag);
-            ab = lane_shift_up(ab);
-            bT = first ? ckv.x : bT;
-            ar = first ? ckv.y : ar;
-            ag = first ? ckv.z : ag;
-            ab = first ? ckv.w : ab;
+            S = lane_shift_up(S);
+            bT = first ? ckT : bT;
+            S = first ? ckS : S;
             const int ph = i - sl;                    // pixel within the half
             const int pc = (128 * half + ph) & 255;   // pixel within the tile
             const uint32_t plast = L.pix_last[pc];
@@ -287,60 +315,52 @@ __global__ __launch_bounds__(512, 8) void k_backward(
             const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;
             const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));   // Q8: no q <= 0 test
             if (!(valid && (unsigned)ph < 128u && sidx < plast) || a < kAlphaMin) continue;
-            const float4 pa = L.pix_a[pc], pb = L.pix_b[pc];
-            const float Er = pb.x, Eg = pb.y, Eb = pb.z;
-            float dar = r2.x * bT, dag = r2.y * bT, dab = r2.z * bT;
-            ar += dar * a; ag += dag * a; ab += dab * a;
-            const float iom = __builtin_amdgcn_rcpf(1.0f - a);
-            dar -= (pa.x - ar) * iom;
-            dag -= (pa.y - ag) * iom;
-            dab -= (pa.z - ab) * iom;
-            dar *= -Er; dag *= -Eg; dab *= -Eb;
-            const float aG = a * (dar + dag + dab);
-            ++dn;
+            const float4 pe = L.pix_e[pc];
+            const float2 pd = L.pix_d[pc];
+            const float EC = pe.x * r2.x + pe.y * r2.y + pe.z * r2.z;   // E . colour
             const float at = a * bT;
-            dcr -= at * Er; dcg -= at * Eg; dcb -= at * Eb;
+            const float Sn = S + at * EC;
+            const float om = 1.0f - a;
+            const float iom = __builtin_amdgcn_rcpf(om);
+            const float aG = a * ((pe.w - Sn) * iom - bT * EC);
+            S = Sn;
+            ++dn;
+            dcr -= at * pe.x; dcg -= at * pe.y; dcb -= at * pe.z;
             dal -= aG;
-            dmx -= aG * (r0.z * dx + r0.w * dy);
-            dmy -= aG * (r0.w * dx + r1.x * dy);
-            dsx -= 0.5f * aG * dx * dx;
-            dsy -= 0.5f * aG * dx * dy;
-            dsz -= 0.5f * aG * dy * dy;
-            const float tT = bT * (1.0f - a);
-            const float Dp = pa.w;
+            const float gx = aG * dx, gy = aG * dy;
+            sgx += gx; sgy += gy;
+            sxx += gx * dx; sxy += gx * dy; syy += gy * dy;
+            const float tT = bT * om;
+            const float Dp = pd.x;
+            float e = 0.0f;   // weight of this pair in the depth chain
             if (bT > 0.5f && tT <= 0.5f) {
-                const float obs = pb.w;
-                const float de = (obs > 0.1f) ? (Dp - obs) : 0.0f;
-                const float wde = bp.w_depth * de;
-                ddep -= wde;
-                dmx -= wde * r1.z; dmy -= wde * r1.w;
-                dphx -= wde * dx; dphy -= wde * dy;
+                const float obs = pd.y;
+                e = wdep * ((obs > 0.1f) ? (Dp - obs) : 0.0f);
             }
             const float di = r1.y + dx * r1.z + dy * r1.w;
-            const float dc = bp.w_dist * a * bT * (di - Dp);
-            ddep -= dc;
-            dmx -= dc * r1.z; dmy -= dc * r1.w;
-            dphx -= dc * dx; dphy -= dc * dy;
+            e += wdist * at * (di - Dp);
+            dz += e;
+            dphx -= e * dx; dphy -= e * dy;
             bT = tT;
         }
     }
-    // Lower half + upper half of each entry (fixed order).
-    ddep = add_halves(ddep); dphx = add_halves(dphx); dphy = add_halves(dphy);
-    dmx = add_halves(dmx); dmy = add_halves(dmy);
-    dsx = add_halves(dsx); dsy = add_halves(dsy); dsz = add_halves(dsz);
+    // Lower half + upper half of each entry (fixed order), then the entry's constant maps.
     dcr = add_halves(dcr); dcg = add_halves(dcg); dcb = add_halves(dcb);
     dal = add_halves(dal);
+    sgx = add_halves(sgx); sgy = add_halves(sgy);
+    sxx = add_halves(sxx); sxy = add_halves(sxy); syy = add_halves(syy);
+    dz = add_halves(dz); dphx = add_halves(dphx); dphy = add_halves(dphy);
     dn += (uint32_t)__shfl_xor((int)dn, 32);
     if (half == 0 && valid && dn > 0) {
         Delta2D* d = delta + idx[s + sidx];
-        atomicAdd(&d->depth, ddep);
+        atomicAdd(&d->depth, -dz);
         atomicAdd(&d->phx, dphx);
         atomicAdd(&d->phy, dphy);
-        atomicAdd(&d->mx, dmx);
-        atomicAdd(&d->my, dmy);
-        atomicAdd(&d->sx, dsx);
-        atomicAdd(&d->sy, dsy);
-        atomicAdd(&d->sz, dsz);
+        atomicAdd(&d->mx, -(r0.z * sgx + r0.w * sgy) - r1.z * dz);
+        atomicAdd(&d->my, -(r0.w * sgx + r1.x * sgy) - r1.w * dz);
+        atomicAdd(&d->sx, -0.5f * sxx);
+        atomicAdd(&d->sy, -0.5f * sxy);
+        atomicAdd(&d->sz, -0.5f * syy);
         atomicAdd(&d->cr, dcr);
         atomicAdd(&d->cg, dcg);
         atomicAdd(&d->cb, dcb);
