@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
     constexpr bool kKahan = kPass == 3;
     __shared__ Rec recs[256];
     __shared__ uint8_t qmask[256];
-    __shared__ uint8_t wlist[4][256];
+    __shared__ __align__(4) uint8_t wlist[4][256];
     __shared__ double red[4][27];
     const int tile = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int tx0 = (tile % v.ntx) * kTile, ty0 = (tile / v.ntx) * kTile;
@@ -197,21 +197,36 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
             if (__syncthreads_count(!done) == 0) break;
             const int wc = stage(base, 1);
             if (done) continue;
-            for (int k = 0; k < wc; ++k) {
-                const int i = wlist[w][k];
-                const float4 r0 = recs[i].r0, r1 = recs[i].r1;
-                const float dx = r0.x - px, dy = r0.y - py;
-                const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;  // Q14
-                const float4 r2 = recs[i].r2;
-                const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));
-                if (a < kAlphaMin || q <= 0.0f) continue;
-                const float tT = T * (1.0f - a);
-                if (tT < 0.0001f) { done = true; break; }
-                C0 += r2.x * a * T; C1 += r2.y * a * T; C2 += r2.z * a * T;
-                const float d = r1.y + dx * r1.z + dy * r1.w;
-                if (T > 0.5f && tT < 0.5f) D = d;
-                T = tT;
-                last = base + (uint32_t)i + 1;
+            // Four entries per round, their record loads issued together (indices past the list
+            // are stale bytes < 256: in bounds, never used).
+            const uint32_t* wl4 = reinterpret_cast<const uint32_t*>(wlist[w]);
+            for (int k = 0; k < wc && !done; k += 4) {
+                const uint32_t w4 = wl4[k >> 2];
+                int ii[4];
+                float4 b0[4], b1[4], b2[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    ii[u] = (int)((w4 >> (8 * u)) & 255u);
+                    b0[u] = recs[ii[u]].r0;
+                    b1[u] = recs[ii[u]].r1;
+                    b2[u] = recs[ii[u]].r2;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (k + u >= wc || done) break;
+                    const float4 r0 = b0[u], r1 = b1[u], r2 = b2[u];
+                    const float dx = r0.x - px, dy = r0.y - py;
+                    const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;  // Q14
+                    const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));
+                    if (a < kAlphaMin || q <= 0.0f) continue;
+                    const float tT = T * (1.0f - a);
+                    if (tT < 0.0001f) { done = true; break; }
+                    C0 += r2.x * a * T; C1 += r2.y * a * T; C2 += r2.z * a * T;
+                    const float d = r1.y + dx * r1.z + dy * r1.w;
+                    if (T > 0.5f && tT < 0.5f) D = d;
+                    T = tT;
+                    last = base + (uint32_t)ii[u] + 1;
+                }
             }
         }
         if (inside) {
