@@ -6,13 +6,16 @@
 // and computeIndicesRanges_kernel (669-707).
 //
 // Design (one GPU, HBM resident):
-//   k_project    one thread per Gaussian, 1024 Gaussians per block: screen record (64 B),
+//   k_project    one thread per Gaussian, kBinChunk Gaussians per block: screen record (64 B),
 //                tile rectangle, and a per-block tile histogram in LDS written as one row of
-//                the [blocks x tiles] count matrix (no global atomics).
-//   k_colscan1/2 column prefix of the count matrix in chunks of kScanRows rows, then the
-//                tile totals' exclusive scan: tile ranges and the instance count M.
+//                the [blocks x tiles] count matrix (no global atomics). The histogram walk
+//                deals each wave's tile instances evenly over its lanes (wave_walk_*).
+//   k_colscan_*  column prefix of the count matrix (64 columns per workgroup, waves over row
+//                runs), then the tile totals' exclusive scan: tile ranges, the instance count
+//                M and each column's base offset.
 //   k_emit       per block, LDS cursors at the block's base offset in every tile; each instance
-//                writes (depth bits << 32 | gaussian index) into its tile's segment.
+//                writes (depth bits << 32 | gaussian index) into its tile's segment, through
+//                the same even walk.
 //   k_tile_sort  one workgroup per tile: LSD radix sort by (depth bits, index), 8-bit
 //                digits, stable wave-level ranks from ballots. Segments up to NT*E entries
 //                stay in LDS and sort on the depth bits, then order equal depths by index;
@@ -159,6 +162,121 @@ __device__ __forceinline__ uint32_t tile_depth_bits(float u, float v, float z, f
     return __float_as_uint(d);
 }
 
+// Tile instances of a packed rectangle (x0 | x1 << 16, y0 | y1 << 16); 0 for the empty
+// rectangle (1, 0) and for the culled marker.
+__device__ __forceinline__ uint32_t rect_count(uint2 rc) {
+    if (rc.y == kCulledY) return 0u;
+    const int x0 = rc.x & 0xffff, x1 = rc.x >> 16, y0 = rc.y & 0xffff, y1 = rc.y >> 16;
+    return x0 > x1 ? 0u : (uint32_t)((x1 - x0 + 1) * (y1 - y0 + 1));
+}
+
+// Wave-cooperative walk over the tile rectangles of a wave's 64 Gaussians. Rectangle sizes are
+// heavy-tailed (a few tiles on average, hundreds for near, large Gaussians), so a loop per lane
+// over its own rectangle runs every wave for its largest one. Instead the wave numbers its
+// instances through a scan of the lane counts and deals them 64 at a time: lane l of window b0
+// takes instance j = b0 + l, finds the owning lane by binary search over the exclusive
+// prefixes and maps its local index to (tx, ty) row-major inside the owner's rectangle.
+// Trip count: ceil(instances / 64) per wave.
+struct WaveWalk {
+    uint32_t excl, total;
+};
+
+__device__ __forceinline__ WaveWalk wave_walk_begin(uint32_t cnt, int lane) {
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, o);
+        if (lane >= o) incl += y;
+    }
+    return WaveWalk{incl - cnt, (uint32_t)__shfl((int)incl, 63)};
+}
+
+struct WalkSlot {
+    int owner, tx, ty;
+    bool on;
+};
+
+// Instance b0 + lane of the walk. All lanes of the wave call it (it shuffles); slots past the
+// wave's total come back with on == false.
+__device__ __forceinline__ WalkSlot wave_walk_slot(const WaveWalk& ww, uint32_t b0, uint2 rc,
+                                                   int lane) {
+    const uint32_t j = b0 + (uint32_t)lane;
+    // largest o with excl[o] <= j: the prefixes are nondecreasing and excl[0] == 0; among lanes
+    // sharing a prefix the last one is the one with instances
+    int o = 0;
+    uint32_t eo = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        const uint32_t e = (uint32_t)__shfl((int)ww.excl, o + step);
+        if (e <= j) {
+            o += step;
+            eo = e;
+        }
+    }
+    const uint32_t rx = (uint32_t)__shfl((int)rc.x, o);
+    const uint32_t ry = (uint32_t)__shfl((int)rc.y, o);
+    WalkSlot s;
+    s.owner = o;
+    s.on = j < ww.total;
+    const uint32_t x0 = rx & 0xffffu, wx = (rx >> 16) - x0 + 1u;
+    const uint32_t k = j - eo;
+    // k / wx for k, wx below 2^16: reciprocal estimate, then one correction each way
+    uint32_t q = (uint32_t)((float)k * __builtin_amdgcn_rcpf((float)wx));
+    int r = (int)k - (int)(q * wx);
+    if (r < 0) {
+        --q;
+        r += (int)wx;
+    }
+    if (r >= (int)wx) {
+        ++q;
+        r -= (int)wx;
+    }
+    s.tx = (int)x0 + r;
+    s.ty = (int)(ry & 0xffffu) + (int)q;
+    return s;
+}
+
+// GSCudaKernels.cu:345-476 and 566-591 for Gaussian g: stores its screen record and tile
+// rectangle and returns the rectangle.
+__device__ __forceinline__ uint2 project_store(const MapRef& map, int g, const View& v,
+                                               const float Rc[3][3], int cull_emit,
+                                               Rec* __restrict__ rec, uint2* __restrict__ rect,
+                                               uint32_t* __restrict__ zinfo) {
+    float u = 0.f, vv = 0.f, z = 0.f, cov[3] = {0.f, 0.f, 0.f}, con[3] = {0.f, 0.f, 0.f},
+          ph[2] = {0.f, 0.f};
+    float pw[3], sc[3], qg[4], col[3], op;
+    map.geometry((size_t)g, pw, sc, qg);
+    map.appearance((size_t)g, col, op);
+    const bool vis = project_exact(pw, sc, qg, v, Rc, u, vv, z, cov, con, ph);
+    if (!vis) {
+        u = vv = z = 0.f;
+        cov[0] = cov[1] = cov[2] = 0.f;
+        con[0] = con[1] = con[2] = 0.f;
+        ph[0] = ph[1] = 0.f;
+    }
+    Rec r;
+    r.r0 = make_float4(u, vv, con[0], con[1]);
+    r.r1 = make_float4(con[2], z, ph[0], ph[1]);
+    r.r2 = make_float4(col[0], col[1], col[2], op);
+    r.r3 = make_float4(cov[0], cov[1], cov[2], 0.f);
+    rec[g] = r;
+    uint2 rc;
+    if (vis) {
+        rc = tile_rect_exact(u, vv, cov[0], cov[1], v);
+        // A visible instance in tile 0 with key 0 (linear depth +0.0) sorts among the culled
+        // ones: flag it, tile 0 then sorts its whole list.
+        if ((rc.x & 0xffff) == 0 && (rc.y & 0xffff) == 0 &&
+            tile_depth_bits(u, vv, z, ph[0], ph[1], 0, 0) == 0u)
+            atomicAdd(zinfo + 1, 1u);
+    } else {
+        // culled: the zero record's single instance (tile 0, key 0) goes to the zero segment
+        // at the head of tile 0's list; none when cull_emit is 0
+        rc = cull_emit ? make_uint2(0u, kCulledY) : make_uint2(1u, 0u);
+    }
+    rect[g] = rc;
+    return rc;
+}
+
 __global__ __launch_bounds__(kBinThreads) void k_project(
     MapRef map, int n, View vh, const View* __restrict__ vd, int cull_emit,
     Rec* __restrict__ rec, uint2* __restrict__ rect, uint32_t* __restrict__ blk_counts,
@@ -170,113 +288,94 @@ __global__ __launch_bounds__(kBinThreads) void k_project(
     __syncthreads();
     float Rc[3][3];
     quat_to_mat_exact(v.qx, v.qy, v.qz, v.qw, Rc);
+    const int lane = threadIdx.x & 63;
     for (int it = 0; it < kBinItems; ++it) {
-        int g = blockIdx.x * kBinChunk + it * kBinThreads + threadIdx.x;
-        if (g >= n) break;
-        float u = 0.f, vv = 0.f, z = 0.f, cov[3] = {0.f, 0.f, 0.f}, con[3] = {0.f, 0.f, 0.f},
-              ph[2] = {0.f, 0.f};
-        float pw[3], sc[3], qg[4], col[3], op;
-        map.geometry((size_t)g, pw, sc, qg);
-        map.appearance((size_t)g, col, op);
-        bool vis = project_exact(pw, sc, qg, v, Rc, u, vv, z, cov, con, ph);
-        if (!vis) {
-            u = vv = z = 0.f;
-            cov[0] = cov[1] = cov[2] = 0.f;
-            con[0] = con[1] = con[2] = 0.f;
-            ph[0] = ph[1] = 0.f;
+        const int g = blockIdx.x * kBinChunk + it * kBinThreads + threadIdx.x;
+        const uint2 rc =
+            g < n ? project_store(map, g, v, Rc, cull_emit, rec, rect, zinfo) : make_uint2(1u, 0u);
+        const uint64_t culled = __ballot(rc.y == kCulledY);
+        if (lane == 0 && culled) atomicAdd(&hist[0], (uint32_t)__popcll(culled));
+        const WaveWalk ww = wave_walk_begin(rect_count(rc), lane);
+        for (uint32_t b0 = 0; b0 < ww.total; b0 += 64) {
+            const WalkSlot sl = wave_walk_slot(ww, b0, rc, lane);
+            if (sl.on) atomicAdd(&hist[1 + sl.ty * v.ntx + sl.tx], 1u);
         }
-        Rec r;
-        r.r0 = make_float4(u, vv, con[0], con[1]);
-        r.r1 = make_float4(con[2], z, ph[0], ph[1]);
-        r.r2 = make_float4(col[0], col[1], col[2], op);
-        r.r3 = make_float4(cov[0], cov[1], cov[2], 0.f);
-        rec[g] = r;
-        uint2 rc;
-        if (vis) {
-            rc = tile_rect_exact(u, vv, cov[0], cov[1], v);
-            // A visible instance in tile 0 with key 0 (linear depth +0.0) sorts among the
-            // culled ones: flag it, tile 0 then sorts its whole list.
-            if ((rc.x & 0xffff) == 0 && (rc.y & 0xffff) == 0 &&
-                tile_depth_bits(u, vv, z, ph[0], ph[1], 0, 0) == 0u)
-                atomicAdd(zinfo + 1, 1u);
-        } else {
-            // culled: the zero record's single instance (tile 0, key 0) goes to the zero
-            // segment at the head of tile 0's list; none when cull_emit is 0
-            rc = cull_emit ? make_uint2(0u, kCulledY) : make_uint2(1u, 0u);
-        }
-        rect[g] = rc;
-        if (rc.y == kCulledY) {
-            atomicAdd(&hist[0], 1u);
-            continue;
-        }
-        int x0 = rc.x & 0xffff, x1 = rc.x >> 16, y0 = rc.y & 0xffff, y1 = rc.y >> 16;
-        for (int ty = y0; ty <= y1; ++ty)
-            for (int tx = x0; tx <= x1; ++tx) atomicAdd(&hist[1 + ty * v.ntx + tx], 1u);
     }
     __syncthreads();
     uint32_t* row = blk_counts + (size_t)blockIdx.x * (T + 1);
     for (int t = threadIdx.x; t <= T; t += kBinThreads) row[t] = hist[t];
 }
 
-// Column prefix in chunks of kScanRows rows: rows become within-chunk exclusive prefixes,
-// chunk_pre[c][t] receives the chunk's column sum.
-__global__ __launch_bounds__(256) void k_colscan1(uint32_t* __restrict__ blk_counts,
-                                                  uint32_t* __restrict__ chunk_pre, int nblk,
-                                                  int T) {
-    int t = blockIdx.x * 256 + threadIdx.x;
-    int c = blockIdx.y;
-    if (t >= T) return;
-    int b0 = c * kScanRows, b1 = min(nblk, b0 + kScanRows);
-    uint32_t run = 0;
-    for (int b = b0; b < b1; ++b) {
-        uint32_t x = blk_counts[(size_t)b * T + t];
-        blk_counts[(size_t)b * T + t] = run;
+// Column prefix of the [blocks x columns] count matrix, 64 columns per workgroup (lane =
+// column): every wave sums a contiguous run of rows, the sums of the waves are combined in LDS,
+// and the rows are rewritten as exclusive column prefixes. tot[c] receives column c's total.
+__global__ __launch_bounds__(1024) void k_colscan_rows(uint32_t* __restrict__ blk_counts,
+                                                      uint32_t* __restrict__ tot, int nblk,
+                                                      int cols) {
+    __shared__ uint32_t wsum[16][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int col = blockIdx.x * 64 + lane;
+    const bool on = col < cols;
+    const int per = (nblk + 15) / 16;
+    const int r0 = min(nblk, w * per), r1 = min(nblk, r0 + per);
+    uint32_t sum = 0;
+    if (on) {
+#pragma unroll 8
+        for (int r = r0; r < r1; ++r) sum += blk_counts[(size_t)r * cols + col];
+    }
+    wsum[w][lane] = sum;
+    __syncthreads();
+    uint32_t run = 0, all = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t x = wsum[k][lane];
+        run += k < w ? x : 0u;
+        all += x;
+    }
+    if (!on) return;
+#pragma unroll 8
+    for (int r = r0; r < r1; ++r) {
+        uint32_t* p = blk_counts + (size_t)r * cols + col;
+        const uint32_t x = *p;
+        *p = run;
         run += x;
     }
-    chunk_pre[(size_t)c * T + t] = run;
+    if (w == 0) tot[col] = all;
 }
 
-// Tile totals, their exclusive scan (ranges), and chunk bases folded with the tile offset.
-// One block of 1024 threads. When the instance count exceeds the capacity of the instance
-// buffers (cap), every range is written empty so that the downstream kernels do no work and
-// touch nothing; the host sees M > cap, grows the buffers and runs the view again.
-// T here counts the columns of the count matrix: the zero segment and the tiles.
-__global__ __launch_bounds__(1024) void k_colscan2(uint32_t* __restrict__ chunk_pre, int nchunk,
-                                                   int T, uint32_t* __restrict__ ranges,
-                                                   uint32_t* __restrict__ d_m,
-                                                   uint32_t* __restrict__ zinfo, uint32_t cap) {
-    extern __shared__ uint32_t tot[];   // T entries, then 1024 + 32 scratch
-    uint32_t* part = tot + T;
+// Exclusive scan of the column totals (one block of 1024 threads): tile ranges, the instance
+// count M, the zero-segment length, and in place of each total the column's base offset. When
+// the instance count exceeds the capacity of the instance buffers (cap), every range is written
+// empty so that the downstream kernels do no work and touch nothing; the host sees M > cap,
+// grows the buffers and runs the view again. Column 0 is the zero segment (culled instances,
+// key 0), placed at the head of tile 0's list; column c >= 1 is tile c - 1.
+__global__ __launch_bounds__(1024) void k_colscan_tiles(uint32_t* __restrict__ tot_base, int cols,
+                                                       uint32_t* __restrict__ ranges,
+                                                       uint32_t* __restrict__ d_m,
+                                                       uint32_t* __restrict__ zinfo,
+                                                       uint32_t cap) {
+    extern __shared__ uint32_t tot[];   // cols entries, then 1024 scratch
+    uint32_t* part = tot + cols;
     const int tid = threadIdx.x;
-    for (int t = tid; t < T; t += 1024) {
-        uint32_t run = 0;
-        for (int c = 0; c < nchunk; ++c) {
-            uint32_t x = chunk_pre[(size_t)c * T + t];
-            chunk_pre[(size_t)c * T + t] = run;
-            run += x;
-        }
-        tot[t] = run;
-    }
+    for (int t = tid; t < cols; t += 1024) tot[t] = tot_base[t];
     __syncthreads();
-    // Block exclusive scan of tot[0..T): contiguous segment per thread.
-    int per = (T + 1023) / 1024;
-    int a = tid * per, b = min(T, a + per);
+    // contiguous segment per thread, then a scan of the segment sums
+    const int per = (cols + 1023) / 1024;
+    const int a = min(cols, tid * per), b = min(cols, a + per);
     uint32_t sum = 0;
     for (int i = a; i < b; ++i) sum += tot[i];
     part[tid] = sum;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t x = (tid >= off) ? part[tid - off] : 0u;
+        const uint32_t x = (tid >= off) ? part[tid - off] : 0u;
         __syncthreads();
         part[tid] += x;
         __syncthreads();
     }
     uint32_t run = part[tid] - sum;   // exclusive prefix of this thread's segment
     const bool overflow = part[1023] > cap;
-    // Column 0 is the zero segment (culled instances, key 0), placed at the head of tile 0's
-    // list; column c >= 1 is tile c - 1.
     for (int i = a; i < b; ++i) {
-        uint32_t x = tot[i];
+        const uint32_t x = tot[i];
         if (i > 0) {
             const uint32_t start = (i == 1) ? 0u : run, end = run + x;
             // tiles without instances keep (0, 0), the reference's memset value
@@ -287,28 +386,22 @@ __global__ __launch_bounds__(1024) void k_colscan2(uint32_t* __restrict__ chunk_
         } else {
             zinfo[0] = overflow ? 0u : x;   // zero-segment length
         }
-        tot[i] = run;
+        tot_base[i] = run;
         run += x;
     }
     if (tid == 1023) *d_m = part[1023];
-    __syncthreads();
-    for (int t = tid; t < T; t += 1024) {
-        uint32_t base = tot[t];
-        for (int c = 0; c < nchunk; ++c) chunk_pre[(size_t)c * T + t] += base;
-    }
 }
 
 __global__ __launch_bounds__(kBinThreads) void k_emit(
     const Rec* __restrict__ rec, const uint2* __restrict__ rect,
-    const uint32_t* __restrict__ blk_counts, const uint32_t* __restrict__ chunk_pre, int n,
+    const uint32_t* __restrict__ blk_counts, const uint32_t* __restrict__ col_base, int n,
     View vh, const View* __restrict__ vd, uint64_t* __restrict__ pairs, uint64_t cap) {
     extern __shared__ uint32_t cur[];   // [0] zero segment, [1 + t] tile t
     __shared__ uint32_t wz[kBinThreads / 64];
     const View v = vd ? *vd : vh;
     const int T = v.ntx * v.nty;
     const uint32_t* row = blk_counts + (size_t)blockIdx.x * (T + 1);
-    const uint32_t* crow = chunk_pre + (size_t)(blockIdx.x / kScanRows) * (T + 1);
-    for (int t = threadIdx.x; t <= T; t += kBinThreads) cur[t] = crow[t] + row[t];
+    for (int t = threadIdx.x; t <= T; t += kBinThreads) cur[t] = col_base[t] + row[t];
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int it = 0; it < kBinItems; ++it) {
@@ -332,16 +425,27 @@ __global__ __launch_bounds__(kBinThreads) void k_emit(
         }
         __syncthreads();
         if (threadIdx.x == 0) cur[0] += total;
-        if (!valid || culled) continue;
-        const int x0 = rc.x & 0xffff, x1 = rc.x >> 16, y0 = rc.y & 0xffff, y1 = rc.y >> 16;
-        if (x0 > x1) continue;
-        const float4 r0 = rec[g].r0, r1 = rec[g].r1;
-        for (int ty = y0; ty <= y1; ++ty)
-            for (int tx = x0; tx <= x1; ++tx) {
-                uint32_t bits = tile_depth_bits(r0.x, r0.y, r1.y, r1.z, r1.w, tx, ty);
-                uint32_t p = atomicAdd(&cur[1 + ty * v.ntx + tx], 1u);
-                if (p < cap) pairs[p] = ((uint64_t)bits << 32) | (uint32_t)g;
+        // visible instances, dealt evenly over the wave's lanes
+        const uint2 vrc = valid ? rc : make_uint2(1u, 0u);
+        const uint32_t cnt = rect_count(vrc);
+        float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0;
+        if (cnt) {
+            r0 = rec[g].r0;
+            r1 = rec[g].r1;
+        }
+        const WaveWalk ww = wave_walk_begin(cnt, lane);
+        const int g0 = g - lane;
+        for (uint32_t b0 = 0; b0 < ww.total; b0 += 64) {
+            const WalkSlot sl = wave_walk_slot(ww, b0, vrc, lane);
+            const float u = __shfl(r0.x, sl.owner), vv = __shfl(r0.y, sl.owner);
+            const float z = __shfl(r1.y, sl.owner), phx = __shfl(r1.z, sl.owner),
+                        phy = __shfl(r1.w, sl.owner);
+            if (sl.on) {
+                const uint32_t bits = tile_depth_bits(u, vv, z, phx, phy, sl.tx, sl.ty);
+                const uint32_t p = atomicAdd(&cur[1 + sl.ty * v.ntx + sl.tx], 1u);
+                if (p < cap) pairs[p] = ((uint64_t)bits << 32) | (uint32_t)(g0 + sl.owner);
             }
+        }
     }
 }
 
@@ -732,28 +836,27 @@ hipError_t launch_dequantize(const MapRef& map, int n, float* out14, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* chunk_pre, int nblk, int ntiles,
+hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* col_base, int nblk, int ntiles,
                           uint32_t* ranges, uint32_t* d_m, uint32_t* zinfo, uint32_t cap,
                           hipStream_t s) {
     const int cols = ntiles + 1;   // zero segment + tiles
-    int nchunk = (nblk + kScanRows - 1) / kScanRows;
-    hipLaunchKernelGGL(k_colscan1, dim3((cols + 255) / 256, nchunk), dim3(256), 0, s,
-                       blk_counts, chunk_pre, nblk, cols);
+    hipLaunchKernelGGL(k_colscan_rows, dim3((cols + 63) / 64), dim3(1024), 0, s, blk_counts,
+                       col_base, nblk, cols);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    size_t lds = ((size_t)cols + 1024 + 32) * 4;
-    hipLaunchKernelGGL(k_colscan2, dim3(1), dim3(1024), lds, s, chunk_pre, nchunk, cols,
-                       ranges, d_m, zinfo, cap);
+    const size_t lds = ((size_t)cols + 1024) * 4;
+    hipLaunchKernelGGL(k_colscan_tiles, dim3(1), dim3(1024), lds, s, col_base, cols, ranges, d_m,
+                       zinfo, cap);
     return hipGetLastError();
 }
 
 hipError_t launch_emit(const Rec* rec, const uint2* rect, const uint32_t* blk_counts,
-                       const uint32_t* chunk_pre, int n, const View& v, const View* vd,
+                       const uint32_t* col_base, int n, const View& v, const View* vd,
                        uint64_t* pairs, uint64_t cap, hipStream_t s) {
     int nblk = (n + kBinChunk - 1) / kBinChunk;
     size_t lds = ((size_t)v.ntx * v.nty + 1) * 4;
     hipLaunchKernelGGL(k_emit, dim3(nblk), dim3(kBinThreads), lds, s, rec, rect, blk_counts,
-                       chunk_pre, n, v, vd, pairs, cap);
+                       col_base, n, v, vd, pairs, cap);
     return hipGetLastError();
 }
 

@@ -78,7 +78,7 @@ struct fvgs_ctx {
     uint32_t m = 0, inst_cap = 0;
     bool pending_m = false;
     // screen + binning
-    DevBuf rec, rect, blk_counts, chunk_pre, ranges, d_m, pairs, tmp, sorted_idx, keys;
+    DevBuf rec, rect, blk_counts, col_base, ranges, d_m, pairs, tmp, sorted_idx, keys;
     // outputs
     DevBuf r_rgb, r_depth, r_alpha;
     DevBuf o_rgb, o_depth, o_T, o_nc, o_err, o_maxc;
@@ -254,11 +254,10 @@ int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int
         return fail(c, FVGS_E_ARG, "%d tiles exceed the supported %d", T, kMaxTiles);
     const int n = (int)c->n;
     const int nblk = (n + kBinChunk - 1) / kBinChunk;
-    const int nchunk = (nblk + kScanRows - 1) / kScanRows;
     TRY(ensure(c, c->rec, sizeof(Rec) * (size_t)n));
     TRY(ensure(c, c->rect, sizeof(uint2) * (size_t)n));
     TRY(ensure(c, c->blk_counts, 4ull * nblk * (T + 1)));
-    TRY(ensure(c, c->chunk_pre, 4ull * nchunk * (T + 1)));
+    TRY(ensure(c, c->col_base, 4ull * (T + 1)));
     TRY(ensure(c, c->ranges, 8ull * T));
     TRY(ensure(c, c->d_m, 16));   // M | zero-segment length | visible key-0 count in tile 0
     if (async && c->inst_cap == 0) async = false;
@@ -269,7 +268,7 @@ int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int
     HIPCHK(c, launch_zero(zinfo + 1, 4, s));
     HIPCHK(c, launch_project(map_ref(c), n, v, c->view_dev, cull_emit, as<Rec>(c->rec),
                              as<uint2>(c->rect), as<uint32_t>(c->blk_counts), zinfo, s));
-    HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->chunk_pre), nblk, T,
+    HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->col_base), nblk, T,
                              as<uint32_t>(c->ranges), as<uint32_t>(c->d_m), zinfo,
                              async ? c->inst_cap : 0xffffffffu, s));
     HIPCHK(c, hipMemcpyAsync(c->h_m, c->d_m.p, 4, hipMemcpyDeviceToHost, s));
@@ -281,7 +280,7 @@ int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int
     }
     uint64_t* keys = c->prm.keep_keys ? as<uint64_t>(c->keys) : nullptr;
     HIPCHK(c, launch_emit(as<Rec>(c->rec), as<uint2>(c->rect), as<uint32_t>(c->blk_counts),
-                          as<uint32_t>(c->chunk_pre), n, v, c->view_dev, as<uint64_t>(c->pairs),
+                          as<uint32_t>(c->col_base), n, v, c->view_dev, as<uint64_t>(c->pairs),
                           c->inst_cap,
                           s));
     HIPCHK(c, launch_tile_sort(as<uint32_t>(c->ranges), T, n, as<uint64_t>(c->pairs),
@@ -539,7 +538,7 @@ int fvgs_destroy(fvgs_ctx* c) {
                           &c->q_origin, &c->q_pos, &c->q_scale, &c->q_quat, &c->q_color,
                           &c->q_opac, &c->map_out, &c->adam,
                           &c->own_rgb, &c->own_depth, &c->own_gx, &c->own_gy, &c->rec, &c->rect,
-                          &c->blk_counts, &c->chunk_pre, &c->ranges, &c->d_m, &c->pairs,
+                          &c->blk_counts, &c->col_base, &c->ranges, &c->d_m, &c->pairs,
                           &c->tmp, &c->sorted_idx, &c->keys, &c->r_rgb, &c->r_depth, &c->r_alpha,
                           &c->o_rgb, &c->o_depth, &c->o_T, &c->o_nc, &c->o_err, &c->o_maxc,
                           &c->d2, &c->d3, &c->ckpt, &c->pose_tiles, &c->pose_sum, &c->t_rgb, &c->t_depth, &c->pose_pix};
@@ -1317,10 +1316,10 @@ int fvgs_track(fvgs_ctx* c, const fvgs_intrinsics* intr, int iters, double pose7
             HIPCHK(c, launch_zero(zinfo + 1, 4, s));
             HIPCHK(c, launch_project(map_ref(c), n, v0, vd, 0, as<Rec>(c->rec), as<uint2>(c->rect),
                                      as<uint32_t>(c->blk_counts), zinfo, s));
-            HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->chunk_pre), nblk,
+            HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->col_base), nblk,
                                      T, as<uint32_t>(c->ranges), dm + it, zinfo, c->inst_cap, s));
             HIPCHK(c, launch_emit(as<Rec>(c->rec), as<uint2>(c->rect), as<uint32_t>(c->blk_counts),
-                                  as<uint32_t>(c->chunk_pre), n, v0, vd, as<uint64_t>(c->pairs),
+                                  as<uint32_t>(c->col_base), n, v0, vd, as<uint64_t>(c->pairs),
                                   c->inst_cap, s));
             HIPCHK(c, launch_tile_sort(as<uint32_t>(c->ranges), T, n, as<uint64_t>(c->pairs),
                                        as<uint64_t>(c->tmp), as<uint32_t>(c->sorted_idx), nullptr,

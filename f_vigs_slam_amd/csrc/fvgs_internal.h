@@ -15,7 +15,6 @@ constexpr int kMaxTiles = 8192;    // LDS tile histogram of the binning kernels 
 constexpr int kBinThreads = 1024;  // binning block
 constexpr int kBinItems = 4;       // Gaussians per thread per binning block
 constexpr int kBinChunk = kBinThreads * kBinItems;
-constexpr int kScanRows = 16;      // binning blocks per column-scan chunk
 
 // View description passed by value to kernels.
 struct View {
@@ -109,11 +108,11 @@ hipError_t launch_quantize(const float* pos, const float* scale, const float* qu
 hipError_t launch_dequantize(const MapRef& map, int n, float* out14, hipStream_t s);
 // Count matrix columns: 0 = zero segment (culled instances, head of tile 0's list), 1 + t =
 // tile t. Writes the tile ranges, *d_m = M and zinfo[0] = zero-segment length.
-hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* chunk_pre, int nblk, int ntiles,
+hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* col_base, int nblk, int ntiles,
                           uint32_t* ranges, uint32_t* d_m, uint32_t* zinfo, uint32_t cap,
                           hipStream_t s);
 hipError_t launch_emit(const Rec* rec, const uint2* rect, const uint32_t* blk_counts,
-                       const uint32_t* chunk_pre, int n, const View& v, const View* vd,
+                       const uint32_t* col_base, int n, const View& v, const View* vd,
                        uint64_t* pairs, uint64_t cap, hipStream_t s);
 hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uint64_t* pairs,
                             uint64_t* tmp, uint32_t* sorted_idx, uint64_t* keys_out,
