@@ -18,9 +18,10 @@
 //                the same even walk.
 //   k_tile_sort  one workgroup per tile: LSD radix sort by (depth bits, index), 8-bit
 //                digits, stable wave-level ranks from ballots. Segments up to NT*E entries
-//                stay in LDS and sort on the depth bits, then order equal depths by index;
-//                longer ones sort the unique key ((depth bits - tile min) << index bits |
-//                index) through a global ping-pong buffer.
+//                stay in LDS and sort on the top 24 bits of the tile's depth span, then put
+//                each short run of equal prefix in (depth, index) order; longer ones sort the
+//                unique key ((depth bits - tile min) << index bits | index) through a global
+//                ping-pong buffer.
 // The resulting order equals the reference's stable sort of (tile << 32 | depth bits) with
 // emission in Gaussian-index order: ties on the depth bits resolve by ascending index.
 //
@@ -453,9 +454,11 @@ __global__ __launch_bounds__(kBinThreads) void k_emit(
 // Per-tile LSD radix sort.
 //
 // One workgroup per tile sorts by (depth bits, index), 8 bits per pass. Lists that fit in LDS
-// are sorted on the depth bits only and the (rare) runs of equal depth are then ordered by
-// index; longer lists, and lists with many equal depths, sort the unique 64-bit key
-// ((depth bits - tile min) << idx_bits | idx) on its significant bits. Ranks are stable: within a wave, item by item, the
+// are sorted on the top 24 bits of (depth bits - tile min) only (three passes where negative
+// depths stretch the span to 32 bits, Q2), and the short runs of equal prefix are then put in
+// (depth, index) order one run per thread; longer lists, and lists with long runs (many equal
+// depths), sort the unique 64-bit key ((depth bits - tile min) << idx_bits | idx) on its
+// significant bits. Ranks are stable: within a wave, item by item, the
 // lanes holding the same digit are found with 8 ballots ("match"), ranked by lane order and
 // offset by a per-wave digit counter in LDS; per-digit offsets across waves come from a prefix
 // over the per-wave counters. Elements are laid out w*64*ipt + e*64 + lane with ipt =
@@ -464,15 +467,26 @@ __global__ __launch_bounds__(kBinThreads) void k_emit(
 // sweep plus chunk-by-chunk stable scatter through HBM (pairs <-> tmp).
 // ------------------------------------------------------------------------------------------
 
+// Lanes of the wave (within active) whose 8-bit digit equals this lane's. Per bit b the ballot
+// m_b is compared with the lane's own bit spread to a word (0 or ~0): a lane differs from lane
+// j iff bit j of some m_b ^ spread_b is set, so the differing lanes are the OR of those words
+// (two VALU per bit and half: xor, or; the ballot is the compare).
 __device__ __forceinline__ uint64_t match8(uint32_t d, uint64_t active) {
-    uint64_t peers = active;
+    uint32_t dlo = 0, dhi = 0;
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-        const int bit = (d >> b) & 1;
-        const uint64_t m = __ballot(bit);
-        peers &= bit ? m : ~m;
+        const uint32_t spread = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+        const uint64_t m = __ballot(spread != 0u);
+        dlo |= (uint32_t)m ^ spread;
+        dhi |= (uint32_t)(m >> 32) ^ spread;
     }
-    return peers;
+    return active & ~(((uint64_t)dhi << 32) | dlo);
+}
+
+// Set bits of mask below this lane (v_mbcnt).
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
 template <int NT>
@@ -535,7 +549,6 @@ __device__ __forceinline__ void radix_pass(const uint64_t* src, int cnt, int ipt
                                            uint64_t* dst_glob, uint32_t limit) {
     constexpr int NW = NT / 64;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     for (int i = tid; i < NW * 256; i += NT) whist[i] = 0;
     __syncthreads();
     uint32_t rk[E];
@@ -547,7 +560,7 @@ __device__ __forceinline__ void radix_pass(const uint64_t* src, int cnt, int ipt
             const bool valid = j < cnt;
             const uint32_t d = valid ? (uint32_t)(src[j] >> shift) & 255u : 0u;
             const uint64_t peers = match8(d, __ballot(valid));
-            const uint32_t lr = __popcll(peers & lt);
+            const uint32_t lr = lanes_below(peers);
             uint32_t base = 0;
             if (valid) base = whist[w * 256 + d];
             if (valid && lr == 0) whist[w * 256 + d] = base + (uint32_t)__popcll(peers);
@@ -636,48 +649,84 @@ __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ r
     }
     const bool resident = n <= (uint32_t)CHUNK;
     uint32_t mn = 0xffffffffu, mx = 0u;
-    for (uint32_t j = tid; j < n; j += NT) {
-        const uint64_t p = pairs[s + j];
-        const uint32_t d = (uint32_t)(p >> 32);
-        mn = min(mn, d);
-        mx = max(mx, d);
-        if (resident) keysA[j] = p;
+    uint64_t held[E];   // resident lists: this thread's entries j = tid + e * NT
+    if (resident) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t j = tid + (uint32_t)(e * NT);
+            held[e] = j < n ? pairs[s + j] : 0ull;
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (tid + (uint32_t)(e * NT) < n) {
+                const uint32_t d = (uint32_t)(held[e] >> 32);
+                mn = min(mn, d);
+                mx = max(mx, d);
+            }
+        }
+    } else {
+        for (uint32_t j = tid; j < n; j += NT) {
+            const uint32_t d = (uint32_t)(pairs[s + j] >> 32);
+            mn = min(mn, d);
+            mx = max(mx, d);
+        }
     }
     block_minmax<NT>(mn, mx, misc);
     const uint32_t span = mx - mn;
-    const int total_bits = (span ? 32 - __clz(span) : 0) + idx_bits;
+    const int span_bits = span ? 32 - __clz(span) : 0;
+    const int total_bits = span_bits + idx_bits;
     const int passes = (total_bits + 7) / 8;
     auto xform = [&](uint64_t p) -> uint64_t {
         return ((uint64_t)((uint32_t)(p >> 32) - mn) << idx_bits) | (p & 0xffffffffull);
     };
     if (resident) {
-        // Depth passes only: key = (depth bits - min) << 32 | idx, sorted on the depth bits.
-        // LSD passes keep equal depths in emission order; the runs of equal depth (rare among
-        // visible Gaussians) are then put in index order by odd-even transposition. Tiles with
-        // more than 64 tied neighbours (e.g. many culled zero records) take the full sort of
-        // the unique key instead.
-        for (uint32_t j = tid; j < n; j += NT) {
-            const uint64_t p = keysA[j];
-            keysA[j] = ((uint64_t)((uint32_t)(p >> 32) - mn) << 32) | (p & 0xffffffffull);
+        // key = (depth bits - min) << 32 | idx. The LSD passes sort the top (at most
+        // kPrefixBits) bits of the depth span and keep equal prefixes in emission order; each run
+        // of equal prefix (a few entries: depths within 2^drop of each other, or equal) is then
+        // put in (depth, index) order by the thread holding its first entry. A tile with a run
+        // longer than kMaxRun (many equal depths) takes the full sort of the unique key instead.
+        constexpr int kPrefixBits = 24;
+        constexpr uint32_t kMaxRun = 16;
+        const int drop = span_bits > kPrefixBits ? span_bits - kPrefixBits : 0;
+        const int dpasses = (span_bits - drop + 7) / 8;
+        const int pshift = 32 + drop;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t j = tid + (uint32_t)(e * NT);
+            if (j < n)
+                keysA[j] = ((uint64_t)((uint32_t)(held[e] >> 32) - mn) << 32) |
+                           (held[e] & 0xffffffffull);
         }
         if (tid == 0) misc[63] = 0;
         __syncthreads();
         const int ipt = (int)((n + NT - 1) / NT);
-        const int dpasses = ((span ? 32 - __clz(span) : 0) + 7) / 8;
         uint64_t* src = keysA;
         uint64_t* dst = keysB;
         for (int p = 0; p < dpasses; ++p) {
-            radix_pass<NT, E>(src, (int)n, ipt, 32 + 8 * p, whist, dtot, dbase, misc, dst, nullptr,
-                              n);
+            radix_pass<NT, E>(src, (int)n, ipt, pshift + 8 * p, whist, dtot, dbase, misc, dst,
+                              nullptr, n);
             uint64_t* t = src; src = dst; dst = t;
         }
-        uint32_t my_ties = 0;
-        for (uint32_t j = tid; j + 1 < n; j += NT) my_ties += (src[j] >> 32) == (src[j + 1] >> 32);
-        if (my_ties) atomicAdd(&misc[63], my_ties);
+        // run starts (read-only here; the runs are reordered after the barrier)
+        uint32_t run_end[E];
+        uint32_t long_run = 0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            run_end[e] = 0;
+            const uint32_t j = tid + (uint32_t)(e * NT);
+            if (j + 1 < n) {
+                const uint64_t pj = src[j] >> pshift;
+                if ((src[j + 1] >> pshift) == pj && (j == 0 || (src[j - 1] >> pshift) != pj)) {
+                    uint32_t q = j + 2;
+                    while (q < n && q - j <= kMaxRun && (src[q] >> pshift) == pj) ++q;
+                    if (q - j > kMaxRun) long_run = 1;
+                    else run_end[e] = q;
+                }
+            }
+        }
+        if (long_run) misc[63] = 1;
         __syncthreads();
-        const uint32_t ties = misc[63];
-        __syncthreads();
-        if (ties > 64u) {
+        if (misc[63]) {
             for (uint32_t j = tid; j < n; j += NT) keysA[j] = xform(pairs[s + j]);
             __syncthreads();
             src = keysA;
@@ -695,21 +744,21 @@ __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ r
             }
             return;
         }
-        if (ties > 0u) {
-            int quiet = 0;
-            for (int round = 0; quiet < 2; ++round) {
-                bool swapped = false;
-                for (uint32_t j = 2 * tid + (round & 1); j + 1 < n; j += 2 * NT) {
-                    const uint64_t a = src[j], b = src[j + 1];
-                    if ((a >> 32) == (b >> 32) && (uint32_t)a > (uint32_t)b) {
-                        src[j] = b;
-                        src[j + 1] = a;
-                        swapped = true;
-                    }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (run_end[e] == 0) continue;
+            const uint32_t j = tid + (uint32_t)(e * NT);
+            for (uint32_t a = j + 1; a < run_end[e]; ++a) {   // insertion sort, run <= kMaxRun
+                const uint64_t x = src[a];
+                uint32_t q = a;
+                while (q > j && src[q - 1] > x) {
+                    src[q] = src[q - 1];
+                    --q;
                 }
-                quiet = __syncthreads_or(swapped) ? 0 : quiet + 1;
+                src[q] = x;
             }
         }
+        __syncthreads();
         for (uint32_t j = tid; j < n; j += NT) {
             const uint64_t key = src[j];
             sorted_idx[s + j] = (uint32_t)key;

@@ -82,6 +82,34 @@ def test_binning_bitexact_with_equal_depths(ctx, dup_frac):
     assert ties > 0
 
 
+def test_binning_bitexact_near_equal_and_long_equal_runs(ctx):
+    """LDS-resident lists sort the top 24 bits of the tile's depth span and then order each run
+    of equal prefix by (depth, index) in place; runs longer than 16 send the tile to the full
+    unique-key sort. Copies of Gaussians nudged by a few ulps along the view axis give keys
+    that differ only below the prefix, and 40 exact copies of one Gaussian a run of 40."""
+    scene, intr, pose = workload(50000, seed=21)
+    rng = np.random.default_rng(21)
+    dst = rng.choice(50000, 2041, replace=False)
+    src = rng.choice(50000, 80, replace=False)
+    for k in scene:
+        scene[k][dst[:40]] = scene[k][src[0]]              # 40 exact copies
+    for i, g in enumerate(dst[40:]):                          # 25 near copies of each of 80
+        o = src[1 + i % 79]
+        for k in scene:
+            scene[k][g] = scene[k][o]
+        scene["pos"][g, 2] = np.nextafter(scene["pos"][g, 2], np.float32(np.inf) if i % 2 else
+                                          np.float32(-np.inf), dtype=np.float32)
+        scene["pos"][g, 2] += np.float32(1e-7) * np.float32(i % 5)
+    _, b, _ = _check_binning(ctx, scene, pose, intr, W, H)
+    k = b["keys"].astype(np.int64)
+    same_tile = (k[1:] >> 32) == (k[:-1] >> 32)
+    d = (k[1:] & 0xffffffff) - (k[:-1] & 0xffffffff)
+    near = np.count_nonzero(same_tile & (d > 0) & (d < 256))
+    equal = np.count_nonzero(same_tile & (d == 0))
+    print(f"[binning] adjacent keys: {near} within 256 ulps, {equal} equal", flush=True)
+    assert near > 0 and equal >= 39
+
+
 def test_binning_long_tile_lists(ctx):
     # Large Gaussians on a 128x128 view: lists in every sort configuration — up to 2048
     # (256 threads), 2049-4096 (512), 4097-8192 (1024, LDS-resident) and past 8192 (streamed
