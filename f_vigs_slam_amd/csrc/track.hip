@@ -36,9 +36,10 @@ __device__ __forceinline__ void qmat_t(float x, float y, float z, float w, float
 // whole view, so the blocks factor as
 //   JtJ = [[R S_H R^T, R S_HK], [., S_KHK]],  Jtr = [R S_b ; S_Kb]
 // with S_H = sum H, S_HK = sum d H K^T, S_KHK = sum d^2 K H K^T, S_b = sum b, S_Kb = sum d K b and
-// K = [ray]x (ray z = 1). Each term is formed in FP32 from the reference's H and b; the 27 running
-// sums are FP64 (JtJ adds terms spanning ~10 orders of magnitude, FP32 sums lose positive
-// definiteness); R is applied once per tile, in FP64.
+// K = [ray]x (ray z = 1). Each term is formed in FP32 from the reference's H and b and summed per
+// pixel in FP32, as the reference does; from the wave reduction on the sums are FP64 (JtJ adds
+// terms spanning ~10 orders of magnitude over a view, and FP32 sums across pixels and tiles lose
+// positive definiteness); R is applied once per tile, in FP64.
 // raw layout: S_H (00 01 02 11 12 22) | S_HK (3x3 row-major) | S_KHK (00 01 02 11 12 22) | S_b | S_Kb
 template <typename Add>
 __device__ __forceinline__ void accumulate_raw(Add&& add, const float H[9], const float b3[3],
@@ -67,6 +68,28 @@ __device__ __forceinline__ void accumulate_raw(Add&& add, const float H[9], cons
     add(24, d * (ry * b3[2] - b3[1]));
     add(25, d * (b3[0] - rx * b3[2]));
     add(26, d * (rx * b3[1] - ry * b3[0]));
+}
+
+// One rank-1 term of the raw sums: H = sigma u u^T and b = beta u (a Gaussian's colour term is
+// wc |a da|^2 jd jd^T and -wc (a da . e) jd, its depth-crossing term wd rg rg^T and wd de rg).
+// With w = d K u: S_H += sigma u u^T, S_HK += sigma u w^T, S_KHK += sigma w w^T, S_b += beta u,
+// S_Kb += beta w — accumulate_raw's sums for this H and b, without forming the 3x3 products.
+template <typename Add>
+__device__ __forceinline__ void accumulate_rank1(Add&& add, const float u[3], float sigma,
+                                                 float beta, float d, float rx, float ry) {
+    const float w0 = d * (ry * u[2] - u[1]), w1 = d * (u[0] - rx * u[2]),
+                w2 = d * (rx * u[1] - ry * u[0]);
+    const float su0 = sigma * u[0], su1 = sigma * u[1], su2 = sigma * u[2];
+    add(0, su0 * u[0]); add(1, su0 * u[1]); add(2, su0 * u[2]);
+    add(3, su1 * u[1]); add(4, su1 * u[2]); add(5, su2 * u[2]);
+    add(6, su0 * w0); add(7, su0 * w1); add(8, su0 * w2);
+    add(9, su1 * w0); add(10, su1 * w1); add(11, su1 * w2);
+    add(12, su2 * w0); add(13, su2 * w1); add(14, su2 * w2);
+    const float sw0 = sigma * w0, sw1 = sigma * w1, sw2 = sigma * w2;
+    add(15, sw0 * w0); add(16, sw0 * w1); add(17, sw0 * w2);
+    add(18, sw1 * w1); add(19, sw1 * w2); add(20, sw2 * w2);
+    add(21, beta * u[0]); add(22, beta * u[1]); add(23, beta * u[2]);
+    add(24, beta * w0); add(25, beta * w1); add(26, beta * w2);
 }
 
 // Compensated FP32 arithmetic of the precision form: a value is an unevaluated pair hi + lo.
@@ -160,9 +183,13 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
     auto stage = [&](uint32_t base, int pass) -> int {
         const int cnt = (int)min(256u, n - base);
         if (tid < cnt) {
-            const Rec r = rec[idx[s + base + tid]];
-            recs[tid] = r;
+            Rec r = rec[idx[s + base + tid]];
             qmask[tid] = (uint8_t)quadrant_mask(r, tx0, ty0, pass == 1);
+            if (pass == 2) {   // the ray of the mean, once per record instead of per pixel
+                r.r3.x = (r.r0.x - v.cx) / v.fx;
+                r.r3.y = (r.r0.y - v.cy) / v.fy;
+            }
+            recs[tid] = r;
         }
         __syncthreads();
         int wc = 0;
@@ -248,10 +275,13 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
     }
     inside = inside && (final_T < bp.alpha_thresh);
 
-    double acc[kKahan ? 1 : 27];
+    // Plain form: FP32 running sums per pixel (as the reference's local_JtJ / local_Jtr,
+    // GSCudaKernels.cu:959-960), FP64 from the wave reduction on. Precision form: Kahan-
+    // compensated FP32 per pixel and a compensated FP32 tile reduction.
+    float acc[kKahan ? 1 : 27];
     float ks[kKahan ? 27 : 1], kc[kKahan ? 27 : 1];   // Kahan sum and compensation
 #pragma unroll
-    for (int k = 0; k < (kKahan ? 1 : 27); ++k) acc[k] = 0.0;
+    for (int k = 0; k < (kKahan ? 1 : 27); ++k) acc[k] = 0.f;
 #pragma unroll
     for (int k = 0; k < (kKahan ? 27 : 1); ++k) ks[k] = kc[k] = 0.f;
     auto add = [&](int k, float x) {
@@ -261,7 +291,7 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
             kc[k] = (t - ks[k]) - y;
             ks[k] = t;
         } else {
-            acc[k] += (double)x;
+            acc[k] += x;
         }
     };
     float e0 = 0.f, e1 = 0.f, e2 = 0.f, de = 0.f, wc_col = 0.f;
@@ -318,6 +348,7 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
                 const int i = wlist[w][k];
                 if (base + (uint32_t)i >= n_contrib) break;
                 const float4 r0 = recs[i].r0, r1 = recs[i].r1, r2 = recs[i].r2;
+                const float2 rg = make_float2(recs[i].r3.x, recs[i].r3.y);   // ray of the mean
                 const float dx = r0.x - px, dy = r0.y - py;
                 const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;
                 const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));   // Q8: no q <= 0 test
@@ -331,28 +362,16 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
                 da1 -= (C1 - a1) * iom;
                 da2 -= (C2 - a2) * iom;
                 const float dm0 = r0.z * dx + r0.w * dy, dm1 = r0.w * dx + r1.x * dy;
-                const float rgx = (r0.x - v.cx) / v.fx, rgy = (r0.y - v.cy) / v.fy;
                 const float id_ = 1.0f / d;
-                const float Jt0 = v.fx * id_, Jt3 = v.fy * id_, Jt4 = -v.fx * rgx * id_,
-                            Jt5 = -v.fy * rgy * id_;
+                const float Jt0 = v.fx * id_, Jt3 = v.fy * id_, Jt4 = -v.fx * rg.x * id_,
+                            Jt5 = -v.fy * rg.y * id_;
+                // Jt_cam = jd (a da)^T is rank one (GSCudaKernels.cu:1241-1272): H = wc |a da|^2 jd jd^T,
+                // b = -wc (a da . e) jd
                 const float jd[3] = {Jt0 * dm0, Jt3 * dm1, Jt4 * dm0 + Jt5 * dm1};
-                float Jc[9];
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    Jc[r * 3] = jd[r] * a * da0;
-                    Jc[r * 3 + 1] = jd[r] * a * da1;
-                    Jc[r * 3 + 2] = jd[r] * a * da2;
-                }
-                float H[9], b3[3];
-#pragma unroll
-                for (int r = 0; r < 3; ++r)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c)
-                        H[r * 3 + c] = wc_col * (Jc[r * 3] * Jc[c * 3] + Jc[r * 3 + 1] * Jc[c * 3 + 1] +
-                                                 Jc[r * 3 + 2] * Jc[c * 3 + 2]);
-#pragma unroll
-                for (int r = 0; r < 3; ++r)
-                    b3[r] = -wc_col * (Jc[r * 3] * e0 + Jc[r * 3 + 1] * e1 + Jc[r * 3 + 2] * e2);
+                const float g0 = a * da0, g1 = a * da1, g2 = a * da2;
+                const float sigma = wc_col * (g0 * g0 + g1 * g1 + g2 * g2);
+                const float beta = -wc_col * (g0 * e0 + g1 * e1 + g2 * e2);
+                accumulate_rank1(add, jd, sigma, beta, d, rg.x, rg.y);
                 prod *= (1.0f - a);
                 if (T2 > 0.5f && prod <= 0.5f && img_depth > 0.5f) {
                     T2 = prod;
@@ -360,15 +379,9 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
                     float wd = (ld < bp.depth_thresh) ? 1.0f : bp.depth_thresh / ld;
                     wd /= img_depth;
                     wd *= bp.depth_weight;
-                    const float rg[3] = {rgx, rgy, 1.0f};
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) H[r * 3 + c] += wd * rg[r] * rg[c];
-                        b3[r] += wd * de * rg[r];
-                    }
+                    const float rv[3] = {rg.x, rg.y, 1.0f};
+                    accumulate_rank1(add, rv, wd, wd * de, d, rg.x, rg.y);
                 }
-                accumulate_raw(add, H, b3, d, rgx, rgy);
                 if (prod < 0.001f) { active = false; break; }
             }
         }
@@ -402,7 +415,7 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
     } else {
 #pragma unroll
         for (int k = 0; k < 27; ++k) {
-            double vk = acc[k];
+            double vk = (double)acc[k];
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) vk += __shfl_xor(vk, o);
             if (lane == 0) red[w][k] = vk;
