@@ -463,7 +463,7 @@ __global__ __launch_bounds__(kBinThreads) void k_emit(
 // offset by a per-wave digit counter in LDS; per-digit offsets across waves come from a prefix
 // over the per-wave counters. Elements are laid out w*64*ipt + e*64 + lane with ipt =
 // ceil(n / NT) items per thread, so every wave works on valid data.
-// Lists of up to CHUNK keys stay in LDS (ping-pong); longer lists run each pass as a histogram
+// Lists of up to CHUNK keys stay in LDS (one buffer); longer lists run each pass as a histogram
 // sweep plus chunk-by-chunk stable scatter through HBM (pairs <-> tmp).
 // ------------------------------------------------------------------------------------------
 
@@ -530,16 +530,29 @@ __device__ __forceinline__ void scan256(uint32_t* a, uint32_t* wsum) {
     __syncthreads();
 }
 
+// Items per thread of the 512- and 1024-thread sort configurations: LDS-resident lists of up to
+// 512 kSortMedE and 1024 kSortLargeE keys.
+#ifndef FVGS_SORT_MED_E
+#define FVGS_SORT_MED_E 10
+#endif
+#ifndef FVGS_SORT_LARGE_E
+#define FVGS_SORT_LARGE_E 7
+#endif
+constexpr int kSortMedE = FVGS_SORT_MED_E;
+constexpr int kSortLargeE = FVGS_SORT_LARGE_E;
+
 template <int NT, int E>
 struct SortCfg {
     static constexpr int NW = NT / 64;
     static constexpr int CHUNK = NT * E;
-    // whist[NW*256] | dtot[256] | dbase[256] | misc[64] | keysA[CHUNK] | keysB[CHUNK]
+    // whist[NW*256] | dtot[256] | dbase[256] | misc[64] | keys[CHUNK]
     static constexpr size_t kHeader = (size_t)(NW * 256 + 256 + 256 + 64) * 4;
-    static constexpr size_t kLds = kHeader + (size_t)CHUNK * 16;
+    static constexpr size_t kLds = kHeader + (size_t)CHUNK * 8;
 };
 
-// One stable counting pass over cnt keys held in LDS (src). Destinations: the LDS buffer dst_lds
+// One stable counting pass over cnt keys held in LDS (src). Each thread keeps its keys in
+// registers from the ranking to the scatter, with barriers in between, so the LDS destination
+// may be src itself (one key buffer per workgroup). Destinations: the LDS buffer dst_lds
 // (resident: bases = this chunk's own digit scan) or HBM dst_glob (streaming: bases = running
 // per-digit offsets in dbase, advanced by this chunk's digit totals).
 template <int NT, int E>
@@ -552,13 +565,16 @@ __device__ __forceinline__ void radix_pass(const uint64_t* src, int cnt, int ipt
     for (int i = tid; i < NW * 256; i += NT) whist[i] = 0;
     __syncthreads();
     uint32_t rk[E];
+    uint64_t kv[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         rk[e] = 0;
+        kv[e] = 0;
         if (e < ipt) {
             const int j = w * 64 * ipt + e * 64 + lane;
             const bool valid = j < cnt;
-            const uint32_t d = valid ? (uint32_t)(src[j] >> shift) & 255u : 0u;
+            if (valid) kv[e] = src[j];
+            const uint32_t d = valid ? (uint32_t)(kv[e] >> shift) & 255u : 0u;
             const uint64_t peers = match8(d, __ballot(valid));
             const uint32_t lr = lanes_below(peers);
             uint32_t base = 0;
@@ -592,7 +608,7 @@ __device__ __forceinline__ void radix_pass(const uint64_t* src, int cnt, int ipt
         if (e < ipt) {
             const int j = w * 64 * ipt + e * 64 + lane;
             if (j < cnt) {
-                const uint64_t key = src[j];
+                const uint64_t key = kv[e];
                 const uint32_t d = (uint32_t)(key >> shift) & 255u;
                 const uint32_t pos = whist[w * 256 + d] + rk[e];
                 if (pos < limit) {   // always true for a consistent ranking
@@ -624,7 +640,6 @@ __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ r
     uint32_t* dbase = dtot + 256;
     uint32_t* misc = dbase + 256;
     uint64_t* keysA = reinterpret_cast<uint64_t*>(smem + C::kHeader);
-    uint64_t* keysB = keysA + CHUNK;
     const int tile = blockIdx.x, tid = threadIdx.x;
     const uint32_t s0 = ranges[2 * tile], n0 = ranges[2 * tile + 1] - s0;
     // Tile 0 starts with the zero segment (culled instances, key 0, already in index order);
@@ -701,12 +716,9 @@ __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ r
         __syncthreads();
         const int ipt = (int)((n + NT - 1) / NT);
         uint64_t* src = keysA;
-        uint64_t* dst = keysB;
-        for (int p = 0; p < dpasses; ++p) {
-            radix_pass<NT, E>(src, (int)n, ipt, pshift + 8 * p, whist, dtot, dbase, misc, dst,
+        for (int p = 0; p < dpasses; ++p)
+            radix_pass<NT, E>(src, (int)n, ipt, pshift + 8 * p, whist, dtot, dbase, misc, src,
                               nullptr, n);
-            uint64_t* t = src; src = dst; dst = t;
-        }
         // run starts (read-only here; the runs are reordered after the barrier)
         uint32_t run_end[E];
         uint32_t long_run = 0;
@@ -729,13 +741,9 @@ __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ r
         if (misc[63]) {
             for (uint32_t j = tid; j < n; j += NT) keysA[j] = xform(pairs[s + j]);
             __syncthreads();
-            src = keysA;
-            dst = keysB;
-            for (int p = 0; p < passes; ++p) {
-                radix_pass<NT, E>(src, (int)n, ipt, 8 * p, whist, dtot, dbase, misc, dst, nullptr,
+            for (int p = 0; p < passes; ++p)
+                radix_pass<NT, E>(src, (int)n, ipt, 8 * p, whist, dtot, dbase, misc, src, nullptr,
                                   n);
-                uint64_t* t = src; src = dst; dst = t;
-            }
             for (uint32_t j = tid; j < n; j += NT) {
                 const uint64_t key = src[j];
                 sorted_idx[s + j] = (uint32_t)(key & idx_mask);
@@ -914,16 +922,16 @@ hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uin
                             const uint32_t* zinfo, size_t lds_limit, hipStream_t s) {
     int idx_bits = 1;
     while (idx_bits < 32 && ((uint64_t)1 << idx_bits) < (uint64_t)n_gauss) ++idx_bits;
-    // Lists up to 2048 keys: 256 threads (~36 KB LDS, several workgroups per CU); up to 4096:
-    // 512 threads (~76 KB, two per CU); longer: 1024 threads, 8192 keys resident (~146 KB),
-    // streaming beyond that. Each launch covers every tile and skips the other lengths.
+    // Lists up to 2048 keys: 256 threads (22 KB LDS, seven workgroups per CU); up to kMedE * 512:
+    // 512 threads; longer: 1024 threads, kLargeE * 1024 keys resident, streaming beyond that. Each
+    // launch covers every tile and skips the other lengths.
     using Small = SortCfg<256, 8>;
-    using Medium = SortCfg<512, 8>;
-    using Large = SortCfg<1024, 8>;
+    using Medium = SortCfg<512, kSortMedE>;
+    using Large = SortCfg<1024, kSortLargeE>;
     static const bool attrs = [] {
-        (void)hipFuncSetAttribute((const void*)k_tile_sort<512, 8>,
+        (void)hipFuncSetAttribute((const void*)k_tile_sort<512, kSortMedE>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)Medium::kLds);
-        (void)hipFuncSetAttribute((const void*)k_tile_sort<1024, 8>,
+        (void)hipFuncSetAttribute((const void*)k_tile_sort<1024, kSortLargeE>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)Large::kLds);
         return true;
     }();
@@ -934,13 +942,13 @@ hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uin
                        idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo, 0u, small_cap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_tile_sort<512, 8>), dim3(ntiles), dim3(512), Medium::kLds, s, ranges,
-                       idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo, small_cap,
+    hipLaunchKernelGGL((k_tile_sort<512, kSortMedE>), dim3(ntiles), dim3(512), Medium::kLds, s,
+                       ranges, idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo, small_cap,
                        medium_cap);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_tile_sort<1024, 8>), dim3(ntiles), dim3(1024), Large::kLds, s, ranges,
-                       idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo, medium_cap,
+    hipLaunchKernelGGL((k_tile_sort<1024, kSortLargeE>), dim3(ntiles), dim3(1024), Large::kLds, s,
+                       ranges, idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo, medium_cap,
                        0xffffffffu);
     return hipGetLastError();
 }
