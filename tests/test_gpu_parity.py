@@ -66,8 +66,9 @@ def test_binning_bitexact_perturbed_with_culls(ctx):
 
 @pytest.mark.parametrize("dup_frac", [0.05, 0.4])
 def test_binning_bitexact_with_equal_depths(ctx, dup_frac):
-    """Duplicated Gaussians give equal depth keys inside a tile: few ties take the depth-only
-    sort plus index ordering of equal runs, many ties (> 64 in a tile) the full-key sort."""
+    """Duplicated Gaussians give equal depth keys inside a tile: short runs of equal keys are put
+    in index order in place after the depth passes, runs longer than 16 send the tile to the
+    full-key sort."""
     scene, intr, pose = workload(50000, seed=8)
     rng = np.random.default_rng(8)
     dst = rng.choice(50000, int(dup_frac * 50000), replace=False)
