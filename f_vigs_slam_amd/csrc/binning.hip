@@ -13,9 +13,9 @@
 //   k_colscan_*  column prefix of the count matrix (64 columns per workgroup, waves over row
 //                runs), then the tile totals' exclusive scan: tile ranges, the instance count
 //                M and each column's base offset.
-//   k_emit       per block, LDS cursors at the block's base offset in every tile; each instance
-//                writes (depth bits << 32 | gaussian index) into its tile's segment, through
-//                the same even walk.
+//   k_emit       per block of kEmitRows projection blocks, LDS cursors at the block's base offset
+//                in every tile; each instance writes (depth bits << 32 | gaussian index) into its
+//                tile's segment, through the same even walk.
 //   k_tile_sort  one workgroup per tile: LSD radix sort by (depth bits, index), 8-bit
 //                digits, stable wave-level ranks from ballots. Segments up to NT*E entries
 //                stay in LDS and sort on the top 24 bits of the tile's depth span, then put
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(kBinThreads) void k_project(
     float Rc[3][3];
     quat_to_mat_exact(v.qx, v.qy, v.qz, v.qw, Rc);
     const int lane = threadIdx.x & 63;
-    for (int it = 0; it < kBinItems; ++it) {
+    for (int it = 0; it < kProjItems; ++it) {
         const int g = blockIdx.x * kBinChunk + it * kBinThreads + threadIdx.x;
         const uint2 rc =
             g < n ? project_store(map, g, v, Rc, cull_emit, rec, rect, zinfo) : make_uint2(1u, 0u);
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(1024) void k_colscan_rows(uint32_t* __restrict__ bl
     const int r0 = min(nblk, w * per), r1 = min(nblk, r0 + per);
     uint32_t sum = 0;
     if (on) {
-#pragma unroll 8
+#pragma unroll 16
         for (int r = r0; r < r1; ++r) sum += blk_counts[(size_t)r * cols + col];
     }
     wsum[w][lane] = sum;
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(1024) void k_colscan_rows(uint32_t* __restrict__ bl
         all += x;
     }
     if (!on) return;
-#pragma unroll 8
+#pragma unroll 16
     for (int r = r0; r < r1; ++r) {
         uint32_t* p = blk_counts + (size_t)r * cols + col;
         const uint32_t x = *p;
@@ -401,12 +401,13 @@ __global__ __launch_bounds__(kBinThreads) void k_emit(
     __shared__ uint32_t wz[kBinThreads / 64];
     const View v = vd ? *vd : vh;
     const int T = v.ntx * v.nty;
-    const uint32_t* row = blk_counts + (size_t)blockIdx.x * (T + 1);
+    // the first of this block's kEmitRows count-matrix rows holds the column prefix at the block
+    const uint32_t* row = blk_counts + (size_t)blockIdx.x * kEmitRows * (T + 1);
     for (int t = threadIdx.x; t <= T; t += kBinThreads) cur[t] = col_base[t] + row[t];
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int it = 0; it < kBinItems; ++it) {
-        const int g = blockIdx.x * kBinChunk + it * kBinThreads + threadIdx.x;
+    for (int it = 0; it < kEmitChunk / kBinThreads; ++it) {
+        const int g = blockIdx.x * kEmitChunk + it * kBinThreads + threadIdx.x;
         const bool valid = g < n;
         const uint2 rc = valid ? rect[g] : make_uint2(1u, 0u);
         // Zero segment: culled instances in Gaussian order (a block-wide ballot prefix), so
@@ -910,7 +911,7 @@ hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* col_base, int nblk, in
 hipError_t launch_emit(const Rec* rec, const uint2* rect, const uint32_t* blk_counts,
                        const uint32_t* col_base, int n, const View& v, const View* vd,
                        uint64_t* pairs, uint64_t cap, hipStream_t s) {
-    int nblk = (n + kBinChunk - 1) / kBinChunk;
+    int nblk = (n + kEmitChunk - 1) / kEmitChunk;
     size_t lds = ((size_t)v.ntx * v.nty + 1) * 4;
     hipLaunchKernelGGL(k_emit, dim3(nblk), dim3(kBinThreads), lds, s, rec, rect, blk_counts,
                        col_base, n, v, vd, pairs, cap);

@@ -13,8 +13,13 @@ constexpr int kBucket = 32;        // backward bucket length (GSCudaKernels.cu:2
 constexpr int kMaxBuckets = 8;     // only the first 256 entries get gradients (Q10)
 constexpr int kMaxTiles = 8192;    // LDS tile histogram of the binning kernels (32 KB)
 constexpr int kBinThreads = 1024;  // binning block
-constexpr int kBinItems = 4;       // Gaussians per thread per binning block
-constexpr int kBinChunk = kBinThreads * kBinItems;
+#ifndef FVGS_PROJ_ITEMS
+#define FVGS_PROJ_ITEMS 4
+#endif
+constexpr int kProjItems = FVGS_PROJ_ITEMS;         // Gaussians per thread of a projection block
+constexpr int kBinChunk = kBinThreads * kProjItems;  // per projection block = count-matrix row
+constexpr int kEmitRows = 4 / kProjItems;            // count-matrix rows per emit block
+constexpr int kEmitChunk = kBinChunk * kEmitRows;    // 4096 Gaussians per emit block
 
 // View description passed by value to kernels.
 struct View {
