@@ -537,7 +537,7 @@ __device__ __forceinline__ void scan256(uint32_t* a, uint32_t* wsum) {
 #define FVGS_SORT_MED_E 10
 #endif
 #ifndef FVGS_SORT_LARGE_E
-#define FVGS_SORT_LARGE_E 7
+#define FVGS_SORT_LARGE_E 10
 #endif
 constexpr int kSortMedE = FVGS_SORT_MED_E;
 constexpr int kSortLargeE = FVGS_SORT_LARGE_E;
@@ -557,7 +557,7 @@ struct SortCfg {
 // (resident: bases = this chunk's own digit scan) or HBM dst_glob (streaming: bases = running
 // per-digit offsets in dbase, advanced by this chunk's digit totals).
 template <int NT, int E>
-__device__ __forceinline__ void radix_pass(const uint64_t* src, int cnt, int ipt, int shift,
+__device__ __attribute__((noinline)) void radix_pass(const uint64_t* src, int cnt, int ipt, int shift,
                                            uint32_t* whist, uint32_t* dtot, uint32_t* dbase,
                                            uint32_t* misc, uint64_t* dst_lds,
                                            uint64_t* dst_glob, uint32_t limit) {
@@ -624,6 +624,54 @@ __device__ __forceinline__ void radix_pass(const uint64_t* src, int cnt, int ipt
     __syncthreads();
 }
 
+// Lists longer than an LDS chunk: each pass of the unique key ((depth bits - tile min) <<
+// idx_bits | idx) is a global digit histogram, then the chunks in order with a stable scatter
+// through HBM (pairs <-> tmp).
+template <int NT, int E>
+__device__ void tile_sort_stream(uint64_t* __restrict__ pairs, uint64_t* __restrict__ tmpbuf,
+                                 uint32_t s, uint32_t n, uint32_t mn, int idx_bits, int passes,
+                                 uint32_t* whist, uint32_t* dtot, uint32_t* dbase, uint32_t* misc,
+                                 uint64_t* keysA, uint32_t* __restrict__ sorted_idx,
+                                 uint64_t* __restrict__ keys_out, int tile) {
+    constexpr int CHUNK = NT * E;
+    const int tid = threadIdx.x;
+    const uint64_t idx_mask = (1ull << idx_bits) - 1ull;
+    auto xform = [&](uint64_t p) -> uint64_t {
+        return ((uint64_t)((uint32_t)(p >> 32) - mn) << idx_bits) | (p & 0xffffffffull);
+    };
+    uint64_t* gsrc = pairs + s;
+    uint64_t* gdst = tmpbuf + s;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        if (tid < 256) dbase[tid] = 0;
+        __syncthreads();
+        for (uint32_t j = tid; j < n; j += NT) {
+            const uint64_t key = (p == 0) ? xform(gsrc[j]) : gsrc[j];
+            atomicAdd(&dbase[(uint32_t)(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        scan256(dbase, misc);
+        for (uint32_t c0 = 0; c0 < n; c0 += CHUNK) {
+            const int cnt = (int)min((uint32_t)CHUNK, n - c0);
+            for (int j = tid; j < cnt; j += NT) {
+                const uint64_t key = gsrc[c0 + j];
+                keysA[j] = (p == 0) ? xform(key) : key;
+            }
+            __syncthreads();
+            radix_pass<NT, E>(keysA, cnt, (cnt + NT - 1) / NT, shift, whist, dtot, dbase, misc,
+                              nullptr, gdst, n);
+        }
+        __threadfence_block();
+        __syncthreads();
+        uint64_t* t = gsrc; gsrc = gdst; gdst = t;
+    }
+    for (uint32_t j = tid; j < n; j += NT) {
+        const uint64_t key = gsrc[j];
+        sorted_idx[s + j] = (uint32_t)(key & idx_mask);
+        if (keys_out) keys_out[s + j] = ((uint64_t)tile << 32) | (uint32_t)((key >> idx_bits) + mn);
+    }
+}
+
 // Sorts the tiles whose list length n satisfies lo < n <= hi.
 template <int NT, int E>
 __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ ranges,
@@ -665,19 +713,16 @@ __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ r
     }
     const bool resident = n <= (uint32_t)CHUNK;
     uint32_t mn = 0xffffffffu, mx = 0u;
-    uint64_t held[E];   // resident lists: this thread's entries j = tid + e * NT
     if (resident) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
+        for (int e = 0; e < E; ++e) {   // unrolled: the list loads are in flight together
             const uint32_t j = tid + (uint32_t)(e * NT);
-            held[e] = j < n ? pairs[s + j] : 0ull;
-        }
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            if (tid + (uint32_t)(e * NT) < n) {
-                const uint32_t d = (uint32_t)(held[e] >> 32);
+            if (j < n) {
+                const uint64_t p = pairs[s + j];
+                const uint32_t d = (uint32_t)(p >> 32);
                 mn = min(mn, d);
                 mx = max(mx, d);
+                keysA[j] = p;
             }
         }
     } else {
@@ -706,38 +751,30 @@ __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ r
         const int drop = span_bits > kPrefixBits ? span_bits - kPrefixBits : 0;
         const int dpasses = (span_bits - drop + 7) / 8;
         const int pshift = 32 + drop;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const uint32_t j = tid + (uint32_t)(e * NT);
-            if (j < n)
-                keysA[j] = ((uint64_t)((uint32_t)(held[e] >> 32) - mn) << 32) |
-                           (held[e] & 0xffffffffull);
+        for (uint32_t j = tid; j < n; j += NT) {   // the entries this thread loaded
+            const uint64_t p = keysA[j];
+            keysA[j] = ((uint64_t)((uint32_t)(p >> 32) - mn) << 32) | (p & 0xffffffffull);
         }
-        if (tid == 0) misc[63] = 0;
+        if (tid == 0) misc[62] = misc[63] = 0;
         __syncthreads();
         const int ipt = (int)((n + NT - 1) / NT);
         uint64_t* src = keysA;
         for (int p = 0; p < dpasses; ++p)
             radix_pass<NT, E>(src, (int)n, ipt, pshift + 8 * p, whist, dtot, dbase, misc, src,
                               nullptr, n);
-        // run starts (read-only here; the runs are reordered after the barrier)
-        uint32_t run_end[E];
-        uint32_t long_run = 0;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            run_end[e] = 0;
-            const uint32_t j = tid + (uint32_t)(e * NT);
-            if (j + 1 < n) {
-                const uint64_t pj = src[j] >> pshift;
-                if ((src[j + 1] >> pshift) == pj && (j == 0 || (src[j - 1] >> pshift) != pj)) {
-                    uint32_t q = j + 2;
-                    while (q < n && q - j <= kMaxRun && (src[q] >> pshift) == pj) ++q;
-                    if (q - j > kMaxRun) long_run = 1;
-                    else run_end[e] = q;
-                }
-            }
+        // Runs of equal prefix, listed in the (now free) digit counters as start | length << 16;
+        // too many runs, or one longer than kMaxRun, take the full sort.
+        uint32_t* runs = whist;
+        const uint32_t max_runs = (uint32_t)(NW * 256);
+        for (uint32_t j = tid; j + 1 < n; j += NT) {
+            const uint64_t pj = src[j] >> pshift;
+            if ((src[j + 1] >> pshift) != pj || (j > 0 && (src[j - 1] >> pshift) == pj)) continue;
+            uint32_t q = j + 2;
+            while (q < n && q - j <= kMaxRun && (src[q] >> pshift) == pj) ++q;
+            const uint32_t r = q - j > kMaxRun ? max_runs : atomicAdd(&misc[62], 1u);
+            if (r < max_runs) runs[r] = j | ((q - j) << 16);
+            else misc[63] = 1;
         }
-        if (long_run) misc[63] = 1;
         __syncthreads();
         if (misc[63]) {
             for (uint32_t j = tid; j < n; j += NT) keysA[j] = xform(pairs[s + j]);
@@ -753,11 +790,10 @@ __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ r
             }
             return;
         }
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            if (run_end[e] == 0) continue;
-            const uint32_t j = tid + (uint32_t)(e * NT);
-            for (uint32_t a = j + 1; a < run_end[e]; ++a) {   // insertion sort, run <= kMaxRun
+        const uint32_t nruns = misc[62];
+        for (uint32_t r = tid; r < nruns; r += NT) {   // insertion sort, run <= kMaxRun
+            const uint32_t j = runs[r] & 0xffffu, end = j + (runs[r] >> 16);
+            for (uint32_t a = j + 1; a < end; ++a) {
                 const uint64_t x = src[a];
                 uint32_t q = a;
                 while (q > j && src[q - 1] > x) {
@@ -775,37 +811,227 @@ __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ r
         }
         return;
     }
-    // Streaming: each pass = global digit histogram, then chunks in order, stable scatter.
-    uint64_t* gsrc = pairs + s;
-    uint64_t* gdst = tmpbuf + s;
-    for (int p = 0; p < passes; ++p) {
-        const int shift = 8 * p;
-        if (tid < 256) dbase[tid] = 0;
-        __syncthreads();
-        for (uint32_t j = tid; j < n; j += NT) {
-            const uint64_t key = (p == 0) ? xform(gsrc[j]) : gsrc[j];
-            atomicAdd(&dbase[(uint32_t)(key >> shift) & 255u], 1u);
-        }
-        __syncthreads();
-        scan256(dbase, misc);
-        for (uint32_t c0 = 0; c0 < n; c0 += CHUNK) {
-            const int cnt = (int)min((uint32_t)CHUNK, n - c0);
-            for (int j = tid; j < cnt; j += NT) {
-                const uint64_t key = gsrc[c0 + j];
-                keysA[j] = (p == 0) ? xform(key) : key;
+    tile_sort_stream<NT, E>(pairs, tmpbuf, s, n, mn, idx_bits, passes, whist, dtot, dbase, misc,
+                            keysA, sorted_idx, keys_out, tile);
+}
+
+// ------------------------------------------------------------------------------------------
+// LDS-resident sort with 6 bytes per entry: the key is the 32-bit (depth bits - tile min) and the
+// payload the entry's 16-bit position in the tile's list; the Gaussian index is read back from
+// the list (L2) where it is needed (runs of equal prefix, the fallback, the output). 1024 threads
+// x 10 entries take 77 KB of LDS, so two workgroups share a CU (8-byte keys: 98 KB, one).
+// ------------------------------------------------------------------------------------------
+template <int NT, int E>
+struct SortKpCfg {
+    static constexpr int NW = NT / 64;
+    static constexpr int CHUNK = NT * E;
+    // whist[NW*256] | dbase[256] | misc[64] | keys[CHUNK] (u32) | pos[CHUNK] (u16)
+    static constexpr size_t kHeader = (size_t)(NW * 256 + 256 + 64) * 4;
+    static constexpr size_t kLds = kHeader + (size_t)CHUNK * 6;
+    // items per thread of the 64-bit chunk that the streamed form fits in the same LDS
+    static constexpr int kStreamE =
+        (int)((kLds - (size_t)(NW * 256 + 256 + 256 + 64) * 4) / 8 / NT);
+};
+
+// One stable counting pass of radix_pass's resident form over (keys, pos) in place.
+template <int NT, int E>
+__device__ __attribute__((noinline)) void radix_pass_kp(uint32_t* keys, uint16_t* pos, int cnt, int ipt,
+                                              int shift, uint32_t* whist, uint32_t* dbase,
+                                              uint32_t* misc) {
+    constexpr int NW = NT / 64;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    for (int i = tid; i < NW * 256; i += NT) whist[i] = 0;
+    __syncthreads();
+    uint32_t kv[E], rp[E];   // key; rank within the wave's digit | position << 16
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        kv[e] = 0;
+        rp[e] = 0;
+        if (e < ipt) {
+            const int j = w * 64 * ipt + e * 64 + lane;
+            const bool valid = j < cnt;
+            uint32_t pv = 0;
+            if (valid) {
+                kv[e] = keys[j];
+                pv = pos[j];
             }
-            __syncthreads();
-            radix_pass<NT, E>(keysA, cnt, (cnt + NT - 1) / NT, shift, whist, dtot, dbase, misc,
-                              nullptr, gdst, n);
+            const uint32_t d = valid ? (kv[e] >> shift) & 255u : 0u;
+            const uint64_t peers = match8(d, __ballot(valid));
+            const uint32_t lr = lanes_below(peers);
+            uint32_t base = 0;
+            if (valid) base = whist[w * 256 + d];
+            if (valid && lr == 0) whist[w * 256 + d] = base + (uint32_t)__popcll(peers);
+            rp[e] = (base + lr) | (pv << 16);
         }
-        __threadfence_block();
+        // keep the items' ballots apart: hoisted, they exhaust the scalar registers
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+    if (tid < 256) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int ww = 0; ww < NW; ++ww) {
+            const uint32_t h = whist[ww * 256 + tid];
+            whist[ww * 256 + tid] = run;
+            run += h;
+        }
+        dbase[tid] = run;
+    }
+    __syncthreads();
+    scan256(dbase, misc);
+    if (tid < 256) {
+        const uint32_t b = dbase[tid];
+#pragma unroll
+        for (int ww = 0; ww < NW; ++ww) whist[ww * 256 + tid] += b;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        if (e < ipt) {
+            const int j = w * 64 * ipt + e * 64 + lane;
+            if (j < cnt) {
+                const uint32_t p = whist[w * 256 + ((kv[e] >> shift) & 255u)] + (rp[e] & 0xffffu);
+                if (p < (uint32_t)cnt) {   // always true for a consistent ranking
+                    keys[p] = kv[e];
+                    pos[p] = (uint16_t)(rp[e] >> 16);
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Sorts the tiles whose list length n (past tile 0's zero segment) satisfies lo < n <= hi, with
+// lo >= 1 and hi <= NT * E: the order of k_tile_sort's resident form.
+template <int NT, int E>
+__global__ __launch_bounds__(NT) void k_tile_sort_kp(const uint32_t* __restrict__ ranges,
+                                                     int idx_bits, uint64_t* __restrict__ pairs,
+                                                     uint64_t* __restrict__ tmpbuf,
+                                                     uint32_t* __restrict__ sorted_idx,
+                                                     uint64_t* __restrict__ keys_out,
+                                                     const uint32_t* __restrict__ zinfo,
+                                                     uint32_t lo, uint32_t hi) {
+    using C = SortKpCfg<NT, E>;
+    constexpr int NW = C::NW, CHUNK = C::CHUNK;
+    static_assert(CHUNK <= 65536, "16-bit positions");
+    extern __shared__ __align__(16) uint8_t smem[];
+    uint32_t* whist = reinterpret_cast<uint32_t*>(smem);
+    uint32_t* dbase = whist + NW * 256;
+    uint32_t* misc = dbase + 256;
+    uint32_t* keys = reinterpret_cast<uint32_t*>(smem + C::kHeader);
+    uint16_t* pos = reinterpret_cast<uint16_t*>(keys + CHUNK);
+    const int tile = blockIdx.x, tid = threadIdx.x;
+    const uint32_t s0 = ranges[2 * tile], n0 = ranges[2 * tile + 1] - s0;
+    const uint32_t zskip = (tile == 0 && zinfo[1] == 0u) ? min(zinfo[0], n0) : 0u;
+    const uint32_t s = s0 + zskip, n = n0 - zskip;
+    if (!(n > lo && n <= hi)) return;
+    for (uint32_t j = tid; j < zskip; j += NT) {
+        sorted_idx[s0 + j] = (uint32_t)pairs[s0 + j];
+        if (keys_out) keys_out[s0 + j] = (uint64_t)tile << 32;
+    }
+    if (n > (uint32_t)CHUNK) {
+        // longer than the LDS chunk: the streamed sort of k_tile_sort, its 64-bit chunk of
+        // kStreamE * NT keys in the same LDS
+        using S64 = SortCfg<NT, C::kStreamE>;
+        static_assert(S64::kLds <= C::kLds, "streamed chunk fits the workgroup's LDS");
+        uint32_t* sw = reinterpret_cast<uint32_t*>(smem);
+        uint32_t* sdtot = sw + NW * 256;
+        uint32_t* sdbase = sdtot + 256;
+        uint32_t* smisc = sdbase + 256;
+        uint64_t* skeys = reinterpret_cast<uint64_t*>(smem + S64::kHeader);
+        uint32_t mn = 0xffffffffu, mx = 0u;
+        for (uint32_t j = tid; j < n; j += NT) {
+            const uint32_t d = (uint32_t)(pairs[s + j] >> 32);
+            mn = min(mn, d);
+            mx = max(mx, d);
+        }
+        block_minmax<NT>(mn, mx, smisc);
+        const uint32_t span = mx - mn;
+        const int passes = ((span ? 32 - __clz(span) : 0) + idx_bits + 7) / 8;
+        tile_sort_stream<NT, C::kStreamE>(pairs, tmpbuf, s, n, mn, idx_bits, passes, sw, sdtot,
+                                          sdbase, smisc, skeys, sorted_idx, keys_out, tile);
+        return;
+    }
+    uint32_t mn = 0xffffffffu, mx = 0u;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {   // unrolled: the list loads are in flight together
+        const uint32_t j = tid + (uint32_t)(e * NT);
+        if (j < n) {
+            const uint32_t d = (uint32_t)(pairs[s + j] >> 32);
+            mn = min(mn, d);
+            mx = max(mx, d);
+            keys[j] = d;
+            pos[j] = (uint16_t)j;
+        }
+    }
+    block_minmax<NT>(mn, mx, misc);   // its barriers order the key writes before the rewrite
+    const uint32_t span = mx - mn;
+    const int span_bits = span ? 32 - __clz(span) : 0;
+    constexpr int kPrefixBits = 24;
+    constexpr uint32_t kMaxRun = 16;
+    const int drop = span_bits > kPrefixBits ? span_bits - kPrefixBits : 0;
+    const int dpasses = (span_bits - drop + 7) / 8;
+    for (uint32_t j = tid; j < n; j += NT) keys[j] -= mn;
+    if (tid == 0) misc[62] = misc[63] = 0;
+    __syncthreads();
+    const int ipt = (int)((n + NT - 1) / NT);
+    for (int p = 0; p < dpasses; ++p)
+        radix_pass_kp<NT, E>(keys, pos, (int)n, ipt, drop + 8 * p, whist, dbase, misc);
+    // Runs of equal prefix, as in k_tile_sort: listed in the (now free) digit counters as
+    // start | length << 16; too many runs, or one longer than kMaxRun, take the fallback.
+    uint32_t* runs = whist;
+    const uint32_t max_runs = (uint32_t)(NW * 256);
+    for (uint32_t j = tid; j + 1 < n; j += NT) {
+        const uint32_t pj = keys[j] >> drop;
+        if ((keys[j + 1] >> drop) != pj || (j > 0 && (keys[j - 1] >> drop) == pj)) continue;
+        uint32_t q = j + 2;
+        while (q < n && q - j <= kMaxRun && (keys[q] >> drop) == pj) ++q;
+        const uint32_t r = q - j > kMaxRun ? max_runs : atomicAdd(&misc[62], 1u);
+        if (r < max_runs) runs[r] = j | ((q - j) << 16);
+        else misc[63] = 1;
+    }
+    __syncthreads();
+    if (misc[63]) {
+        // (depth, index) by two stable LSD sorts: the index first, then the depth
+        for (uint32_t j = tid; j < n; j += NT) {
+            keys[j] = (uint32_t)pairs[s + j];
+            pos[j] = (uint16_t)j;
+        }
         __syncthreads();
-        uint64_t* t = gsrc; gsrc = gdst; gdst = t;
+        for (int p = 0; p < (idx_bits + 7) / 8; ++p)
+            radix_pass_kp<NT, E>(keys, pos, (int)n, ipt, 8 * p, whist, dbase, misc);
+        for (uint32_t j = tid; j < n; j += NT)
+            keys[j] = (uint32_t)(pairs[s + pos[j]] >> 32) - mn;
+        __syncthreads();
+        for (int p = 0; p < (span_bits + 7) / 8; ++p)
+            radix_pass_kp<NT, E>(keys, pos, (int)n, ipt, 8 * p, whist, dbase, misc);
+    } else {
+        auto full = [&](uint32_t q) -> uint64_t {   // (depth - min, index) of sorted slot q
+            return ((uint64_t)keys[q] << 32) | (uint32_t)pairs[s + pos[q]];
+        };
+        const uint32_t nruns = misc[62];
+        for (uint32_t r = tid; r < nruns; r += NT) {   // insertion sort, run <= kMaxRun
+            const uint32_t j = runs[r] & 0xffffu, end = j + (runs[r] >> 16);
+            for (uint32_t a = j + 1; a < end; ++a) {
+                const uint32_t xk = keys[a];
+                const uint16_t xp = pos[a];
+                const uint64_t xv = full(a);
+                uint32_t q = a;
+                while (q > j && full(q - 1) > xv) {
+                    keys[q] = keys[q - 1];
+                    pos[q] = pos[q - 1];
+                    --q;
+                }
+                keys[q] = xk;
+                pos[q] = xp;
+            }
+        }
+        __syncthreads();
     }
     for (uint32_t j = tid; j < n; j += NT) {
-        const uint64_t key = gsrc[j];
-        sorted_idx[s + j] = (uint32_t)(key & idx_mask);
-        if (keys_out) keys_out[s + j] = ((uint64_t)tile << 32) | (uint32_t)((key >> idx_bits) + mn);
+        const uint64_t pr = pairs[s + pos[j]];
+        sorted_idx[s + j] = (uint32_t)pr;
+        if (keys_out) keys_out[s + j] = ((uint64_t)tile << 32) | (pr >> 32);
     }
 }
 
@@ -923,16 +1149,16 @@ hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uin
                             const uint32_t* zinfo, size_t lds_limit, hipStream_t s) {
     int idx_bits = 1;
     while (idx_bits < 32 && ((uint64_t)1 << idx_bits) < (uint64_t)n_gauss) ++idx_bits;
-    // Lists up to 2048 keys: 256 threads (22 KB LDS, seven workgroups per CU); up to kMedE * 512:
-    // 512 threads; longer: 1024 threads, kLargeE * 1024 keys resident, streaming beyond that. Each
-    // launch covers every tile and skips the other lengths.
+    // Lists up to 2048 keys: 256 threads (22 KB LDS); up to kMedE * 512: 512 threads (50 KB, three
+    // per CU); longer: 1024 threads with 6-byte entries, kLargeE * 1024 resident (77 KB, two per
+    // CU), streamed through HBM beyond that. Each launch covers every tile and skips the other
+    // lengths. (6-byte entries for the 512-thread lists fit four per CU but measured slower:
+    // the index gathers from the list cost more than the extra workgroup gains.)
     using Small = SortCfg<256, 8>;
     using Medium = SortCfg<512, kSortMedE>;
-    using Large = SortCfg<1024, kSortLargeE>;
+    using Large = SortKpCfg<1024, kSortLargeE>;
     static const bool attrs = [] {
-        (void)hipFuncSetAttribute((const void*)k_tile_sort<512, kSortMedE>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)Medium::kLds);
-        (void)hipFuncSetAttribute((const void*)k_tile_sort<1024, kSortLargeE>,
+        (void)hipFuncSetAttribute((const void*)k_tile_sort_kp<1024, kSortLargeE>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)Large::kLds);
         return true;
     }();
@@ -943,14 +1169,14 @@ hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uin
                        idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo, 0u, small_cap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_tile_sort<512, kSortMedE>), dim3(ntiles), dim3(512), Medium::kLds, s,
-                       ranges, idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo, small_cap,
+    hipLaunchKernelGGL((k_tile_sort<512, kSortMedE>), dim3(ntiles), dim3(512), Medium::kLds,
+                       s, ranges, idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo, small_cap,
                        medium_cap);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_tile_sort<1024, kSortLargeE>), dim3(ntiles), dim3(1024), Large::kLds, s,
-                       ranges, idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo, medium_cap,
-                       0xffffffffu);
+    hipLaunchKernelGGL((k_tile_sort_kp<1024, kSortLargeE>), dim3(ntiles), dim3(1024),
+                       Large::kLds, s, ranges, idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo,
+                       medium_cap, 0xffffffffu);
     return hipGetLastError();
 }
 

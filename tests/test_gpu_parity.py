@@ -113,7 +113,7 @@ def test_binning_bitexact_near_equal_and_long_equal_runs(ctx):
 
 def test_binning_long_tile_lists(ctx):
     # Large Gaussians on a 128x128 view: lists in every sort configuration — up to 2048
-    # (256 threads), 2049-5120 (512), 5121-7168 (1024, LDS-resident) and past 7168 (streamed
+    # (256 threads), 2049-5120 (512), 5121-10240 (1024, LDS-resident) and past 10240 (streamed
     # passes through HBM).
     rng = np.random.default_rng(11)
     scene = S.make_scene(60000, seed=11, width=128, height=128)
@@ -122,7 +122,7 @@ def test_binning_long_tile_lists(ctx):
     _, b, _ = _check_binning(ctx, scene, np.array([0, 0, 0, 0, 0, 0, 1.0]), intr, 128, 128)
     lens = b["ranges"][:, 1] - b["ranges"][:, 0]
     print(f"[binning] long lists: max {lens.max()} min {lens.min()}", flush=True)
-    hist = np.histogram(lens, [0, 2049, 5121, 7169, 1 << 40])[0]
+    hist = np.histogram(lens, [0, 2049, 5121, 10241, 1 << 40])[0]
     assert lens.max() > 16384 and (hist > 0).all(), hist
 
 
