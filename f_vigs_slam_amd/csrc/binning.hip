@@ -175,11 +175,14 @@ __device__ __forceinline__ uint32_t rect_count(uint2 rc) {
 // heavy-tailed (a few tiles on average, hundreds for near, large Gaussians), so a loop per lane
 // over its own rectangle runs every wave for its largest one. Instead the wave numbers its
 // instances through a scan of the lane counts and deals them 64 at a time: lane l of window b0
-// takes instance j = b0 + l, finds the owning lane by binary search over the exclusive
-// prefixes and maps its local index to (tx, ty) row-major inside the owner's rectangle.
-// Trip count: ceil(instances / 64) per wave.
+// takes instance j = b0 + l and maps its local index row-major into the owning lane's
+// rectangle. The owner of each slot of a window comes from a 64-entry LDS row: every lane whose
+// instances start inside the window marks its first slot with (lane + 1) << 24 | first
+// instance, and an inclusive max-scan over the slots (DPP within rows of 16, then the rows'
+// maxima) carries each mark to the following slots; slot 0 also takes the last owner of the
+// previous window. Trip count: ceil(instances / 64) per wave.
 struct WaveWalk {
-    uint32_t excl, total;
+    uint32_t excl, total, cnt, carry;
 };
 
 __device__ __forceinline__ WaveWalk wave_walk_begin(uint32_t cnt, int lane) {
@@ -189,7 +192,21 @@ __device__ __forceinline__ WaveWalk wave_walk_begin(uint32_t cnt, int lane) {
         const uint32_t y = (uint32_t)__shfl_up((int)incl, o);
         if (lane >= o) incl += y;
     }
-    return WaveWalk{incl - cnt, (uint32_t)__shfl((int)incl, 63)};
+    return WaveWalk{incl - cnt, (uint32_t)__shfl((int)incl, 63), cnt, 0u};
+}
+
+// Inclusive max over lanes 0..lane.
+__device__ __forceinline__ uint32_t wave_max_scan(uint32_t v, int lane) {
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111 /* row_shr:1 */, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112 /* row_shr:2 */, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114 /* row_shr:4 */, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118 /* row_shr:8 */, 0xf, 0xf, false));
+    const uint32_t m0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 15);
+    const uint32_t m1 = max(m0, (uint32_t)__builtin_amdgcn_readlane((int)v, 31));
+    const uint32_t m2 = max(m1, (uint32_t)__builtin_amdgcn_readlane((int)v, 47));
+    const int row = lane >> 4;
+    const uint32_t pre = row == 0 ? 0u : row == 1 ? m0 : row == 2 ? m1 : m2;
+    return max(v, pre);
 }
 
 struct WalkSlot {
@@ -197,27 +214,28 @@ struct WalkSlot {
     bool on;
 };
 
-// Instance b0 + lane of the walk. All lanes of the wave call it (it shuffles); slots past the
-// wave's total come back with on == false.
-__device__ __forceinline__ WalkSlot wave_walk_slot(const WaveWalk& ww, uint32_t b0, uint2 rc,
-                                                   int lane) {
+// Instance b0 + lane of the walk. All lanes of the wave call it, window by window in order
+// (it shuffles, and carries the last owner in ww); mark is the wave's 64-entry LDS row. Slots
+// past the wave's total come back with on == false.
+__device__ __forceinline__ WalkSlot wave_walk_slot(WaveWalk& ww, uint32_t b0, uint2 rc, int lane,
+                                                   uint32_t* mark) {
     const uint32_t j = b0 + (uint32_t)lane;
-    // largest o with excl[o] <= j: the prefixes are nondecreasing and excl[0] == 0; among lanes
-    // sharing a prefix the last one is the one with instances
-    int o = 0;
-    uint32_t eo = 0;
-#pragma unroll
-    for (int step = 32; step > 0; step >>= 1) {
-        const uint32_t e = (uint32_t)__shfl((int)ww.excl, o + step);
-        if (e <= j) {
-            o += step;
-            eo = e;
-        }
-    }
-    const uint32_t rx = (uint32_t)__shfl((int)rc.x, o);
-    const uint32_t ry = (uint32_t)__shfl((int)rc.y, o);
+    mark[lane] = 0u;
+    __builtin_amdgcn_wave_barrier();
+    if (ww.cnt > 0u && ww.excl >= b0 && ww.excl < b0 + 64u)
+        mark[ww.excl - b0] = ((uint32_t)(lane + 1) << 24) | ww.excl;
+    __builtin_amdgcn_wave_barrier();
+    uint32_t v = mark[lane];
+    if (lane == 0) v = max(v, ww.carry);
+    v = wave_max_scan(v, lane);
+    ww.carry = (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+    __builtin_amdgcn_wave_barrier();
+    const int o = (int)(v >> 24) - 1;   // every slot below the total has an owner
+    const uint32_t eo = v & 0xffffffu;
+    const uint32_t rx = (uint32_t)__shfl((int)rc.x, o < 0 ? 0 : o);
+    const uint32_t ry = (uint32_t)__shfl((int)rc.y, o < 0 ? 0 : o);
     WalkSlot s;
-    s.owner = o;
+    s.owner = o < 0 ? 0 : o;
     s.on = j < ww.total;
     const uint32_t x0 = rx & 0xffffu, wx = (rx >> 16) - x0 + 1u;
     const uint32_t k = j - eo;
@@ -237,18 +255,26 @@ __device__ __forceinline__ WalkSlot wave_walk_slot(const WaveWalk& ww, uint32_t 
     return s;
 }
 
+// A Gaussian's map attributes, loaded ahead of their use.
+struct GaussIn {
+    float pw[3], sc[3], qg[4], col[3], op;
+    __device__ __forceinline__ void load(const MapRef& map, int g) {
+        map.geometry((size_t)g, pw, sc, qg);
+        map.appearance((size_t)g, col, op);
+    }
+};
+
 // GSCudaKernels.cu:345-476 and 566-591 for Gaussian g: stores its screen record and tile
 // rectangle and returns the rectangle.
-__device__ __forceinline__ uint2 project_store(const MapRef& map, int g, const View& v,
+__device__ __forceinline__ uint2 project_store(const GaussIn& in, int g, const View& v,
                                                const float Rc[3][3], int cull_emit,
                                                Rec* __restrict__ rec, uint2* __restrict__ rect,
                                                uint32_t* __restrict__ zinfo) {
     float u = 0.f, vv = 0.f, z = 0.f, cov[3] = {0.f, 0.f, 0.f}, con[3] = {0.f, 0.f, 0.f},
           ph[2] = {0.f, 0.f};
-    float pw[3], sc[3], qg[4], col[3], op;
-    map.geometry((size_t)g, pw, sc, qg);
-    map.appearance((size_t)g, col, op);
-    const bool vis = project_exact(pw, sc, qg, v, Rc, u, vv, z, cov, con, ph);
+    const float* col = in.col;
+    const float op = in.op;
+    const bool vis = project_exact(in.pw, in.sc, in.qg, v, Rc, u, vv, z, cov, con, ph);
     if (!vis) {
         u = vv = z = 0.f;
         cov[0] = cov[1] = cov[2] = 0.f;
@@ -283,6 +309,7 @@ __global__ __launch_bounds__(kBinThreads) void k_project(
     Rec* __restrict__ rec, uint2* __restrict__ rect, uint32_t* __restrict__ blk_counts,
     uint32_t* __restrict__ zinfo) {   // zinfo[1]: visible tile-0 instances with key 0
     extern __shared__ uint32_t hist[];   // [0] culled (zero segment), [1 + t] tile t
+    __shared__ uint32_t wmark[kBinThreads / 64][64];
     const View v = vd ? *vd : vh;   // device-resident view (pose updated on the device) or argument
     const int T = v.ntx * v.nty;
     for (int t = threadIdx.x; t <= T; t += kBinThreads) hist[t] = 0;
@@ -290,15 +317,23 @@ __global__ __launch_bounds__(kBinThreads) void k_project(
     float Rc[3][3];
     quat_to_mat_exact(v.qx, v.qy, v.qz, v.qw, Rc);
     const int lane = threadIdx.x & 63;
+    // the next item's attributes load while this one is projected
+    GaussIn cur, nxt;
+    {
+        const int g = blockIdx.x * kBinChunk + threadIdx.x;
+        if (g < n) cur.load(map, g);
+    }
     for (int it = 0; it < kProjItems; ++it) {
         const int g = blockIdx.x * kBinChunk + it * kBinThreads + threadIdx.x;
+        if (it + 1 < kProjItems && g + kBinThreads < n) nxt.load(map, g + kBinThreads);
         const uint2 rc =
-            g < n ? project_store(map, g, v, Rc, cull_emit, rec, rect, zinfo) : make_uint2(1u, 0u);
+            g < n ? project_store(cur, g, v, Rc, cull_emit, rec, rect, zinfo) : make_uint2(1u, 0u);
+        cur = nxt;
         const uint64_t culled = __ballot(rc.y == kCulledY);
         if (lane == 0 && culled) atomicAdd(&hist[0], (uint32_t)__popcll(culled));
-        const WaveWalk ww = wave_walk_begin(rect_count(rc), lane);
+        WaveWalk ww = wave_walk_begin(rect_count(rc), lane);
         for (uint32_t b0 = 0; b0 < ww.total; b0 += 64) {
-            const WalkSlot sl = wave_walk_slot(ww, b0, rc, lane);
+            const WalkSlot sl = wave_walk_slot(ww, b0, rc, lane, wmark[threadIdx.x >> 6]);
             if (sl.on) atomicAdd(&hist[1 + sl.ty * v.ntx + sl.tx], 1u);
         }
     }
@@ -399,6 +434,7 @@ __global__ __launch_bounds__(kBinThreads) void k_emit(
     View vh, const View* __restrict__ vd, uint64_t* __restrict__ pairs, uint64_t cap) {
     extern __shared__ uint32_t cur[];   // [0] zero segment, [1 + t] tile t
     __shared__ uint32_t wz[kBinThreads / 64];
+    __shared__ uint32_t wmark[kBinThreads / 64][64];
     const View v = vd ? *vd : vh;
     const int T = v.ntx * v.nty;
     // the first of this block's kEmitRows count-matrix rows holds the column prefix at the block
@@ -435,10 +471,10 @@ __global__ __launch_bounds__(kBinThreads) void k_emit(
             r0 = rec[g].r0;
             r1 = rec[g].r1;
         }
-        const WaveWalk ww = wave_walk_begin(cnt, lane);
+        WaveWalk ww = wave_walk_begin(cnt, lane);
         const int g0 = g - lane;
         for (uint32_t b0 = 0; b0 < ww.total; b0 += 64) {
-            const WalkSlot sl = wave_walk_slot(ww, b0, vrc, lane);
+            const WalkSlot sl = wave_walk_slot(ww, b0, vrc, lane, wmark[w]);
             const float u = __shfl(r0.x, sl.owner), vv = __shfl(r0.y, sl.owner);
             const float z = __shfl(r1.y, sl.owner), phx = __shfl(r1.z, sl.owner),
                         phy = __shfl(r1.w, sl.owner);

@@ -97,7 +97,7 @@ struct BlendParams {
     float bg[3];
     float alpha_thresh, color_thresh, depth_thresh, depth_weight;
     float w_depth, w_dist, lambda_iso;
-    int kahan;   // pose sums: 0 FP64 running sums, 1 Kahan FP32 (precision form)
+    int kahan;   // pose sums: 0 FP32 per pixel, FP64 reductions; 1 Kahan FP32 (precision form)
 };
 
 // ---- binning (binning.hip) ----
