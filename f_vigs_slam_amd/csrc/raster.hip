@@ -85,9 +85,10 @@ __global__ __launch_bounds__(256) void k_render(View v, const uint32_t* __restri
 }
 
 // ---- A14 ------------------------------------------------------------------------------------
-// One workgroup per tile, one wave per 8x8 quadrant, one thread per pixel. Each batch of 256
-// records is staged in LDS (16 KB) with its quadrant masks; each wave walks the compacted list of
-// the entries its quadrant needs. The (T, colour) checkpoints at list indices 0, 32, ..., 224 go
+// One workgroup per tile, one wave per 8x8 quadrant, one thread per pixel. Each wave walks the
+// tile list on its own in chunks of 64 entries (one per lane, records prefetched a chunk ahead),
+// keeps the entries its quadrant needs in its own LDS rows, and stops when its pixels are done:
+// no block barrier ties a quadrant to the slowest one of its tile. The (T, colour) checkpoints at list indices 0, 32, ..., 224 go
 // to HBM as float4 per (tile, bucket, pixel) — only the first 256 entries of a tile ever get
 // gradients (GSCudaKernels.cu:2329-2330), so no other bucket needs one. A checkpoint is written
 // when the walk reaches or passes its index; skipped entries do not change the state.
@@ -98,9 +99,11 @@ __global__ __launch_bounds__(256) void k_forward(
     float* __restrict__ out_rgb, float* __restrict__ out_depth, float* __restrict__ out_T,
     uint32_t* __restrict__ out_nc, float* __restrict__ out_err, uint32_t* __restrict__ max_contrib,
     float4* __restrict__ ckpt, const uint32_t* __restrict__ zinfo) {
-    __shared__ Rec recs[256];
-    __shared__ uint8_t qmask[256];
-    __shared__ __align__(4) uint8_t wlist[4][256];
+    struct Rec3 {
+        float4 r0, r1, r2;
+    };
+    __shared__ Rec3 wrec[4][64];                    // a wave's compacted entries of one chunk
+    __shared__ __align__(4) uint8_t wpos[4][64];    // their offsets in the chunk
     __shared__ uint32_t smax;
     const int tile = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int tx0 = (tile % v.ntx) * kTile, ty0 = (tile / v.ntx) * kTile;
@@ -136,56 +139,60 @@ __global__ __launch_bounds__(256) void k_forward(
     const uint32_t base0 = zlead & ~255u;
     if (base0 > 0 && !done)
         for (int b = 0; b < kMaxBuckets; ++b) ck[b * 256] = make_float4(1.f, 0.f, 0.f, 0.f);
-    for (uint32_t base = base0; base < n; base += 256) {
-        if (__syncthreads_count(!done) == 0) break;
-        const int cnt = (int)min(256u, n - base);
-        if (tid < cnt) {
-            const Rec r = rec[idx[s + base + tid]];
-            recs[tid] = r;
-            qmask[tid] = (uint8_t)quadrant_mask<QW>(r, tx0, ty0, true);
+    // Each wave walks the list on its own, 64 entries at a time, with no block barrier: lane l
+    // holds entry base + l (its records loaded one chunk ahead, its index two), tests it against
+    // the wave's pixel block, and the entries that pass are compacted into the wave's LDS rows.
+    // A wave stops when its own pixels are done.
+    const int nb = (int)((min(n, 256u) + kBucket - 1) / kBucket);   // checkpoint buckets
+    int nextb = base0 > 0 ? nb : 0;   // a zero segment of 256 or more wrote them above
+    float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0, p2 = p0;
+    uint32_t nidx = 0;
+    if (base0 + lane < n) {
+        const uint32_t g = idx[s + base0 + lane];
+        p0 = rec[g].r0; p1 = rec[g].r1; p2 = rec[g].r2;
+    }
+    if (base0 + 64 + lane < n) nidx = idx[s + base0 + 64 + lane];
+    for (uint32_t base = base0; base < n; base += 64) {
+        if (__ballot(!done) == 0) break;   // uniform
+        const uint32_t e = base + (uint32_t)lane;
+        const bool bit = e < n && quadrant_hit<QW>(p0, p1, p2, tx0, ty0, w, true);
+        const uint64_t bal = __ballot(bit);
+        const int wc = __popcll(bal);
+        if (bit) {
+            const int q = __popcll(bal & ((1ull << lane) - 1ull));
+            wrec[w][q].r0 = p0; wrec[w][q].r1 = p1; wrec[w][q].r2 = p2;
+            wpos[w][q] = (uint8_t)lane;
         }
-        __syncthreads();
-        int wc = 0;
-#pragma unroll
-        for (int r4 = 0; r4 < 4; ++r4) {
-            const int e = r4 * 64 + lane;
-            const bool bit = e < cnt && ((qmask[e] >> w) & 1u);
-            const uint64_t bal = __ballot(bit);
-            if (bit) wlist[w][wc + __popcll(bal & ((1ull << lane) - 1ull))] = (uint8_t)e;
-            wc += __popcll(bal);
+        if (e + 64 < n) {
+            p0 = rec[nidx].r0; p1 = rec[nidx].r1; p2 = rec[nidx].r2;
         }
-        __syncthreads();
+        if (e + 128 < n) nidx = idx[s + e + 128];
+        __builtin_amdgcn_wave_barrier();
         if (done) continue;
-        const int nb = base == 0 ? (cnt + kBucket - 1) / kBucket : 0;
-        int nextb = 0;
-        // Four entries per round: one 32-bit read of their indices and their 12 record loads
-        // issued together (unconditionally: indices past the list are stale bytes < 256, in
-        // bounds), so the LDS latencies overlap each other and the blending.
-        const uint32_t* wl4 = reinterpret_cast<const uint32_t*>(wlist[w]);
-        uint32_t w4n = wl4[0];
+        // Four entries per round: one 32-bit read of their offsets and their 12 record loads
+        // issued together (slots past wc hold stale in-bounds bytes, never used).
+        const uint32_t* wp4 = reinterpret_cast<const uint32_t*>(wpos[w]);
         for (int k = 0; k < wc && !done; k += 4) {
-            const uint32_t w4 = w4n;
-            w4n = wl4[((k >> 2) + 1) & 63];   // next round's indices (in bounds; unused last)
-            int ii[4];
+            const uint32_t o4 = wp4[k >> 2];
             float4 a0[4], a1[4], a2[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                ii[u] = (int)((w4 >> (8 * u)) & 255u);
-                a0[u] = recs[ii[u]].r0;
-                a1[u] = recs[ii[u]].r1;
-                a2[u] = recs[ii[u]].r2;
+                a0[u] = wrec[w][k + u].r0;
+                a1[u] = wrec[w][k + u].r1;
+                a2[u] = wrec[w][k + u].r2;
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 if (k + u >= wc || done) break;
-                const int i = ii[u];
-                for (; nextb < nb && nextb * kBucket <= i; ++nextb)
+                const uint32_t i = base + ((o4 >> (8 * u)) & 255u);   // list index
+                for (; nextb < nb && (uint32_t)(nextb * kBucket) <= i; ++nextb)
                     ck[nextb * 256] = make_float4(T, cr, cg, cb);
-                if (step(a0[u], a1[u], a2[u], base + (uint32_t)i)) done = true;
+                if (step(a0[u], a1[u], a2[u], i)) done = true;
             }
         }
-        if (!done)
-            for (; nextb < nb; ++nextb) ck[nextb * 256] = make_float4(T, cr, cg, cb);
+        if (!done)   // buckets starting in this chunk after its last entry of this wave
+            for (; nextb < nb && (uint32_t)(nextb * kBucket) < base + 64u; ++nextb)
+                ck[nextb * 256] = make_float4(T, cr, cg, cb);
     }
     if (inside) {
         const size_t p = (size_t)y * v.W + x;
