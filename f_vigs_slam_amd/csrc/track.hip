@@ -4,12 +4,12 @@
 // GSSlam::computeRgbdPoseJacobians (GSSlam.cu:2167-2226).
 //
 // k_pose: one workgroup per 16x16 tile, one thread per pixel. Pass 1 re-renders the pixel
-// (LDS-staged records, wave-uniform early exit), pass 2 walks the same list again for the
-// per-Gaussian terms. Each thread accumulates its 21 + 6 products in registers; the tile's
-// partial is a wave butterfly reduction plus a fixed-order sum of the four waves, written
-// once per tile (no atomics, deterministic). k_pose_reduce sums the tile partials in FP64 in
-// tile order. The reference's divergent barriers inside `if (inside)` (Q12) are replaced by a
-// block-uniform batch loop in which every pixel sees the whole list.
+// (per-wave walks of LDS-staged 64-entry chunks, wave-uniform early exit), pass 2 walks the same
+// list again for the per-Gaussian terms. Each thread accumulates its 21 + 6 products in
+// registers; the tile's partial is a wave butterfly reduction plus a fixed-order sum of the four
+// waves, written once per tile (no atomics, deterministic). k_pose_reduce sums the tile partials
+// in FP64 in tile order. The reference's divergent barriers inside `if (inside)` (Q12) are
+// replaced by a wave-uniform chunk loop in which every pixel sees the whole list.
 #include "fvgs_internal.h"
 #include "gn_step.h"
 #include "blend_common.h"
@@ -162,9 +162,8 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
     // kPass 1: pass 1 only; 2: Jacobians with FP64 running sums; 3: Jacobians with Kahan-
     // compensated FP32 running sums and a compensated FP32 block reduction (precision form).
     constexpr bool kKahan = kPass == 3;
-    __shared__ Rec recs[256];
-    __shared__ uint8_t qmask[256];
-    __shared__ __align__(4) uint8_t wlist[4][256];
+    __shared__ Rec wrec[4][64];                     // a wave's compacted entries of one chunk
+    __shared__ __align__(4) uint8_t wpos[4][64];    // their offsets in the chunk
     __shared__ double red[4][27];
     const int tile = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int tx0 = (tile % v.ntx) * kTile, ty0 = (tile / v.ntx) * kTile;
@@ -177,32 +176,41 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
     const uint32_t s = ranges[2 * tile], n = ranges[2 * tile + 1] - s;
     const float px = (float)x, py = (float)y;
 
-    // Stages batch `base` in LDS and builds this wave's list of the entries its quadrant needs in
-    // this pass (pass 1: q > 0 and alpha >= 1/255; pass 2: alpha >= 1/255). Returns the list
-    // length (uniform).
-    auto stage = [&](uint32_t base, int pass) -> int {
-        const int cnt = (int)min(256u, n - base);
-        if (tid < cnt) {
-            Rec r = rec[idx[s + base + tid]];
-            qmask[tid] = (uint8_t)quadrant_mask(r, tx0, ty0, pass == 1);
-            if (pass == 2) {   // the ray of the mean, once per record instead of per pixel
-                r.r3.x = (r.r0.x - v.cx) / v.fx;
-                r.r3.y = (r.r0.y - v.cy) / v.fy;
+    // Each wave walks the tile list on its own, 64 entries at a time, with no block barrier: lane
+    // l holds entry base + l (its record loaded one chunk ahead, its index two), tests it against
+    // the wave's quadrant for this pass (pass 1: q > 0 and alpha >= 1/255; pass 2: alpha >= 1/255)
+    // and the entries that pass are compacted into the wave's LDS rows. chunk_stage returns their
+    // number (uniform); the pass-2 rows carry the ray of the mean in r3.
+    Rec pr;
+    uint32_t nidx = 0;
+    auto chunk_init = [&]() {
+        pr.r0 = pr.r1 = pr.r2 = pr.r3 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if ((uint32_t)lane < n) {
+            const uint32_t g = idx[s + lane];
+            pr.r0 = rec[g].r0; pr.r1 = rec[g].r1; pr.r2 = rec[g].r2;
+        }
+        if (64u + lane < n) nidx = idx[s + 64 + lane];
+    };
+    auto chunk_stage = [&](uint32_t base, int pass) -> int {
+        const uint32_t e = base + (uint32_t)lane;
+        const bool bit = e < n && quadrant_hit<8>(pr.r0, pr.r1, pr.r2, tx0, ty0, w, pass == 1);
+        const uint64_t bal = __ballot(bit);
+        if (bit) {
+            const int q = __popcll(bal & ((1ull << lane) - 1ull));
+            Rec r = pr;
+            if (pass == 2) {   // the ray of the mean, once per entry instead of per pixel
+                r.r3.x = (pr.r0.x - v.cx) / v.fx;
+                r.r3.y = (pr.r0.y - v.cy) / v.fy;
             }
-            recs[tid] = r;
+            wrec[w][q] = r;
+            wpos[w][q] = (uint8_t)lane;
         }
-        __syncthreads();
-        int wc = 0;
-#pragma unroll
-        for (int r4 = 0; r4 < 4; ++r4) {
-            const int e = r4 * 64 + lane;
-            const bool bit = e < cnt && ((qmask[e] >> w) & 1u);
-            const uint64_t bal = __ballot(bit);
-            if (bit) wlist[w][wc + __popcll(bal & ((1ull << lane) - 1ull))] = (uint8_t)e;
-            wc += __popcll(bal);
+        if (e + 64 < n) {
+            pr.r0 = rec[nidx].r0; pr.r1 = rec[nidx].r1; pr.r2 = rec[nidx].r2;
         }
-        __syncthreads();
-        return wc;
+        if (e + 128 < n) nidx = idx[s + e + 128];
+        __builtin_amdgcn_wave_barrier();
+        return __popcll(bal);
     };
 
     // ---------------- pass 1: render ----------------
@@ -220,23 +228,24 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
         float T = 1.f;
         uint32_t last = 0;
         bool done = !inside;
-        for (uint32_t base = 0; base < n; base += 256) {
-            if (__syncthreads_count(!done) == 0) break;
-            const int wc = stage(base, 1);
+        chunk_init();
+        for (uint32_t base = 0; base < n; base += 64) {
+            if (__ballot(!done) == 0) break;   // uniform
+            const int wc = chunk_stage(base, 1);
             if (done) continue;
-            // Four entries per round, their record loads issued together (indices past the list
-            // are stale bytes < 256: in bounds, never used).
-            const uint32_t* wl4 = reinterpret_cast<const uint32_t*>(wlist[w]);
+            // Four entries per round, their record loads issued together (slots past wc hold
+            // stale in-bounds bytes, never used).
+            const uint32_t* wp4 = reinterpret_cast<const uint32_t*>(wpos[w]);
             for (int k = 0; k < wc && !done; k += 4) {
-                const uint32_t w4 = wl4[k >> 2];
+                const uint32_t o4 = wp4[k >> 2];
                 int ii[4];
                 float4 b0[4], b1[4], b2[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    ii[u] = (int)((w4 >> (8 * u)) & 255u);
-                    b0[u] = recs[ii[u]].r0;
-                    b1[u] = recs[ii[u]].r1;
-                    b2[u] = recs[ii[u]].r2;
+                    ii[u] = (int)((o4 >> (8 * u)) & 255u);
+                    b0[u] = wrec[w][k + u].r0;
+                    b1[u] = wrec[w][k + u].r1;
+                    b2[u] = wrec[w][k + u].r2;
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -339,16 +348,16 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
     {
         float prod = 1.f, T2 = 1.f, a0 = 0.f, a1 = 0.f, a2 = 0.f;
         bool active = inside && n_contrib > 0;
-        for (uint32_t base = 0; base < n; base += 256) {
+        chunk_init();
+        for (uint32_t base = 0; base < n; base += 64) {
             const bool want = active && base < n_contrib;
-            if (__syncthreads_count(want) == 0) break;
-            const int wc = stage(base, 2);
+            if (__ballot(want) == 0) break;   // uniform
+            const int wc = chunk_stage(base, 2);
             if (!want) continue;
             for (int k = 0; k < wc; ++k) {
-                const int i = wlist[w][k];
-                if (base + (uint32_t)i >= n_contrib) break;
-                const float4 r0 = recs[i].r0, r1 = recs[i].r1, r2 = recs[i].r2;
-                const float2 rg = make_float2(recs[i].r3.x, recs[i].r3.y);   // ray of the mean
+                if (base + (uint32_t)wpos[w][k] >= n_contrib) break;
+                const float4 r0 = wrec[w][k].r0, r1 = wrec[w][k].r1, r2 = wrec[w][k].r2;
+                const float2 rg = make_float2(wrec[w][k].r3.x, wrec[w][k].r3.y);   // ray of the mean
                 const float dx = r0.x - px, dy = r0.y - py;
                 const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;
                 const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));   // Q8: no q <= 0 test
