@@ -307,7 +307,7 @@ __device__ __forceinline__ uint2 project_store(const GaussIn& in, int g, const V
 __global__ __launch_bounds__(kBinThreads) void k_project(
     MapRef map, int n, View vh, const View* __restrict__ vd, int cull_emit,
     Rec* __restrict__ rec, uint2* __restrict__ rect, uint32_t* __restrict__ blk_counts,
-    uint32_t* __restrict__ zinfo) {   // zinfo[1]: visible tile-0 instances with key 0
+    uint32_t* __restrict__ zinfo) {   // zinfo[1]: visible tile-0 instances with key 0 (zero on entry)
     extern __shared__ uint32_t hist[];   // [0] culled (zero segment), [1 + t] tile t
     __shared__ uint32_t wmark[kBinThreads / 64][64];
     const View v = vd ? *vd : vh;   // device-resident view (pose updated on the device) or argument
@@ -393,6 +393,12 @@ __global__ __launch_bounds__(1024) void k_colscan_tiles(uint32_t* __restrict__ t
     extern __shared__ uint32_t tot[];   // cols entries, then 1024 scratch
     uint32_t* part = tot + cols;
     const int tid = threadIdx.x;
+    // hand the projection's count of visible key-0 tile-0 instances to the later kernels and
+    // clear it for the next projection
+    if (tid == 0) {
+        zinfo[2] = zinfo[1];
+        zinfo[1] = 0u;
+    }
     for (int t = tid; t < cols; t += 1024) tot[t] = tot_base[t];
     __syncthreads();
     // contiguous segment per thread, then a scan of the segment sums
@@ -728,8 +734,8 @@ __global__ __launch_bounds__(NT) void k_tile_sort(const uint32_t* __restrict__ r
     const int tile = blockIdx.x, tid = threadIdx.x;
     const uint32_t s0 = ranges[2 * tile], n0 = ranges[2 * tile + 1] - s0;
     // Tile 0 starts with the zero segment (culled instances, key 0, already in index order);
-    // unless a visible instance also has key 0 (zinfo[1]), only the rest is sorted.
-    const uint32_t zskip = (tile == 0 && zinfo[1] == 0u) ? min(zinfo[0], n0) : 0u;
+    // unless a visible instance also has key 0 (zinfo[2]), only the rest is sorted.
+    const uint32_t zskip = (tile == 0 && zinfo[2] == 0u) ? min(zinfo[0], n0) : 0u;
     const uint32_t s = s0 + zskip, n = n0 - zskip;
     const bool mine = (n > lo && n <= hi) || (n == 0 && n0 > 0 && lo == 0);
     if (!mine) return;
@@ -958,7 +964,7 @@ __global__ __launch_bounds__(NT) void k_tile_sort_kp(const uint32_t* __restrict_
     uint16_t* pos = reinterpret_cast<uint16_t*>(keys + CHUNK);
     const int tile = blockIdx.x, tid = threadIdx.x;
     const uint32_t s0 = ranges[2 * tile], n0 = ranges[2 * tile + 1] - s0;
-    const uint32_t zskip = (tile == 0 && zinfo[1] == 0u) ? min(zinfo[0], n0) : 0u;
+    const uint32_t zskip = (tile == 0 && zinfo[2] == 0u) ? min(zinfo[0], n0) : 0u;
     const uint32_t s = s0 + zskip, n = n0 - zskip;
     if (!(n > lo && n <= hi)) return;
     for (uint32_t j = tid; j < zskip; j += NT) {
@@ -1185,12 +1191,11 @@ hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uin
                             const uint32_t* zinfo, size_t lds_limit, hipStream_t s) {
     int idx_bits = 1;
     while (idx_bits < 32 && ((uint64_t)1 << idx_bits) < (uint64_t)n_gauss) ++idx_bits;
-    // Lists up to 2048 keys: 256 threads (22 KB LDS); up to kMedE * 512: 512 threads (50 KB, three
-    // per CU); longer: 1024 threads with 6-byte entries, kLargeE * 1024 resident (77 KB, two per
-    // CU), streamed through HBM beyond that. Each launch covers every tile and skips the other
-    // lengths. (6-byte entries for the 512-thread lists fit four per CU but measured slower:
+    // Lists up to kMedE * 512 keys: 512 threads (50 KB LDS, three per CU); longer: 1024 threads
+    // with 6-byte entries, kLargeE * 1024 resident (77 KB, two per CU), streamed through HBM beyond
+    // that. Each launch covers every tile and skips the other lengths (a launch whose tiles all
+    // skip still costs ~4 us, so short lists share the 512-thread launch). (6-byte entries for the 512-thread lists fit four per CU but measured slower:
     // the index gathers from the list cost more than the extra workgroup gains.)
-    using Small = SortCfg<256, 8>;
     using Medium = SortCfg<512, kSortMedE>;
     using Large = SortKpCfg<1024, kSortLargeE>;
     static const bool attrs = [] {
@@ -1200,15 +1205,11 @@ hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uin
     }();
     (void)attrs;
     if (Large::kLds > lds_limit) return hipErrorInvalidConfiguration;
-    const uint32_t small_cap = (uint32_t)Small::CHUNK, medium_cap = (uint32_t)Medium::CHUNK;
-    hipLaunchKernelGGL((k_tile_sort<256, 8>), dim3(ntiles), dim3(256), Small::kLds, s, ranges,
-                       idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo, 0u, small_cap);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    const uint32_t medium_cap = (uint32_t)Medium::CHUNK;
     hipLaunchKernelGGL((k_tile_sort<512, kSortMedE>), dim3(ntiles), dim3(512), Medium::kLds,
-                       s, ranges, idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo, small_cap,
+                       s, ranges, idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo, 0u,
                        medium_cap);
-    e = hipGetLastError();
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_tile_sort_kp<1024, kSortLargeE>), dim3(ntiles), dim3(1024),
                        Large::kLds, s, ranges, idx_bits, pairs, tmp, sorted_idx, keys_out, zinfo,

@@ -211,6 +211,15 @@ void collect_timing(fvgs_ctx* c, int nmarks) {
 constexpr int kOverflow = -100;   // internal: instance buffers too small for this view
 
 // Instance buffers sized for at least m entries; inst_cap = entries every one of them holds.
+// M | zero-segment length | visible key-0 count in tile 0 (kept zero between projections: the
+// column scan takes it) | that count for the sort and the blend. Zeroed once when allocated.
+int ensure_dm(fvgs_ctx* c) {
+    const bool fresh = c->d_m.p == nullptr;
+    TRY(ensure(c, c->d_m, 16));
+    if (fresh) HIPCHK(c, launch_zero(c->d_m.p, 16, c->stream));
+    return FVGS_OK;
+}
+
 int ensure_instances(fvgs_ctx* c, uint64_t m) {
     TRY(ensure(c, c->pairs, 8ull * m));
     TRY(ensure(c, c->tmp, 8ull * m));
@@ -259,13 +268,12 @@ int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int
     TRY(ensure(c, c->blk_counts, 4ull * nblk * (T + 1)));
     TRY(ensure(c, c->col_base, 4ull * (T + 1)));
     TRY(ensure(c, c->ranges, 8ull * T));
-    TRY(ensure(c, c->d_m, 16));   // M | zero-segment length | visible key-0 count in tile 0
+    TRY(ensure_dm(c));
     if (async && c->inst_cap == 0) async = false;
     hipStream_t s = c->stream;
     c->view = v;
     mark(c, 0);
     uint32_t* zinfo = as<uint32_t>(c->d_m) + 1;
-    HIPCHK(c, launch_zero(zinfo + 1, 4, s));
     HIPCHK(c, launch_project(map_ref(c), n, v, c->view_dev, cull_emit, as<Rec>(c->rec),
                              as<uint2>(c->rect), as<uint32_t>(c->blk_counts), zinfo, s));
     HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->col_base), nblk, T,
@@ -1282,6 +1290,7 @@ int fvgs_track(fvgs_ctx* c, const fvgs_intrinsics* intr, int iters, double pose7
     TRY(ensure(c, c->track, bytes));
     TRY(ensure(c, c->pose_tiles, 27 * 8ull * T));
     TRY(ensure(c, c->pose_sum, 27 * 8));
+    TRY(ensure_dm(c));
     float4* pc = nullptr;
     uint2* ps = nullptr;
     TRY(pose_pixel_state(c, (size_t)v0.W * v0.H, &pc, &ps));
@@ -1313,7 +1322,6 @@ int fvgs_track(fvgs_ctx* c, const fvgs_intrinsics* intr, int iters, double pose7
         uint32_t* zinfo = as<uint32_t>(c->d_m) + 1;
         for (int it = 0; it < iters; it++) {
             HIPCHK(c, hipEventRecord(c->track_ev[it], s));
-            HIPCHK(c, launch_zero(zinfo + 1, 4, s));
             HIPCHK(c, launch_project(map_ref(c), n, v0, vd, 0, as<Rec>(c->rec), as<uint2>(c->rect),
                                      as<uint32_t>(c->blk_counts), zinfo, s));
             HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->col_base), nblk,

@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void k_forward(
     // Tile 0 starts with the zero segment (culled zero records: q = 0, never contributing) unless
     // a visible key-0 instance is mixed in; its whole batches are not staged. The state stays
     // (T = 1, colour 0) over it, which is also every checkpoint inside it.
-    const uint32_t zlead = (tile == 0 && zinfo[1] == 0u) ? min(zinfo[0], n) : 0u;
+    const uint32_t zlead = (tile == 0 && zinfo[2] == 0u) ? min(zinfo[0], n) : 0u;
     const uint32_t base0 = zlead & ~255u;
     if (base0 > 0 && !done)
         for (int b = 0; b < kMaxBuckets; ++b) ck[b * 256] = make_float4(1.f, 0.f, 0.f, 0.f);

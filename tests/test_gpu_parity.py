@@ -112,8 +112,8 @@ def test_binning_bitexact_near_equal_and_long_equal_runs(ctx):
 
 
 def test_binning_long_tile_lists(ctx):
-    # Large Gaussians on a 128x128 view: lists in every sort configuration — up to 2048
-    # (256 threads), 2049-5120 (512), 5121-10240 (1024, LDS-resident) and past 10240 (streamed
+    # Large Gaussians on a 128x128 view: lists in every sort configuration — up to 5120 (512
+    # threads; short and long ones), 5121-10240 (1024, LDS-resident) and past 10240 (streamed
     # passes through HBM).
     rng = np.random.default_rng(11)
     scene = S.make_scene(60000, seed=11, width=128, height=128)
