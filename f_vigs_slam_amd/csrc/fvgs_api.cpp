@@ -332,18 +332,22 @@ int check_frame(fvgs_ctx* c) {
     return FVGS_OK;
 }
 
-int run_forward(fvgs_ctx* c) {
+// clear_d2: the same launch also clears the 2D gradient accumulators for the backward that
+// follows (its clearing blocks run beside the tiles).
+int run_forward(fvgs_ctx* c, bool clear_d2 = false) {
     if (!c->prepared) return fail(c, FVGS_E_STATE, "call fvgs_prepare_rasterization first");
     TRY(check_frame(c));
     TRY(alloc_pixel_outputs(c));
     const int T = c->view.ntx * c->view.nty;
     TRY(ensure(c, c->ckpt, sizeof(float4) * kMaxBuckets * 256ull * T));
+    const size_t d2_bytes = sizeof(Delta2D) * (size_t)c->n;
+    if (clear_d2) TRY(ensure(c, c->d2, d2_bytes + 16));
     HIPCHK(c, launch_forward(c->view, as<uint32_t>(c->ranges), as<uint32_t>(c->sorted_idx),
                              as<Rec>(c->rec), c->obs_rgb, blend_params(c->prm),
                              as<float>(c->o_rgb), as<float>(c->o_depth), as<float>(c->o_T),
                              as<uint32_t>(c->o_nc), as<float>(c->o_err), as<uint32_t>(c->o_maxc),
                              as<float4>(c->ckpt), as<uint32_t>(c->d_m) + 1, c->wave_rows,
-                             c->stream));
+                             c->stream, clear_d2 ? c->d2.p : nullptr, clear_d2 ? d2_bytes : 0));
     c->forwarded = true;
     c->backwarded = false;
     return FVGS_OK;
@@ -812,9 +816,8 @@ int fvgs_mapping_step(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics*
     if (!c->obs_rgb) return fail(c, FVGS_E_STATE, "no frame set");
     auto attempt = [&](bool async) -> int {
         TRY(prepare(c, pose, intr, c->fw, c->fh, async, c->prm.cull_emit));
-        TRY(clear_delta2d(c));
         mark(c, 2);
-        TRY(run_forward(c));
+        TRY(run_forward(c, true));   // also clears the 2D gradient accumulators
         mark(c, 3);
         TRY(run_backward(c));
         mark(c, 4);
@@ -1538,8 +1541,7 @@ int wfail(fvgs_window* w, int code, const char* fmt, ...) {
 // synchronising), clear, forward, backward, 2D->3D.
 int enqueue_mapping(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr) {
     TRY(prepare(c, pose, intr, c->fw, c->fh, true, c->prm.cull_emit));
-    TRY(clear_delta2d(c));
-    TRY(run_forward(c));
+    TRY(run_forward(c, true));   // also clears the 2D gradient accumulators
     TRY(run_backward(c));
     TRY(run_delta3d(c));
     return FVGS_OK;

@@ -130,7 +130,7 @@ hipError_t launch_forward(const View& v, const uint32_t* ranges, const uint32_t*
                           const Rec* rec, const float* obs_rgb, const BlendParams& bp, float* rgb,
                           float* depth, float* final_T, uint32_t* n_contrib, float* color_err,
                           uint32_t* max_contrib, float4* ckpt, const uint32_t* zinfo, int wave_rows,
-                          hipStream_t s);
+                          hipStream_t s, void* zero_p = nullptr, size_t zero_bytes = 0);
 hipError_t launch_backward(const View& v, const uint32_t* ranges, const uint32_t* idx,
                            const Rec* rec, const float* obs_depth, const BlendParams& bp,
                            const float* rgb, const float* depth, const uint32_t* n_contrib,

@@ -92,13 +92,28 @@ __global__ __launch_bounds__(256) void k_render(View v, const uint32_t* __restri
 // to HBM as float4 per (tile, bucket, pixel) — only the first 256 entries of a tile ever get
 // gradients (GSCudaKernels.cu:2329-2330), so no other bucket needs one. A checkpoint is written
 // when the walk reaches or passes its index; skipped entries do not change the state.
+constexpr int kFwdZero16 = 16;   // 16-byte stores per thread of a clearing block
+
 template <int QW>
 __global__ __launch_bounds__(256) void k_forward(
     View v, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ idx,
     const Rec* __restrict__ rec, const float* __restrict__ obs_rgb, BlendParams bp,
     float* __restrict__ out_rgb, float* __restrict__ out_depth, float* __restrict__ out_T,
     uint32_t* __restrict__ out_nc, float* __restrict__ out_err, uint32_t* __restrict__ max_contrib,
-    float4* __restrict__ ckpt, const uint32_t* __restrict__ zinfo) {
+    float4* __restrict__ ckpt, const uint32_t* __restrict__ zinfo, uint4* __restrict__ zero_p,
+    uint32_t zero_n16) {
+    // Blocks past the tiles clear zero_n16 x 16 bytes at zero_p (the backward's gradient
+    // accumulators), kFwdZero16 per thread; they are dispatched after the tiles and fill the
+    // SIMDs that the forward's last, longest tiles leave idle.
+    if ((int)blockIdx.x >= v.ntx * v.nty) {
+        const size_t i0 = (size_t)(blockIdx.x - v.ntx * v.nty) * 256 * kFwdZero16 + threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < kFwdZero16; ++k) {
+            const size_t i = i0 + (size_t)k * 256;
+            if (i < zero_n16) zero_p[i] = make_uint4(0u, 0u, 0u, 0u);
+        }
+        return;
+    }
     struct Rec3 {
         float4 r0, r1, r2;
     };
@@ -713,15 +728,20 @@ hipError_t launch_forward(const View& v, const uint32_t* ranges, const uint32_t*
                           const Rec* rec, const float* obs_rgb, const BlendParams& bp, float* rgb,
                           float* depth, float* final_T, uint32_t* n_contrib, float* color_err,
                           uint32_t* max_contrib, float4* ckpt, const uint32_t* zinfo, int wave_rows,
-                          hipStream_t s) {
+                          hipStream_t s, void* zero_p, size_t zero_bytes) {
+    // zero_bytes rounded up to 16: the caller's allocation covers it
+    const size_t n16 = (zero_bytes + 15) / 16;
+    if (n16 > 0xffffffffu) return hipErrorInvalidValue;
+    const unsigned zblocks = (unsigned)((n16 + 256 * kFwdZero16 - 1) / (256 * kFwdZero16));
+    const dim3 grid(v.ntx * v.nty + zblocks);
     if (wave_rows)   // four 16x4 row bands per tile
-        hipLaunchKernelGGL(k_forward<16>, dim3(v.ntx * v.nty), dim3(256), 0, s, v, ranges, idx,
-                           rec, obs_rgb, bp, rgb, depth, final_T, n_contrib, color_err,
-                           max_contrib, ckpt, zinfo);
+        hipLaunchKernelGGL(k_forward<16>, grid, dim3(256), 0, s, v, ranges, idx, rec, obs_rgb, bp,
+                           rgb, depth, final_T, n_contrib, color_err, max_contrib, ckpt, zinfo,
+                           static_cast<uint4*>(zero_p), (uint32_t)n16);
     else             // 2x2 quadrants of 8x8
-        hipLaunchKernelGGL(k_forward<8>, dim3(v.ntx * v.nty), dim3(256), 0, s, v, ranges, idx,
-                           rec, obs_rgb, bp, rgb, depth, final_T, n_contrib, color_err,
-                           max_contrib, ckpt, zinfo);
+        hipLaunchKernelGGL(k_forward<8>, grid, dim3(256), 0, s, v, ranges, idx, rec, obs_rgb, bp,
+                           rgb, depth, final_T, n_contrib, color_err, max_contrib, ckpt, zinfo,
+                           static_cast<uint4*>(zero_p), (uint32_t)n16);
     return hipGetLastError();
 }
 
