@@ -10,6 +10,8 @@
 // waves, written once per tile (no atomics, deterministic). k_pose_reduce sums the tile partials
 // in FP64 in tile order. The reference's divergent barriers inside `if (inside)` (Q12) are
 // replaced by a wave-uniform chunk loop in which every pixel sees the whole list.
+#include <type_traits>
+
 #include "fvgs_internal.h"
 #include "gn_step.h"
 #include "blend_common.h"
@@ -144,6 +146,14 @@ __device__ __forceinline__ void raw_to_pose(const double* raw, const float Rf[3]
 // The tile's partial is a wave butterfly of the raw FP64 sums plus a fixed-order sum of the four
 // waves, transformed with R and written once per tile (no atomics, deterministic).
 // Render-only mode (obs_depth == nullptr): pass 1 for every in-bounds pixel, no Jacobians.
+// Staged entry of pass 1 (screen record without the covariance) and its pass-2 form (Rec, the
+// ray of the mean in r3.xy).
+struct RecP1 {
+    float4 r0, r1, r2;
+};
+__device__ __forceinline__ float2 mean_ray(const Rec& r) { return make_float2(r.r3.x, r.r3.y); }
+__device__ __forceinline__ float2 mean_ray(const RecP1&) { return make_float2(0.f, 0.f); }
+
 template <int kPass>
 __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ vd,
                                               const uint32_t* __restrict__ ranges,
@@ -162,7 +172,9 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
     // kPass 1: pass 1 only; 2: Jacobians with FP64 running sums; 3: Jacobians with Kahan-
     // compensated FP32 running sums and a compensated FP32 block reduction (precision form).
     constexpr bool kKahan = kPass == 3;
-    __shared__ Rec wrec[4][64];                     // a wave's compacted entries of one chunk
+    // a wave's compacted entries of one chunk (pass 2 adds the ray of the mean)
+    using WRec = typename std::conditional<kPass == 1, RecP1, Rec>::type;
+    __shared__ WRec wrec[4][64];
     __shared__ __align__(4) uint8_t wpos[4][64];    // their offsets in the chunk
     __shared__ double red[4][27];
     const int tile = blockIdx.x, tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -197,12 +209,13 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
         const uint64_t bal = __ballot(bit);
         if (bit) {
             const int q = __popcll(bal & ((1ull << lane) - 1ull));
-            Rec r = pr;
-            if (pass == 2) {   // the ray of the mean, once per entry instead of per pixel
-                r.r3.x = (pr.r0.x - v.cx) / v.fx;
-                r.r3.y = (pr.r0.y - v.cy) / v.fy;
+            wrec[w][q].r0 = pr.r0;
+            wrec[w][q].r1 = pr.r1;
+            wrec[w][q].r2 = pr.r2;
+            if constexpr (kPass != 1) {   // the ray of the mean, once per entry, not per pixel
+                wrec[w][q].r3 = make_float4((pr.r0.x - v.cx) / v.fx, (pr.r0.y - v.cy) / v.fy,
+                                            0.f, 0.f);
             }
-            wrec[w][q] = r;
             wpos[w][q] = (uint8_t)lane;
         }
         if (e + 64 < n) {
@@ -257,9 +270,9 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
                     if (a < kAlphaMin || q <= 0.0f) continue;
                     const float tT = T * (1.0f - a);
                     if (tT < 0.0001f) { done = true; break; }
-                    C0 += r2.x * a * T; C1 += r2.y * a * T; C2 += r2.z * a * T;
-                    const float d = r1.y + dx * r1.z + dy * r1.w;
-                    if (T > 0.5f && tT < 0.5f) D = d;
+                    const float aT = a * T;
+                    C0 += r2.x * aT; C1 += r2.y * aT; C2 += r2.z * aT;
+                    if (T > 0.5f && tT < 0.5f) D = r1.y + dx * r1.z + dy * r1.w;
                     T = tT;
                     last = base + (uint32_t)ii[u] + 1;
                 }
@@ -357,7 +370,7 @@ __global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ 
             for (int k = 0; k < wc; ++k) {
                 if (base + (uint32_t)wpos[w][k] >= n_contrib) break;
                 const float4 r0 = wrec[w][k].r0, r1 = wrec[w][k].r1, r2 = wrec[w][k].r2;
-                const float2 rg = make_float2(wrec[w][k].r3.x, wrec[w][k].r3.y);   // ray of the mean
+                const float2 rg = mean_ray(wrec[w][k]);
                 const float dx = r0.x - px, dy = r0.y - py;
                 const float q = r0.z * dx * dx + 2.0f * r0.w * dx * dy + r1.x * dy * dy;
                 const float a = fminf(0.99f, r2.w * __expf(-0.5f * q));   // Q8: no q <= 0 test
