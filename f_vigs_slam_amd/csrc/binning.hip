@@ -389,7 +389,8 @@ __global__ __launch_bounds__(1024) void k_colscan_tiles(uint32_t* __restrict__ t
                                                        uint32_t* __restrict__ ranges,
                                                        uint32_t* __restrict__ d_m,
                                                        uint32_t* __restrict__ zinfo,
-                                                       uint32_t cap) {
+                                                       uint32_t cap,
+                                                       uint32_t* __restrict__ m_host) {
     extern __shared__ uint32_t tot[];   // cols entries, then 1024 scratch
     uint32_t* part = tot + cols;
     const int tid = threadIdx.x;
@@ -431,7 +432,10 @@ __global__ __launch_bounds__(1024) void k_colscan_tiles(uint32_t* __restrict__ t
         tot_base[i] = run;
         run += x;
     }
-    if (tid == 1023) *d_m = part[1023];
+    if (tid == 1023) {
+        *d_m = part[1023];
+        if (m_host) *m_host = part[1023];   // mapped host memory: read after the stream syncs
+    }
 }
 
 __global__ __launch_bounds__(kBinThreads) void k_emit(
@@ -1164,7 +1168,7 @@ hipError_t launch_dequantize(const MapRef& map, int n, float* out14, hipStream_t
 
 hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* col_base, int nblk, int ntiles,
                           uint32_t* ranges, uint32_t* d_m, uint32_t* zinfo, uint32_t cap,
-                          hipStream_t s) {
+                          hipStream_t s, uint32_t* m_host) {
     const int cols = ntiles + 1;   // zero segment + tiles
     hipLaunchKernelGGL(k_colscan_rows, dim3((cols + 63) / 64), dim3(1024), 0, s, blk_counts,
                        col_base, nblk, cols);
@@ -1172,7 +1176,7 @@ hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* col_base, int nblk, in
     if (e != hipSuccess) return e;
     const size_t lds = ((size_t)cols + 1024) * 4;
     hipLaunchKernelGGL(k_colscan_tiles, dim3(1), dim3(1024), lds, s, col_base, cols, ranges, d_m,
-                       zinfo, cap);
+                       zinfo, cap, m_host);
     return hipGetLastError();
 }
 

@@ -85,6 +85,7 @@ struct fvgs_ctx {
     DevBuf d2, d3, ckpt;
     DevBuf pose_tiles, pose_sum, t_rgb, t_depth, pose_pix;
     uint32_t* h_m = nullptr;      // pinned
+    uint32_t* h_m_dev = nullptr;  // its device address: the column scan writes M there
     double* h_sum = nullptr;      // pinned, 27
     // device-resident tracking loop: [View | pose7 | fail | status[iters] | M[iters]]
     DevBuf track;
@@ -278,8 +279,8 @@ int prepare(fvgs_ctx* c, const fvgs_pose* pose, const fvgs_intrinsics* intr, int
                              as<uint2>(c->rect), as<uint32_t>(c->blk_counts), zinfo, s));
     HIPCHK(c, launch_colscan(as<uint32_t>(c->blk_counts), as<uint32_t>(c->col_base), nblk, T,
                              as<uint32_t>(c->ranges), as<uint32_t>(c->d_m), zinfo,
-                             async ? c->inst_cap : 0xffffffffu, s));
-    HIPCHK(c, hipMemcpyAsync(c->h_m, c->d_m.p, 4, hipMemcpyDeviceToHost, s));
+                             async ? c->inst_cap : 0xffffffffu, s, c->h_m_dev));
+    if (!c->h_m_dev) HIPCHK(c, hipMemcpyAsync(c->h_m, c->d_m.p, 4, hipMemcpyDeviceToHost, s));
     if (!async) {
         HIPCHK(c, hipStreamSynchronize(s));
         c->m = *c->h_m;
@@ -531,6 +532,10 @@ int fvgs_create(const fvgs_params* params, fvgs_ctx** out) {
         hipHostMalloc((void**)&c->h_sum, 27 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         fvgs_destroy(c);
         return FVGS_E_HIP;
+    }
+    if (hipHostGetDevicePointer((void**)&c->h_m_dev, c->h_m, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        c->h_m_dev = nullptr;   // M then comes back by a copy
     }
     for (auto& e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) {

@@ -115,7 +115,7 @@ hipError_t launch_dequantize(const MapRef& map, int n, float* out14, hipStream_t
 // tile t. Writes the tile ranges, *d_m = M and zinfo[0] = zero-segment length.
 hipError_t launch_colscan(uint32_t* blk_counts, uint32_t* col_base, int nblk, int ntiles,
                           uint32_t* ranges, uint32_t* d_m, uint32_t* zinfo, uint32_t cap,
-                          hipStream_t s);
+                          hipStream_t s, uint32_t* m_host = nullptr);
 hipError_t launch_emit(const Rec* rec, const uint2* rect, const uint32_t* blk_counts,
                        const uint32_t* col_base, int n, const View& v, const View* vd,
                        uint64_t* pairs, uint64_t cap, hipStream_t s);
