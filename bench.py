@@ -157,7 +157,10 @@ def tracking(args, log_fn):
                 p99_ms=round(float(np.percentile(st, 99)), 4),
                 mean_ms=round(float(st.mean()), 4),
                 final_rot_err_rad=float(max(e[0] for e in errs)),
-                final_trans_err_m=float(max(e[1] for e in errs)))
+                final_trans_err_m=float(max(e[1] for e in errs)),
+                note="latency of the transcribed GN step; on this random-cloud target the "
+                     "step from a 2 deg / 3 cm start is ~1e-7 on the GPU and the oracle alike "
+                     "(pass-2 terms dominate JtJ), so the final error stays near the start")
 
 
 def served(args, log_fn):

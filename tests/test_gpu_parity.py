@@ -149,9 +149,8 @@ def test_render_parity(ctx):
     assert frac >= 0.999
 
 
-@pytest.fixture(scope="module")
-def mapping_case():
-    scene, intr, pose = workload(50000, seed=0)
+def _mapping_case(n, seed):
+    scene, intr, pose = workload(n, seed=seed)
     pf, b, rec = oracle_view(scene, pose, intr, W, H)
     obs_rgb, obs_depth = mapping_observations(scene, b, rec, W, H)
     fwd = O.forward(W, H, b, rec, scene["color"], scene["opacity"], obs_rgb)
@@ -164,6 +163,11 @@ def mapping_case():
     d3_32 = O.delta3d(scene["pos"], scene["scale"], scene["quat"], cp, cq, intr, g32)
     return dict(scene=scene, intr=intr, pose=pose, obs_rgb=obs_rgb, obs_depth=obs_depth,
                 fwd=fwd, d2=d2, d3=d3, b=b, g32=g32, d3_32=d3_32)
+
+
+@pytest.fixture(scope="module")
+def mapping_case():
+    return _mapping_case(50000, seed=0)
 
 
 def _check_gradient_field(tag, name, got, ref, ref32):
@@ -180,8 +184,7 @@ def _check_gradient_field(tag, name, got, ref, ref32):
     assert l2 <= max(1e-3, 2.0 * l2_32)
 
 
-def test_forward_parity(ctx, mapping_case):
-    c = mapping_case
+def _check_forward(ctx, c):
     ctx.set_scene(c["scene"])
     ctx.set_frame(c["obs_rgb"], c["obs_depth"])
     ctx.prepare_rasterization(c["pose"], c["intr"], W, H)
@@ -199,8 +202,11 @@ def test_forward_parity(ctx, mapping_case):
         assert frac >= 0.999
 
 
-def test_backward_parity(ctx, mapping_case):
-    c = mapping_case
+def test_forward_parity(ctx, mapping_case):
+    _check_forward(ctx, mapping_case)
+
+
+def _check_backward(ctx, c):
     ctx.set_scene(c["scene"])
     ctx.set_frame(c["obs_rgb"], c["obs_depth"])
     ctx.prepare_rasterization(c["pose"], c["intr"], W, H)
@@ -215,6 +221,10 @@ def test_backward_parity(ctx, mapping_case):
     for f, (a, z) in cols.items():
         _check_gradient_field("backward", f, g[f].reshape(len(g), -1), r[:, a:z],
                               c["g32"][:, a:z])
+
+
+def test_backward_parity(ctx, mapping_case):
+    _check_backward(ctx, mapping_case)
 
 
 def test_instance_capacity_growth_reruns_the_view(mapping_case):
@@ -243,8 +253,7 @@ def test_instance_capacity_growth_reruns_the_view(mapping_case):
     grow.close()
 
 
-def test_gradients_3d_parity(ctx, mapping_case):
-    c = mapping_case
+def _check_gradients_3d(ctx, c):
     ctx.set_scene(c["scene"])
     ctx.set_frame(c["obs_rgb"], c["obs_depth"])
     ctx.mapping_step(c["pose"], c["intr"])
@@ -256,10 +265,12 @@ def test_gradients_3d_parity(ctx, mapping_case):
                               c["d3_32"][:, a:z])
 
 
-@pytest.fixture(scope="module")
-def track_case():
-    """Config-1 style tracking case at reduced N: target = pass-1 self-render at identity."""
-    n = 50000
+def test_gradients_3d_parity(ctx, mapping_case):
+    _check_gradients_3d(ctx, mapping_case)
+
+
+def _track_case(n):
+    """Tracking case: target = pass-1 self-render at identity, start 2 deg / 3 cm away."""
     scene, intr, gt = workload(n, seed=21)
     pf, b, rec = oracle_view(scene, gt, intr, W, H)
     ones = np.ones((H, W), np.float32)
@@ -272,6 +283,12 @@ def track_case():
     start = S.perturbed_pose(21, rot_deg=2.0, trans_m=0.03)
     return dict(scene=scene, intr=intr, gt=gt, obs_rgb=obs_rgb, obs_depth=obs_depth, gx=gx, gy=gy,
                 start=start)
+
+
+@pytest.fixture(scope="module")
+def track_case():
+    """The 500k-Gaussian tracking configuration at reduced N (50k)."""
+    return _track_case(50000)
 
 
 def _oracle_jacobians(c, pose7, depth_weight=1.0):
@@ -293,7 +310,10 @@ def test_pose_jacobians_parity(ctx, track_case):
     """The pass-2 terms make JtJ heavy-tailed (single pixel-entries carry 1/(1 - alpha) factors),
     so FP32-vs-FP64 threshold flips move individual entries by a few percent; the quantity the
     tracker consumes, the Gauss-Newton step JtJ^-1 Jtr, is compared at 1e-6."""
-    c = track_case
+    _check_pose_jacobians(ctx, track_case)
+
+
+def _check_pose_jacobians(ctx, c):
     ctx.set_scene(c["scene"])
     ctx.set_frame(c["obs_rgb"], c["obs_depth"], c["gx"], c["gy"])
     for pose in (c["gt"], c["start"]):
@@ -388,3 +408,37 @@ def test_error_paths(ctx):
         c3.prepare_rasterization(pose, intr, W, H)
     assert e.value.code == -3
     c3.close()
+
+
+# ---- full configuration sizes -----------------------------------------------------------------
+# The same bars at the sizes the benchmark runs: the 1M-Gaussian 640x480 mapping view (5.5M tile
+# instances; lists of 2.4k-5.1k entries) and the 500k-Gaussian tracking view. The FP64
+# oracle takes a few seconds per view on the host cores.
+
+@pytest.fixture(scope="module")
+def mapping_case_1m():
+    return _mapping_case(1_000_000, seed=0)
+
+
+def test_fullsize_binning_bitexact_1m(ctx, mapping_case_1m):
+    c = mapping_case_1m
+    _check_binning(ctx, c["scene"], c["pose"], c["intr"], W, H)
+    lens = c["b"]["ranges"][:, 1] - c["b"]["ranges"][:, 0]
+    print(f"[fullsize] M {c['b']['n_instances']}, list lengths {lens.min()}..{lens.max()}",
+          flush=True)
+
+
+def test_fullsize_forward_parity_1m(ctx, mapping_case_1m):
+    _check_forward(ctx, mapping_case_1m)
+
+
+def test_fullsize_backward_parity_1m(ctx, mapping_case_1m):
+    _check_backward(ctx, mapping_case_1m)
+
+
+def test_fullsize_gradients_3d_parity_1m(ctx, mapping_case_1m):
+    _check_gradients_3d(ctx, mapping_case_1m)
+
+
+def test_fullsize_pose_jacobians_500k(ctx):
+    _check_pose_jacobians(ctx, _track_case(500_000))
