@@ -638,7 +638,7 @@ def main():
         ctx.prepare_rasterization(pose, intr, w, h)
         for _ in range(args.steps):
             ctx.forward()
-    log(f"step {t * 1e3:.3f} ms; stages binning {stage[0]:.3f} clear {stage[1]:.3f} "
+    log(f"step {t * 1e3:.3f} ms; stages binning {stage[0]:.3f} gap {stage[1]:.3f} "
         f"forward {stage[2]:.3f} backward {stage[3]:.3f} 3d {stage[4]:.3f} ms; M={M} K={K}")
     # Algorithmic bytes (compulsory traffic, each counted once), K = walked entries, P = pixels:
     #   k_forward:  52 B per walked entry (index 4 + 48 B of record) + 48 B per pixel
@@ -673,7 +673,7 @@ def main():
                                "backward, 2D->3D gradients",
                    "n_gaussians": n, "width": w, "height": h, "instances_M": M,
                    "walked_K": K, "pixels_P": P,
-                   "stage_ms": {"binning": round(stage[0], 4), "clear": round(stage[1], 4),
+                   "stage_ms": {"binning": round(stage[0], 4), "gap": round(stage[1], 4),
                                 "forward": round(stage[2], 4), "backward": round(stage[3], 4),
                                 "grad3d": round(stage[4], 4)},
                    "step_algorithmic_GBs": round(step_bytes / t / 1e9, 2)},

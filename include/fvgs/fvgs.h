@@ -351,9 +351,9 @@ int fvgs_stats(fvgs_ctx* ctx, uint64_t* m, uint64_t* k, uint32_t* tiles, uint32_
 
 /* Device time of the stages of the last fvgs_mapping_step in milliseconds, measured with HIP
  * events on the context's stream: [0] projection + binning (including the host read of M),
- * [1] clearing the 2D gradient buffer, [2] the blend forward, [3] the blend backward, [4] the
- * 2D->3D gradients. [0] is also set by fvgs_prepare_rasterization. Enable with
- * fvgs_set_timing. */
+ * [1] the interval between the binning and the forward (no work: the forward launch itself
+ * clears the 2D gradient buffer), [2] the blend forward, [3] the blend backward, [4] the 2D->3D
+ * gradients. [0] is also set by fvgs_prepare_rasterization. Enable with fvgs_set_timing. */
 #define FVGS_TIMING_STAGES 5
 int fvgs_set_timing(fvgs_ctx* ctx, int enable);
 int fvgs_last_timing(fvgs_ctx* ctx, float ms[FVGS_TIMING_STAGES]);
