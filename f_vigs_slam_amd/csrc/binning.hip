@@ -1198,8 +1198,9 @@ hipError_t launch_tile_sort(const uint32_t* ranges, int ntiles, int n_gauss, uin
     // Lists up to kMedE * 512 keys: 512 threads (50 KB LDS, three per CU); longer: 1024 threads
     // with 6-byte entries, kLargeE * 1024 resident (77 KB, two per CU), streamed through HBM beyond
     // that. Each launch covers every tile and skips the other lengths (a launch whose tiles all
-    // skip still costs ~4 us, so short lists share the 512-thread launch). (6-byte entries for the 512-thread lists fit four per CU but measured slower:
-    // the index gathers from the list cost more than the extra workgroup gains.)
+    // skip still costs ~4 us, so short lists share the 512-thread launch). 6-byte entries for the
+    // 512-thread lists fit four per CU but measured slower: the index gathers from the list cost
+    // more than the extra workgroup gains.
     using Medium = SortCfg<512, kSortMedE>;
     using Large = SortKpCfg<1024, kSortLargeE>;
     static const bool attrs = [] {
