@@ -592,8 +592,8 @@ def window_optimize(args, log_fn, scene, poses, views):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
-              "(the hot path runs on one MI355X; see the sharding notes in DESIGN.md).",
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the hot path "
+              "runs on one MI355X; a view's tiles would shard across GPUs, which is not built).",
               file=sys.stderr)
         sys.exit(2)
     from f_vigs_slam_amd import Context
