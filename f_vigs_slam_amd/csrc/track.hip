@@ -154,8 +154,11 @@ struct RecP1 {
 __device__ __forceinline__ float2 mean_ray(const Rec& r) { return make_float2(r.r3.x, r.r3.y); }
 __device__ __forceinline__ float2 mean_ray(const RecP1&) { return make_float2(0.f, 0.f); }
 
+// Pass 2 at five waves per SIMD (at most 96 VGPRs): a 1200-tile view's 4800 waves are then all
+// resident at once instead of leaving a partial second round.
 template <int kPass>
-__global__ __launch_bounds__(256) void k_pose(View vh, const View* __restrict__ vd,
+__global__ __launch_bounds__(256, kPass == 2 ? 5 : 1) void k_pose(View vh,
+                                                                  const View* __restrict__ vd,
                                               const uint32_t* __restrict__ ranges,
                                               const uint32_t* __restrict__ idx,
                                               const Rec* __restrict__ rec,
